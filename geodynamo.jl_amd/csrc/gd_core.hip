@@ -1,0 +1,278 @@
+// gd_core.hip -- context, error state, index maps, FFT plans and the host FP64
+// radial set-up (radial grid, FD weights, Laplacian).
+#include <cmath>
+#include <cstdarg>
+
+#include "gd_internal.hpp"
+
+static thread_local char g_err[1024] = "";
+
+void gd_set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+extern "C" const char *gd_last_error(void) { return g_err; }
+extern "C" const char *gd_version(void) { return "gdhip 0.1 (gfx950)"; }
+
+int gd_check_launch(const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        gd_set_error("launch %s: %s", what, hipGetErrorString(e));
+        return GD_ERR_HIP;
+    }
+    return GD_OK;
+}
+
+int DevBuf::ensure(size_t nbytes)
+{
+    if (nbytes <= bytes && p) return GD_OK;
+    release();
+    if (nbytes == 0) return GD_OK;
+    hipError_t e = hipMalloc(&p, nbytes);
+    if (e != hipSuccess) {
+        p = nullptr;
+        gd_set_error("hipMalloc(%zu): %s", nbytes, hipGetErrorString(e));
+        return GD_ERR_NOMEM;
+    }
+    bytes = nbytes;
+    return GD_OK;
+}
+
+void DevBuf::release()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+int gd_to_device(gd_ctx *ctx, DevBuf &buf, const void *host, size_t bytes)
+{
+    GD_TRY(buf.ensure(bytes));
+    if (bytes) GD_HIP(hipMemcpyAsync(buf.p, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return GD_OK;
+}
+
+int gd_from_device(gd_ctx *ctx, void *host, const void *dev, size_t bytes)
+{
+    if (bytes) GD_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    GD_HIP(hipStreamSynchronize(ctx->stream));
+    return GD_OK;
+}
+
+// ---------------------------------------------------------------- context
+extern "C" int gd_ctx_create(int device, gd_ctx **out)
+{
+    GD_CHECK(out, GD_ERR_ARG, "gd_ctx_create: null out");
+    int n = 0;
+    GD_HIP(hipGetDeviceCount(&n));
+    GD_CHECK(device >= 0 && device < n, GD_ERR_ARG, "gd_ctx_create: device %d of %d", device, n);
+    GD_HIP(hipSetDevice(device));
+    gd_ctx *c = new gd_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        gd_set_error("hipStreamCreate: %s", hipGetErrorString(e));
+        return GD_ERR_HIP;
+    }
+    c->stream = c->own_stream;
+    *out = c;
+    return GD_OK;
+}
+
+extern "C" int gd_ctx_set_stream(gd_ctx *ctx, void *s)
+{
+    GD_CHECK(ctx, GD_ERR_ARG, "null ctx");
+    ctx->stream = s ? (hipStream_t)s : ctx->own_stream;
+    return GD_OK;
+}
+
+extern "C" int gd_ctx_synchronize(gd_ctx *ctx)
+{
+    GD_CHECK(ctx, GD_ERR_ARG, "null ctx");
+    GD_HIP(hipSetDevice(ctx->device));
+    GD_HIP(hipStreamSynchronize(ctx->stream));
+    return GD_OK;
+}
+
+extern "C" int gd_ctx_destroy(gd_ctx *ctx)
+{
+    if (!ctx) return GD_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->own_stream) {
+        (void)hipStreamSynchronize(ctx->own_stream);
+        (void)hipStreamDestroy(ctx->own_stream);
+    }
+    delete ctx;
+    return GD_OK;
+}
+
+// ---------------------------------------------------------------- lm maps
+int gd_nlm_of(int lmax, int mmax)
+{
+    int n = 0;
+    for (int l = 0; l <= lmax; ++l) n += (l < mmax ? l : mmax) + 1;
+    return n;
+}
+
+void LmMaps::build(int lmax_, int mmax_)
+{
+    lmax = lmax_;
+    mmax = mmax_;
+    nlm = gd_nlm_of(lmax, mmax);
+    moff.assign(mmax + 2, 0);
+    ne.assign(mmax + 1, 0);
+    lmaj_of_idx.assign(nlm, 0);
+    idx_of_lmaj.assign(nlm, 0);
+    l_of_idx.assign(nlm, 0);
+    m_of_idx.assign(nlm, 0);
+    // l-major offsets (shtnskit_transforms.jl:177-188)
+    std::vector<int> loff(lmax + 2, 0);
+    for (int l = 0; l <= lmax; ++l) loff[l + 1] = loff[l] + (l < mmax ? l : mmax) + 1;
+    int idx = 0;
+    for (int m = 0; m <= mmax; ++m) {
+        moff[m] = idx;
+        int K = lmax - m + 1;
+        ne[m] = (K + 1) / 2;
+        for (int par = 0; par < 2; ++par)
+            for (int l = m + par; l <= lmax; l += 2) {
+                int lm = loff[l] + m;
+                lmaj_of_idx[idx] = lm;
+                idx_of_lmaj[lm] = idx;
+                l_of_idx[idx] = l;
+                m_of_idx[idx] = m;
+                ++idx;
+            }
+    }
+    moff[mmax + 1] = idx;
+}
+
+// ---------------------------------------------------------------- FFT plan
+int FftPlan::build(int n_)
+{
+    n = n_;
+    nstages = 0;
+    int rem = n;
+    // prefer radix 8, 4, then 6, 3, 2
+    static const int order[] = {8, 4, 6, 3, 2};
+    while (rem > 1) {
+        bool found = false;
+        for (int r : order) {
+            if (rem % r == 0) {
+                // keep a factor 3 available: use 8/4 only while a power of two remains
+                if (nstages >= 12) return GD_ERR_ARG;
+                radix[nstages++] = r;
+                rem /= r;
+                found = true;
+                break;
+            }
+        }
+        if (!found) {
+            gd_set_error("nlon=%d must factor into 2 and 3", n);
+            return GD_ERR_ARG;
+        }
+    }
+    return GD_OK;
+}
+
+// ---------------------------------------------------------------- radial host set-up
+// fields.jl:108-131
+extern "C" int gd_radial_grid(int nr, double rratio, double *r)
+{
+    GD_CHECK(nr >= 2 && r && rratio > 0.0 && rratio < 1.0, GD_ERR_ARG, "gd_radial_grid: bad args");
+    double ri = rratio / (1.0 - rratio);
+    for (int n = 0; n < nr; ++n)
+        r[n + nr * 3] = 0.5 * (1.0 + std::cos(M_PI * (double)(nr - (n + 1)) / (double)(nr - 1))) + ri;
+    for (int p = 0; p < 7; ++p) {
+        if (p == 3) continue;
+        for (int n = 0; n < nr; ++n) r[n + nr * p] = std::pow(r[n + nr * 3], (double)(p - 3));
+    }
+    return GD_OK;
+}
+
+static int small_solve(int s, double *A, double *b)
+{
+    for (int k = 0; k < s; ++k) {
+        int p = k;
+        for (int i = k + 1; i < s; ++i)
+            if (std::fabs(A[i * s + k]) > std::fabs(A[p * s + k])) p = i;
+        if (A[p * s + k] == 0.0) return -1;
+        if (p != k) {
+            for (int j = 0; j < s; ++j) std::swap(A[k * s + j], A[p * s + j]);
+            std::swap(b[k], b[p]);
+        }
+        for (int i = k + 1; i < s; ++i) {
+            double f = A[i * s + k] / A[k * s + k];
+            for (int j = k; j < s; ++j) A[i * s + j] -= f * A[k * s + j];
+            b[i] -= f * b[k];
+        }
+    }
+    for (int i = s - 1; i >= 0; --i) {
+        double acc = b[i];
+        for (int j = i + 1; j < s; ++j) acc -= A[i * s + j] * b[j];
+        b[i] = acc / A[i * s + i];
+    }
+    return 0;
+}
+
+// linear_algebra.jl:90-132: weights on the local stencil max(1,n-bw):min(N,n+bw)
+extern "C" int gd_derivative_matrix(int order, int nr, int bw, const double *r, int fd_mode,
+                                    double *band)
+{
+    GD_CHECK(nr >= 2 && bw >= 1 && bw <= 7 && r && band && order >= 0, GD_ERR_ARG,
+             "gd_derivative_matrix: bad args");
+    const double *r4 = r + nr * 3;
+    std::memset(band, 0, sizeof(double) * (size_t)(2 * bw + 1) * nr);
+    double V[256], VT[256], rhs[16];
+    for (int n = 0; n < nr; ++n) {
+        int left = n - bw < 0 ? 0 : n - bw;
+        int right = n + bw > nr - 1 ? nr - 1 : n + bw;
+        int s = right - left + 1;
+        for (int i = 0; i < s; ++i) {
+            V[i * s] = 1.0;
+            for (int j = 1; j < s; ++j) V[i * s + j] = V[i * s + j - 1] * (r4[left + i] - r4[n]);
+        }
+        for (int i = 0; i < s; ++i) rhs[i] = 0.0;
+        if (order <= s - 1) {
+            double f = 1.0;
+            for (int k = 2; k <= order; ++k) f *= k;
+            rhs[order] = f;
+        }
+        int rc;
+        if (fd_mode == GD_FD_TRANSPOSE) {
+            for (int i = 0; i < s; ++i)
+                for (int j = 0; j < s; ++j) VT[i * s + j] = V[j * s + i];
+            rc = small_solve(s, VT, rhs);
+        } else {
+            rc = small_solve(s, V, rhs);
+        }
+        GD_CHECK(rc == 0, GD_ERR_NUMERIC, "gd_derivative_matrix: singular stencil at n=%d", n);
+        for (int i = 0; i < s; ++i) {
+            int idx = left + i, row = bw + n - idx;
+            if (row >= 0 && row <= 2 * bw) band[row + (2 * bw + 1) * idx] = rhs[i];
+        }
+    }
+    return GD_OK;
+}
+
+// linear_algebra.jl:134-151
+extern "C" int gd_radial_laplacian(int nr, int bw, const double *r, int fd_mode, double *band)
+{
+    std::vector<double> d1((size_t)(2 * bw + 1) * nr);
+    GD_TRY(gd_derivative_matrix(2, nr, bw, r, fd_mode, band));
+    GD_TRY(gd_derivative_matrix(1, nr, bw, r, fd_mode, d1.data()));
+    for (int n = 0; n < nr; ++n) {
+        double rinv = r[n + nr * 2];
+        int j0 = n - bw < 0 ? 0 : n - bw, j1 = n + bw > nr - 1 ? nr - 1 : n + bw;
+        for (int j = j0; j <= j1; ++j) {
+            int row = bw + n - j;
+            band[row + (2 * bw + 1) * j] += 2.0 * rinv * d1[row + (2 * bw + 1) * j];
+        }
+    }
+    return GD_OK;
+}

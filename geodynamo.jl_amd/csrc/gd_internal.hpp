@@ -1,0 +1,102 @@
+// gd_internal.hpp -- shared internals of libgdhip (MI355X / gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gdhip.h"
+
+// ---------------------------------------------------------------- errors
+void gd_set_error(const char *fmt, ...);
+
+#define GD_HIP(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            gd_set_error("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+            return GD_ERR_HIP;                                                            \
+        }                                                                                 \
+    } while (0)
+
+#define GD_CHECK(cond, code, ...)        \
+    do {                                 \
+        if (!(cond)) {                   \
+            gd_set_error(__VA_ARGS__);   \
+            return (code);               \
+        }                                \
+    } while (0)
+
+#define GD_TRY(call)                \
+    do {                            \
+        int rc_ = (call);           \
+        if (rc_ != GD_OK) return rc_; \
+    } while (0)
+
+// ---------------------------------------------------------------- context
+struct gd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+};
+
+// device buffer helper (raw, explicit lifetime)
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t nbytes);
+    void release();
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// host<->device staging for GD_HOST calls
+int gd_to_device(gd_ctx *ctx, DevBuf &buf, const void *host, size_t bytes);
+int gd_from_device(gd_ctx *ctx, void *host, const void *dev, size_t bytes);
+
+// ---------------------------------------------------------------- lm maps
+// m-major "parity-grouped" spectral index used on the device:
+//   for m = 0..mmax the block [moff[m], moff[m+1]) holds l = m..lmax, first the
+//   ne[m] entries with (l-m) even in increasing l, then the odd ones.
+struct LmMaps {
+    int lmax = 0, mmax = 0, nlm = 0;
+    std::vector<int> moff, ne;          // mmax+2, mmax+1
+    std::vector<int> lmaj_of_idx;       // device idx -> l-major lm
+    std::vector<int> idx_of_lmaj;       // l-major lm -> device idx
+    std::vector<int> l_of_idx, m_of_idx;
+    void build(int lmax_, int mmax_);
+};
+int gd_nlm_of(int lmax, int mmax);
+
+// ---------------------------------------------------------------- vector types
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Acc4;
+template <> struct Acc4<float> { using type = floatx4; };
+template <> struct Acc4<double> { using type = doublex4; };
+
+// 16x16x4 MFMA: lane l holds A[l%16][l/16], B[l/16][l%16];
+// D[4*(l/16)+v][l%16] in accumulator element v.
+__device__ __forceinline__ floatx4 mfma16x16x4(float a, float b, floatx4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ doublex4 mfma16x16x4(double a, double b, doublex4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+template <typename T> struct Cplx { T re, im; };
+
+// ---------------------------------------------------------------- FFT plan
+struct FftPlan {
+    int n = 0;
+    int nstages = 0;
+    int radix[12] = {0};
+    int build(int n_);
+};
+
+// launch helper: returns GD_ERR_HIP on launch failure
+int gd_check_launch(const char *what);

@@ -1,0 +1,718 @@
+// gd_sht.hip -- spherical-harmonic transform kernels for gfx950.
+//
+//  plm_kernel        FP64 orthonormal P_lm, dP_lm/dtheta, m P_lm/sin(theta) by
+//                    the three-term recurrence with a carried base-2 exponent
+//  synth_kernel      per-m Legendre contraction on MFMA (16x16x4 f32 / f64):
+//                    Fourier(i, col) = sum_l Tab(l, i) a(l, col), split into
+//                    (l-m) even / odd partial sums over the northern nodes;
+//                    the epilogue forms north/south rings and, for vector
+//                    fields, v_theta = dP S + i Q T, v_phi = i Q S - dP T
+//  ana_kernel        the transpose contraction over the northern nodes with
+//                    G+- = Gn +- Gs, epilogue S = X_t - i Y_p, T = -i Y_t - X_p
+//  ring_kernel       longitude FFTs of a northern ring and its southern mirror
+//                    as one complex FFT (Stockham, radices 2,3,4,6,8, in LDS);
+//                    RING_STEP also evaluates the pointwise nonlinear products
+//                    between the inverse and forward FFTs, so physical-space
+//                    values never leave LDS.
+#include <cmath>
+
+#include "gd_sht.hpp"
+
+// ====================================================================== P_lm
+__global__ void plm_kernel(const double *__restrict__ x, const int *__restrict__ moff, int lmax, int mmax, int nlat2,
+                           int nlm, double *__restrict__ tab64, float *__restrict__ tab32)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blockIdx.y;
+    if (i >= nlat2 || m > mmax) return;
+    const double xi = x[i], s = sqrt(1.0 - xi * xi);
+    double lg = 0.5 * log(1.0 / (4.0 * M_PI)) + 0.5 * log(2.0 * m + 1.0) + m * log(s);
+    for (int k = 1; k <= m; ++k) lg += 0.5 * log((2.0 * k - 1.0) / (2.0 * k));
+    int e = (int)floor(lg / M_LN2);
+    double val = exp(lg - e * M_LN2);
+    if (m & 1) val = -val;
+    double pm2 = 0.0, pm1 = val, tprev = 0.0;
+    const int K = lmax - m + 1, ne = (K + 1) / 2;
+    for (int l = m; l <= lmax; ++l) {
+        if (l == m + 1) {
+            double cur = sqrt(2.0 * m + 3.0) * xi * pm1;
+            pm2 = pm1;
+            pm1 = cur;
+        } else if (l > m + 1) {
+            double a = sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
+            double b = sqrt((((double)l - 1) * ((double)l - 1) - (double)m * m) /
+                            (4.0 * ((double)l - 1) * ((double)l - 1) - 1.0));
+            double cur = a * (xi * pm1 - b * pm2);
+            pm2 = pm1;
+            pm1 = cur;
+        }
+        if (fabs(pm1) > 0x1p600) {
+            pm1 = ldexp(pm1, -600);
+            pm2 = ldexp(pm2, -600);
+            e += 600;
+        }
+        const double t = ldexp(pm1, e);
+        const double c = (l == m) ? 0.0 : sqrt((2.0 * l + 1.0) * ((double)l * l - (double)m * m) / (2.0 * l - 1.0));
+        const double dp = (l * xi * t - c * tprev) / s;
+        const double q = m * t / s;
+        tprev = t;
+        const int d = l - m;
+        const int idx = moff[m] + ((d & 1) ? ne + (d - 1) / 2 : d / 2);
+        const size_t o = (size_t)idx * nlat2 + i;
+        tab64[o] = t;
+        tab64[(size_t)nlm * nlat2 + o] = dp;
+        tab64[(size_t)2 * nlm * nlat2 + o] = q;
+        if (tab32) {
+            tab32[o] = (float)t;
+            tab32[(size_t)nlm * nlat2 + o] = (float)dp;
+            tab32[(size_t)2 * nlm * nlat2 + o] = (float)q;
+        }
+    }
+}
+
+int gd_launch_plm(hipStream_t st, const double *x, const int *moff, int lmax, int mmax, int nlat2, double *tab64,
+                  float *tab32, int nlm)
+{
+    hipLaunchKernelGGL(plm_kernel, dim3((nlat2 + 63) / 64, mmax + 1), dim3(64), 0, st, x, moff, lmax, mmax, nlat2, nlm,
+                       tab64, tab32);
+    return gd_check_launch("plm_kernel");
+}
+
+// ====================================================================== Legendre GEMMs
+namespace {
+constexpr int KT = 16; // K tile staged in LDS
+constexpr int TM = 32; // rows per tile (two 16-row MFMA blocks)
+
+template <typename T> using acc_t = typename Acc4<T>::type;
+
+template <typename T> __device__ __forceinline__ T shfl_xor3(T v) { return __shfl_xor(v, 3); }
+
+// One parity group of the synthesis contraction for a (m, node tile, level tile).
+template <typename T, int NT>
+__device__ __forceinline__ void synth_group(const SynthArgs &a, const T *__restrict__ in, const T *__restrict__ tab0,
+                                            const T *__restrict__ tab1, T *sA, T *sB, const int *scol, int kbase,
+                                            int Kp, int i0, int r0, int VC, int NTILE, acc_t<T> (&acc)[NT][2])
+{
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    for (int k0 = 0; k0 < Kp; k0 += KT) {
+        for (int e = tid; e < NT * KT * TM; e += nthr) {
+            const int i = e % TM, k = (e / TM) % KT, t = e / (TM * KT);
+            T v = T(0);
+            if (k0 + k < Kp && i0 + i < a.nlat2) {
+                const T *tb = (t == 0) ? tab0 : tab1;
+                v = tb[(size_t)(kbase + k0 + k) * a.nlat2 + i0 + i];
+            }
+            sA[(t * KT + k) * TM + i] = v;
+        }
+        for (int e = tid; e < KT * NTILE; e += nthr) {
+            const int k = e % KT, vv = e / KT;
+            const int lv = vv / VC, fv = vv - lv * VC;
+            const int c = scol[fv], r = r0 + lv;
+            T v = T(0);
+            if (c >= 0 && r < a.nr && k0 + k < Kp) v = in[(size_t)(r * a.cin + c) * a.nlm + kbase + k0 + k];
+            sB[k * NTILE + vv] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < KT; ks += 4) {
+            const T b = sB[(ks + (lane >> 4)) * NTILE + wave * 16 + (lane & 15)];
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb) {
+                    const T av = sA[(t * KT + ks + (lane >> 4)) * TM + rb * 16 + (lane & 15)];
+                    acc[t][rb] = mfma16x16x4(av, b, acc[t][rb]);
+                }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, bool VEC> __global__ void __launch_bounds__(320) synth_kernel(SynthArgs a)
+{
+    constexpr int NT = VEC ? 2 : 1;
+    extern __shared__ char smem_raw[];
+    const int VC = VEC ? 4 * a.nf : 2 * a.nf;
+    const int NTILE = 16 * a.nf;
+    const int LT = NTILE / VC;
+    T *sA = reinterpret_cast<T *>(smem_raw);
+    T *sB = sA + NT * KT * TM;
+    int *scol = reinterpret_cast<int *>(sB + KT * NTILE);
+    const int m = blockIdx.z, i0 = blockIdx.x * TM, r0 = blockIdx.y * LT;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    if (tid < 20) scol[tid] = a.colmap[tid];
+    __syncthreads();
+    const T *in = static_cast<const T *>(a.in);
+    const T *tab0 = static_cast<const T *>(a.tab0);
+    const T *tab1 = static_cast<const T *>(a.tab1);
+    const int K = a.moff[m + 1] - a.moff[m], ne = a.ne[m];
+    acc_t<T> accE[NT][2], accO[NT][2];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) {
+            accE[t][rb] = acc_t<T>{0, 0, 0, 0};
+            accO[t][rb] = acc_t<T>{0, 0, 0, 0};
+        }
+    synth_group<T, NT>(a, in, tab0, tab1, sA, sB, scol, a.moff[m], ne, i0, r0, VC, NTILE, accE);
+    synth_group<T, NT>(a, in, tab0, tab1, sA, sB, scol, a.moff[m] + ne, K - ne, i0, r0, VC, NTILE, accO);
+
+    T *out = static_cast<T *>(a.out);
+    const int vcol = wave * 16 + (lane & 15);
+    const int lv = vcol / VC, fv = vcol - lv * VC;
+    const int r = r0 + lv;
+    const int nm = a.mmax + 1;
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = i0 + rb * 16 + 4 * (lane >> 4) + v;
+            T on, os;
+            int slot;
+            if constexpr (VEC) {
+                // north / south parity combination: dP flips sign once more than P, Q
+                const T dPn = accE[0][rb][v] + accO[0][rb][v], dPs = accO[0][rb][v] - accE[0][rb][v];
+                const T Qn = accE[1][rb][v] + accO[1][rb][v], Qs = accE[1][rb][v] - accO[1][rb][v];
+                const T Qxn = shfl_xor3(Qn), Qxs = shfl_xor3(Qs);
+                const int q = lane & 3; // 0 S_re, 1 S_im, 2 T_re, 3 T_im
+                if (q == 0) {
+                    on = dPn - Qxn;
+                    os = dPs - Qxs;
+                } else if (q == 1) {
+                    on = dPn + Qxn;
+                    os = dPs + Qxs;
+                } else if (q == 2) {
+                    on = -Qxn - dPn;
+                    os = -Qxs - dPs;
+                } else {
+                    on = Qxn - dPn;
+                    os = Qxs - dPs;
+                }
+                slot = a.slot0 + 8 * (fv >> 2) + q; // + 4 for south
+            } else {
+                on = accE[0][rb][v] + accO[0][rb][v];
+                os = accE[0][rb][v] - accO[0][rb][v];
+                slot = a.slot0 + 4 * (fv >> 1) + (fv & 1); // + 2 for south
+            }
+            if (i < a.nlat2 && r < a.nr) {
+                T *dst = out + (((size_t)i * a.nr + r) * nm + m) * a.nslot;
+                dst[slot] = on;
+                dst[slot + (VEC ? 4 : 2)] = os;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- analysis
+template <typename T, bool VEC> __global__ void __launch_bounds__(320) ana_kernel(AnaArgs a)
+{
+    constexpr int NT = VEC ? 2 : 1;
+    extern __shared__ char smem_raw[];
+    const int VC = VEC ? 4 * a.nf : 2 * a.nf;
+    const int NTILE = 16 * a.nf;
+    const int LT = NTILE / VC;
+    T *sA = reinterpret_cast<T *>(smem_raw);
+    T *sB = sA + NT * KT * TM;
+    const int m = blockIdx.z >> 1, p = blockIdx.z & 1;
+    const int K = a.moff[m + 1] - a.moff[m], ne = a.ne[m];
+    const int Kp = p ? K - ne : ne;
+    const int kbase = a.moff[m] + (p ? ne : 0);
+    const int row0 = blockIdx.x * TM, r0 = blockIdx.y * LT;
+    if (row0 >= Kp) return; // uniform over the block
+    const int tid = threadIdx.x, nthr = blockDim.x, wave = tid >> 6, lane = tid & 63;
+    const T *gb = static_cast<const T *>(a.gb);
+    const T *tab0 = static_cast<const T *>(a.tab0), *tab1 = static_cast<const T *>(a.tab1);
+    acc_t<T> acc[NT][2];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) acc[t][rb] = acc_t<T>{0, 0, 0, 0};
+
+    for (int k0 = 0; k0 < a.nlat2; k0 += KT) {
+        for (int e = tid; e < NT * KT * TM; e += nthr) {
+            const int kk = e % KT, i = (e / KT) % TM, t = e / (KT * TM);
+            const int row = row0 + i;
+            T v = T(0);
+            if (row < Kp && k0 + kk < a.nlat2) v = (t == 0 ? tab0 : tab1)[(size_t)(kbase + row) * a.nlat2 + k0 + kk];
+            sA[(t * KT + kk) * TM + i] = v;
+        }
+        for (int e = tid; e < NT * KT * NTILE; e += nthr) {
+            const int vv = e % NTILE, kk = (e / NTILE) % KT, t = e / (NTILE * KT);
+            const int lv = vv / VC, fv = vv - lv * VC, r = r0 + lv;
+            int comp, reim, pm;
+            if (VEC) {
+                const int V = fv >> 2, q = fv & 3;
+                comp = 2 * V + (q >> 1);
+                reim = q & 1;
+                // dP has southern sign -(-1)^(l+m): even group uses G-, odd G+; Q the reverse
+                pm = (t == 0) ? (p == 0 ? 1 : 0) : p;
+            } else {
+                comp = fv >> 1;
+                reim = fv & 1;
+                pm = p;
+            }
+            T v = T(0);
+            if (r < a.nr && k0 + kk < a.nlat2)
+                v = gb[(((size_t)m * a.nlat2 + k0 + kk) * a.nr + r) * a.nslot_g + a.gslot0 + 4 * comp + 2 * pm + reim];
+            sB[(t * KT + kk) * NTILE + vv] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < KT; ks += 4) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const T b = sB[(t * KT + ks + (lane >> 4)) * NTILE + wave * 16 + (lane & 15)];
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb) {
+                    const T av = sA[(t * KT + ks + (lane >> 4)) * TM + rb * 16 + (lane & 15)];
+                    acc[t][rb] = mfma16x16x4(av, b, acc[t][rb]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    const int vcol = wave * 16 + (lane & 15);
+    const int lv = vcol / VC, fv = vcol - lv * VC, r = r0 + lv;
+    const size_t fs = (size_t)a.nr * a.nlm;
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = row0 + rb * 16 + 4 * (lane >> 4) + v;
+            double val;
+            int outk, reim;
+            if constexpr (VEC) {
+                const T X = acc[0][rb][v], Y = acc[1][rb][v];
+                const T Yx = shfl_xor3(Y);
+                const int q = lane & 3;
+                T o;
+                if (q == 0) o = X + Yx;       // S_re
+                else if (q == 1) o = X - Yx;  // S_im
+                else if (q == 2) o = Yx - X;  // T_re
+                else o = -Yx - X;             // T_im
+                val = (double)o;
+                outk = 2 * (fv >> 2) + (q >> 1);
+                reim = q & 1;
+            } else {
+                val = (double)acc[0][rb][v];
+                outk = fv >> 1;
+                reim = fv & 1;
+            }
+            if (row < Kp && r < a.nr) {
+                const int idx = kbase + row;
+                const int l = a.l_of_idx[idx];
+                const int sc = outk == 0 ? a.scale[0] : outk == 1 ? a.scale[1] : outk == 2 ? a.scale[2] : a.scale[3];
+                double *op = outk == 0 ? a.out[0] : outk == 1 ? a.out[1] : outk == 2 ? a.out[2] : a.out[3];
+                if (sc == 0) val = (l > 0) ? val / ((double)l * (double)(l + 1)) : 0.0;
+                else if (sc == 2) val = -val;
+                if (m == 0 && reim) val = 0.0; // imag(m=0) := 0 (shtnskit_field_functions.jl:432-435)
+                op[(size_t)reim * fs + (size_t)r * a.nlm + idx] = val;
+            }
+        }
+    }
+}
+} // namespace
+
+template <typename T> int gd_launch_synth_vec(hipStream_t st, const SynthArgs &a)
+{
+    GD_CHECK(a.nf >= 1 && a.nf <= 5, GD_ERR_ARG, "synth_vec: nf=%d", a.nf);
+    const int LT = 4;
+    dim3 grid((a.nlat2 + TM - 1) / TM, (a.nr + LT - 1) / LT, a.mmax + 1);
+    size_t lds = sizeof(T) * (2 * KT * TM + KT * 16 * a.nf) + 32 * sizeof(int);
+    hipLaunchKernelGGL((synth_kernel<T, true>), grid, dim3(64 * a.nf), lds, st, a);
+    return gd_check_launch("synth_kernel<vec>");
+}
+
+template <typename T> int gd_launch_synth_scal(hipStream_t st, const SynthArgs &a)
+{
+    GD_CHECK(a.nf >= 1 && a.nf <= 5, GD_ERR_ARG, "synth_scal: nf=%d", a.nf);
+    const int LT = 8;
+    dim3 grid((a.nlat2 + TM - 1) / TM, (a.nr + LT - 1) / LT, a.mmax + 1);
+    size_t lds = sizeof(T) * (KT * TM + KT * 16 * a.nf) + 32 * sizeof(int);
+    hipLaunchKernelGGL((synth_kernel<T, false>), grid, dim3(64 * a.nf), lds, st, a);
+    return gd_check_launch("synth_kernel<scal>");
+}
+
+// grid rows cover the largest parity group (maxgroup = ceil((lmax+1)/2))
+template <typename T> static int launch_ana(hipStream_t st, const AnaArgs &a, bool vec, int maxgroup)
+{
+    const int VC = vec ? 4 * a.nf : 2 * a.nf;
+    const int LT = 16 * a.nf / VC;
+    dim3 grid((maxgroup + TM - 1) / TM, (a.nr + LT - 1) / LT, 2 * (a.mmax + 1));
+    const int NT = vec ? 2 : 1;
+    size_t lds = sizeof(T) * (NT * KT * TM + NT * KT * 16 * a.nf);
+    if (vec) hipLaunchKernelGGL((ana_kernel<T, true>), grid, dim3(64 * a.nf), lds, st, a);
+    else hipLaunchKernelGGL((ana_kernel<T, false>), grid, dim3(64 * a.nf), lds, st, a);
+    return gd_check_launch(vec ? "ana_kernel<vec>" : "ana_kernel<scal>");
+}
+
+template <typename T> int gd_launch_ana(hipStream_t st, const AnaArgs &a, bool vec, int maxgroup)
+{
+    GD_CHECK(a.nf >= 1 && a.nf <= (vec ? 2 : 5), GD_ERR_ARG, "ana: nf=%d", a.nf);
+    return launch_ana<T>(st, a, vec, maxgroup);
+}
+
+// ====================================================================== FFT ring kernel
+namespace {
+template <typename T> struct C2 {
+    T x, y;
+};
+template <typename T> __device__ __forceinline__ C2<T> cadd(C2<T> a, C2<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T> __device__ __forceinline__ C2<T> csub(C2<T> a, C2<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T> __device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// multiply by i*dir
+template <typename T> __device__ __forceinline__ C2<T> cmuli(C2<T> a, int dir)
+{
+    return dir > 0 ? C2<T>{-a.y, a.x} : C2<T>{a.y, -a.x};
+}
+
+template <typename T> __device__ __forceinline__ void dft2(C2<T> &a, C2<T> &b)
+{
+    C2<T> t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+template <typename T> __device__ __forceinline__ void dft3(C2<T> &v0, C2<T> &v1, C2<T> &v2, int dir)
+{
+    const T s3 = T(0.86602540378443864676);
+    C2<T> t1 = cadd(v1, v2);
+    C2<T> t2 = {v0.x - T(0.5) * t1.x, v0.y - T(0.5) * t1.y};
+    C2<T> d = csub(v1, v2);
+    C2<T> t3 = cmuli(C2<T>{s3 * d.x, s3 * d.y}, dir);
+    v0 = cadd(v0, t1);
+    v1 = cadd(t2, t3);
+    v2 = csub(t2, t3);
+}
+template <typename T> __device__ __forceinline__ void dft4(C2<T> &v0, C2<T> &v1, C2<T> &v2, C2<T> &v3, int dir)
+{
+    C2<T> a = cadd(v0, v2), b = csub(v0, v2), c = cadd(v1, v3), d = cmuli(csub(v1, v3), dir);
+    v0 = cadd(a, c);
+    v2 = csub(a, c);
+    v1 = cadd(b, d);
+    v3 = csub(b, d);
+}
+
+template <int R, typename T> __device__ __forceinline__ void dftR(C2<T> *v, int dir)
+{
+    if constexpr (R == 2) {
+        dft2(v[0], v[1]);
+    } else if constexpr (R == 3) {
+        dft3(v[0], v[1], v[2], dir);
+    } else if constexpr (R == 4) {
+        dft4(v[0], v[1], v[2], v[3], dir);
+    } else if constexpr (R == 6) {
+        // 6 = 2 x 3: E = DFT3(v0,v2,v4), O = DFT3(v1,v3,v5), X_k = E_k + w6^k O_k
+        C2<T> e0 = v[0], e1 = v[2], e2 = v[4], o0 = v[1], o1 = v[3], o2 = v[5];
+        dft3(e0, e1, e2, dir);
+        dft3(o0, o1, o2, dir);
+        const T h = T(0.5), s3 = T(0.86602540378443864676);
+        C2<T> w1 = {h, dir > 0 ? s3 : -s3}, w2 = {-h, dir > 0 ? s3 : -s3};
+        o1 = cmul(o1, w1);
+        o2 = cmul(o2, w2);
+        v[0] = cadd(e0, o0);
+        v[3] = csub(e0, o0);
+        v[1] = cadd(e1, o1);
+        v[4] = csub(e1, o1);
+        v[2] = cadd(e2, o2);
+        v[5] = csub(e2, o2);
+    } else if constexpr (R == 8) {
+        C2<T> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6], o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+        dft4(e0, e1, e2, e3, dir);
+        dft4(o0, o1, o2, o3, dir);
+        const T r2 = T(0.70710678118654752440);
+        const T sg = dir > 0 ? T(1) : T(-1);
+        o1 = cmul(o1, C2<T>{r2, sg * r2});
+        o2 = cmuli(o2, dir);
+        o3 = cmul(o3, C2<T>{-r2, sg * r2});
+        v[0] = cadd(e0, o0);
+        v[4] = csub(e0, o0);
+        v[1] = cadd(e1, o1);
+        v[5] = csub(e1, o1);
+        v[2] = cadd(e2, o2);
+        v[6] = csub(e2, o2);
+        v[3] = cadd(e3, o3);
+        v[7] = csub(e3, o3);
+    }
+}
+
+constexpr int MAXC = 32; // complex values a thread holds across one stage's barrier
+
+// One Stockham stage over nfft transforms of length n stored back to back in
+// buf; in place, with every butterfly read before any is written.
+template <int R, typename T>
+__device__ __noinline__ void fft_stage(C2<T> *buf, const C2<T> *tw, int nfft, int n, int Ns, int dir)
+{
+    constexpr int IT = MAXC / R;
+    const int nb = n / R, total = nfft * nb;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int twstep = n / (Ns * R);
+    C2<T> v[IT][R];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int g = tid + it * nthr;
+        if (g < total) {
+            const int f = g / nb, j = g - f * nb;
+            const int k = j % Ns;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[it][r] = buf[f * n + j + r * nb];
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                C2<T> w = tw[k * r * twstep];
+                if (dir > 0) w.y = -w.y;
+                v[it][r] = cmul(v[it][r], w);
+            }
+            dftR<R, T>(v[it], dir);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int g = tid + it * nthr;
+        if (g < total) {
+            const int f = g / nb, j = g - f * nb;
+            const int k = j % Ns;
+            const int idxD = (j / Ns) * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[f * n + idxD + r * Ns] = v[it][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ void fft_batch(C2<T> *buf, const C2<T> *tw, int nfft, int n, const int *radix, int nstages, int dir)
+{
+    int Ns = 1;
+    for (int s = 0; s < nstages; ++s) {
+        const int R = radix[s];
+        switch (R) {
+        case 8: fft_stage<8, T>(buf, tw, nfft, n, Ns, dir); break;
+        case 6: fft_stage<6, T>(buf, tw, nfft, n, Ns, dir); break;
+        case 4: fft_stage<4, T>(buf, tw, nfft, n, Ns, dir); break;
+        case 3: fft_stage<3, T>(buf, tw, nfft, n, Ns, dir); break;
+        default: fft_stage<2, T>(buf, tw, nfft, n, Ns, dir); break;
+        }
+        Ns *= R;
+    }
+}
+
+// One block per (northern node i, level r).
+template <typename T, int MODE> __global__ void __launch_bounds__(256) ring_kernel(RingArgs a)
+{
+    extern __shared__ char smem_raw[];
+    const int n = a.nlon, nm = a.mmax + 1;
+    C2<T> *tw = reinterpret_cast<C2<T> *>(smem_raw);
+    C2<T> *buf = tw + n;
+    __shared__ int srad[12];
+    __shared__ int sfn[16], sfs[16];
+    const int i = blockIdx.x, r = blockIdx.y;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int is = a.nlat - 1 - i; // southern mirror
+    const C2<T> *twg = static_cast<const C2<T> *>(a.tw);
+    for (int k = tid; k < n; k += nthr) tw[k] = twg[k];
+    if (tid < 12) srad[tid] = a.radix[tid];
+    if (tid < 16) {
+        sfn[tid] = a.fn_off[tid];
+        sfs[tid] = a.fs_off[tid];
+    }
+    __syncthreads();
+    const int NF = a.nfield;
+
+    if (MODE == RING_SYNTH || MODE == RING_STEP) {
+        // Hermitian spectra of the north (X) and south (Y) rings packed as Z = X + iY
+        const T *src = static_cast<const T *>(a.fb) + ((size_t)i * a.nr + r) * nm * a.nslot_f;
+        for (int e = tid; e < NF * nm; e += nthr) {
+            const int F = e % NF, m = e / NF;
+            const T *s = src + (size_t)m * a.nslot_f;
+            const T xr = s[sfn[F]], xi = s[sfn[F] + 1], yr = s[sfs[F]], yi = s[sfs[F] + 1];
+            if (m == 0) {
+                buf[F * n] = C2<T>{xr, yr};
+            } else {
+                buf[F * n + m] = C2<T>{xr - yi, xi + yr};
+                buf[F * n + n - m] = C2<T>{xr + yi, yr - xi};
+            }
+        }
+        const int nz = n - 2 * nm + 1;
+        for (int e = tid; e < NF * nz; e += nthr) {
+            const int F = e / nz, k = nm + e % nz;
+            buf[F * n + k] = C2<T>{T(0), T(0)};
+        }
+        __syncthreads();
+        fft_batch<T>(buf, tw, NF, n, srad, a.nstages, +1);
+    }
+
+    if (MODE == RING_SYNTH) {
+        for (int j = tid; j < n; j += nthr)
+            for (int F = 0; F < NF; ++F) {
+                double *o = F == 0 ? a.phys_out[0] : a.phys_out[1];
+                const C2<T> z = buf[F * n + j];
+                o[((size_t)r * n + j) * a.nlat + i] = (double)z.x;
+                o[((size_t)r * n + j) * a.nlat + is] = (double)z.y;
+            }
+        return;
+    }
+
+    int NP;
+    if (MODE == RING_ANALYSIS) {
+        NP = NF;
+        for (int j = tid; j < n; j += nthr)
+            for (int F = 0; F < NF; ++F) {
+                const double *p = F == 0 ? a.phys_in[0] : a.phys_in[1];
+                buf[F * n + j] = C2<T>{(T)p[((size_t)r * n + j) * a.nlat + i], (T)p[((size_t)r * n + j) * a.nlat + is]};
+            }
+        __syncthreads();
+    } else {
+        // pointwise nonlinear terms on the north (.x) and south (.y) rings
+        // fields: 0 u_t 1 u_p 2 w_t 3 w_p 4 B_t 5 B_p 6 j_t 7 j_p 8 gT_t 9 gT_p
+        //         10 u_r 11 w_r 12 B_r 13 j_r 14 dT_r
+        NP = 5;
+        const T ro = (T)a.ro;
+        const T *sinth = static_cast<const T *>(a.sinth), *costh = static_cast<const T *>(a.costh);
+        for (int j = tid; j < n; j += nthr) {
+            C2<T> f[15];
+#pragma unroll
+            for (int F = 0; F < 15; ++F) f[F] = buf[F * n + j];
+            C2<T> o[5];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                int ti;
+                T sth, cth;
+                if (a.coriolis_index) {
+                    ti = h ? is : i;
+                } else {
+                    ti = j < a.nlat - 1 ? j : a.nlat - 1; // velocity.jl:494 theta_{min(j,nlat)}
+                }
+                sth = sinth[ti];
+                cth = costh[ti];
+#define FV(F) (h ? f[F].y : f[F].x)
+                const T ut = FV(0), up = FV(1), wt = FV(2), wp = FV(3), Bt = FV(4), Bp = FV(5);
+                const T jt = FV(6), jp = FV(7), gt = FV(8), gp = FV(9);
+                const T ur = FV(10), wr = FV(11), Br = FV(12), jr = FV(13), gr = FV(14);
+#undef FV
+                // velocity.jl:512-527, 664-670; magnetic.jl:399-401; scalar_field_common.jl:401-407
+                T at = -ro * (up * wr - ur * wp) + (-ro * (-cth * up));
+                T ap = -ro * (ur * wt - ut * wr) + (-ro * (cth * ut + sth * ur));
+                at += ro * (jp * Br - jr * Bp);
+                ap += ro * (jr * Bt - jt * Br);
+                const T bt = up * Br - ur * Bp;
+                const T bp = ur * Bt - ut * Br;
+                const T aT = -(ur * gr + ut * gt + up * gp);
+                const T vals[5] = {at, ap, bt, bp, aT};
+#pragma unroll
+                for (int P = 0; P < 5; ++P) {
+                    if (h) o[P].y = vals[P];
+                    else o[P].x = vals[P];
+                }
+            }
+#pragma unroll
+            for (int P = 0; P < 5; ++P) buf[P * n + j] = o[P];
+        }
+        __syncthreads();
+    }
+
+    fft_batch<T>(buf, tw, NP, n, srad, a.nstages, -1);
+
+    // split Z into north X and south Y, form G+- = (X +- Y) * 2 pi w_i / n
+    const T c = static_cast<const T *>(a.wq)[i];
+    T *gb = static_cast<T *>(a.gb);
+    for (int e = tid; e < NP * nm; e += nthr) {
+        const int P = e % NP, m = e / NP;
+        const C2<T> Z = buf[P * n + m];
+        C2<T> X, Y;
+        if (m == 0) {
+            X = C2<T>{Z.x, T(0)};
+            Y = C2<T>{Z.y, T(0)};
+        } else {
+            const C2<T> Zc = buf[P * n + n - m];
+            X = C2<T>{T(0.5) * (Z.x + Zc.x), T(0.5) * (Z.y - Zc.y)};
+            Y = C2<T>{T(0.5) * (Z.y + Zc.y), T(0.5) * (Zc.x - Z.x)};
+        }
+        T *d = gb + (((size_t)m * a.nlat2 + i) * a.nr + r) * a.nslot_g + 4 * P;
+        d[0] = (X.x + Y.x) * c;
+        d[1] = (X.y + Y.y) * c;
+        d[2] = (X.x - Y.x) * c;
+        d[3] = (X.y - Y.y) * c;
+    }
+}
+} // namespace
+
+template <typename T> int gd_launch_ring(hipStream_t st, int mode, const RingArgs &a)
+{
+    const int nthr = 256;
+    int nf = (mode == RING_STEP) ? 15 : a.nfield;
+    GD_CHECK(nf >= 1 && nf <= 15, GD_ERR_ARG, "ring: nfield=%d", nf);
+    for (int s = 0; s < a.nstages; ++s) {
+        int R = a.radix[s];
+        long per = ((long)nf * (a.nlon / R) + nthr - 1) / nthr;
+        GD_CHECK(per * R <= MAXC, GD_ERR_ARG, "ring: nlon=%d too large for %d fields", a.nlon, nf);
+    }
+    size_t lds = sizeof(C2<T>) * (size_t)a.nlon * (nf + 1);
+    GD_CHECK(lds <= 160 * 1024, GD_ERR_ARG, "ring: LDS %zu too large", lds);
+    dim3 grid(a.nlat2, a.nr);
+    switch (mode) {
+    case RING_SYNTH: hipLaunchKernelGGL((ring_kernel<T, RING_SYNTH>), grid, dim3(nthr), lds, st, a); break;
+    case RING_ANALYSIS: hipLaunchKernelGGL((ring_kernel<T, RING_ANALYSIS>), grid, dim3(nthr), lds, st, a); break;
+    default: hipLaunchKernelGGL((ring_kernel<T, RING_STEP>), grid, dim3(nthr), lds, st, a); break;
+    }
+    return gd_check_launch("ring_kernel");
+}
+
+// ====================================================================== permutations
+template <typename T>
+__global__ void pack_kernel(const double *__restrict__ re, const double *__restrict__ im,
+                            const int *__restrict__ lmaj_of_idx, int nlm, int nr, int cin, int c0, T *__restrict__ out)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+    if (idx >= nlm) return;
+    const int lm = lmaj_of_idx[idx];
+    out[(size_t)(r * cin + c0) * nlm + idx] = (T)re[(size_t)r * nlm + lm];
+    out[(size_t)(r * cin + c0 + 1) * nlm + idx] = (T)im[(size_t)r * nlm + lm];
+}
+
+template <typename T>
+int gd_launch_pack(hipStream_t st, const double *re, const double *im, const int *lmaj_of_idx, int nlm, int nr,
+                   int cin, int c0, T *out)
+{
+    hipLaunchKernelGGL((pack_kernel<T>), dim3((nlm + 255) / 256, nr), dim3(256), 0, st, re, im, lmaj_of_idx, nlm, nr,
+                       cin, c0, out);
+    return gd_check_launch("pack_kernel");
+}
+
+// dst[r][j] = src[r][map[j]]
+__global__ void perm_kernel(const double *__restrict__ src, const int *__restrict__ map, int nlm, int nr,
+                            double *__restrict__ dst)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+    if (j >= nlm) return;
+    dst[(size_t)r * nlm + j] = src[(size_t)r * nlm + map[j]];
+}
+
+int gd_launch_perm(hipStream_t st, const double *src, const int *map, int nlm, int nr, double *dst)
+{
+    hipLaunchKernelGGL(perm_kernel, dim3((nlm + 255) / 256, nr), dim3(256), 0, st, src, map, nlm, nr, dst);
+    return gd_check_launch("perm_kernel");
+}
+
+int gd_launch_unpack(hipStream_t st, const double *src, const int *idx_of_lmaj, int nlm, int nr, double *dst)
+{
+    return gd_launch_perm(st, src, idx_of_lmaj, nlm, nr, dst);
+}
+
+// explicit instantiations
+template int gd_launch_synth_vec<float>(hipStream_t, const SynthArgs &);
+template int gd_launch_synth_vec<double>(hipStream_t, const SynthArgs &);
+template int gd_launch_synth_scal<float>(hipStream_t, const SynthArgs &);
+template int gd_launch_synth_scal<double>(hipStream_t, const SynthArgs &);
+template int gd_launch_ana<float>(hipStream_t, const AnaArgs &, bool, int);
+template int gd_launch_ana<double>(hipStream_t, const AnaArgs &, bool, int);
+template int gd_launch_ring<float>(hipStream_t, int, const RingArgs &);
+template int gd_launch_ring<double>(hipStream_t, int, const RingArgs &);
+template int gd_launch_pack<float>(hipStream_t, const double *, const double *, const int *, int, int, int, int,
+                                   float *);
+template int gd_launch_pack<double>(hipStream_t, const double *, const double *, const int *, int, int, int, int,
+                                    double *);

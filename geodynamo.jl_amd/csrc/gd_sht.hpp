@@ -1,0 +1,72 @@
+// gd_sht.hpp -- SHT device kernels (gd_sht.hip): argument blocks and launchers.
+//
+// Device spectral layout ("m-major"): index idx in [0, nlm) with, for each m,
+// the block [moff[m], moff[m+1]) holding l = m..lmax, first the ne[m] degrees
+// with l-m even, then those with l-m odd (LmMaps).  A spectral column set is
+// stored [col][idx] with col = level*cin + c.
+//
+// Legendre tables Tab[t][idx][i], i in [0, nlat/2) the northern Gauss nodes,
+// t = 0 P, 1 dP/dtheta, 2 m P / sin(theta).  Southern values follow from the
+// parity P(pi-theta) = (-1)^(l+m) P(theta) (dP flips sign once more).
+//
+// Fourier layouts (complex of the real longitude transform, m = 0..mmax):
+//   FB[i][level][m][slot]  synthesis output, north and south rings of node i
+//   GB[m][i][level][slot]  analysis input G+ = Gn + Gs, G- = Gn - Gs, already
+//                          weighted by 2 pi w_i / nlon
+#pragma once
+#include "gd_internal.hpp"
+
+struct SynthArgs {
+    const void *in;       // T*, [col][idx], col = level*cin + c
+    int cin;              // input columns per level
+    int nf;               // vector: fields (4 virtual cols each); scalar: fields (2 each)
+    int colmap[20];       // virtual col within a level -> input col c, -1 = zero
+    const void *tab0;     // vector: dP, scalar: P
+    const void *tab1;     // vector: Q = m P / sin
+    const int *moff, *ne;
+    int nlat2, nr, nlm, mmax;
+    void *out;            // T*, FB
+    int nslot, slot0;     // vector field f -> slot0 + 8f + 4h + (2c + reim); scalar -> slot0 + 4f + 2h + reim
+};
+
+struct AnaArgs {
+    const void *gb;       // T*, GB
+    int nslot_g, gslot0;  // component k of field f: gslot0 + 4*(comp index) + 2*pm + reim
+    int nf;               // vector fields (components 2V, 2V+1) or scalar fields
+    const void *tab0, *tab1;
+    const int *moff, *ne, *l_of_idx;
+    int nlat2, nr, nlm, mmax;
+    // outputs, FP64 m-major [reim][level][idx]; vector field V: out[2V] gets S, out[2V+1] gets T
+    double *out[4];
+    int scale[4];         // 0: /l(l+1) (0 at l=0), 1: x1, 2: x(-1)
+};
+
+struct RingArgs {
+    const void *fb;       // T*
+    void *gb;             // T*
+    int nslot_f, nslot_g;
+    int nfield;           // real fields to synthesise (SYNTH/STEP) or analyse (ANALYSIS)
+    int fn_off[16], fs_off[16]; // FB slot (re) of north / south spectra of each field
+    const double *phys_in[2];
+    double *phys_out[2];
+    const void *tw;       // T2[nlon], e^{-2 pi i k / nlon}
+    const void *sinth, *costh, *wq; // T[nlat] sin, cos of colatitude; T[nlat2] 2 pi w / nlon
+    int nlat, nlat2, nlon, nr, mmax;
+    int nstages, radix[12];
+    double ro;            // Pm / E
+    int coriolis_index;
+};
+
+enum { RING_SYNTH = 0, RING_ANALYSIS = 1, RING_STEP = 2 };
+
+int gd_launch_plm(hipStream_t st, const double *x, const int *moff, int lmax, int mmax, int nlat2, double *tab64,
+                  float *tab32, int nlm);
+template <typename T> int gd_launch_synth_vec(hipStream_t st, const SynthArgs &a);
+template <typename T> int gd_launch_synth_scal(hipStream_t st, const SynthArgs &a);
+template <typename T> int gd_launch_ana(hipStream_t st, const AnaArgs &a, bool vec, int maxgroup);
+template <typename T> int gd_launch_ring(hipStream_t st, int mode, const RingArgs &a);
+template <typename T>
+int gd_launch_pack(hipStream_t st, const double *re, const double *im, const int *lmaj_of_idx, int nlm, int nr,
+                   int cin, int c0, T *out);
+int gd_launch_unpack(hipStream_t st, const double *src, const int *idx_of_lmaj, int nlm, int nr, double *dst);
+int gd_launch_perm(hipStream_t st, const double *src, const int *map, int nlm, int nr, double *dst);

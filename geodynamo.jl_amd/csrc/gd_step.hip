@@ -1,0 +1,461 @@
+// gd_step.hip -- the device-resident pseudo-spectral CNAB2 step.
+//
+// One step (scripts/bench_timestepping.jl:6-18 semantics, all nonlinear terms
+// from the current state):
+//   1. spectral_pre_kernel  curl of u and B (velocity.jl:427-435,
+//      magnetic.jl:335-343), radial components X_r = l(l+1) P_X / r^2
+//      (magnetic.jl:257-259), T/r and dT/dr (scalar_field_common.jl:233-327),
+//      written as Legendre-GEMM input columns in the plan's arithmetic type
+//   2. synth_kernel (vector: u, omega, B, j, grad T; scalar: u_r, omega_r,
+//      B_r, j_r, dT/dr) -> Fourier rings FB
+//   3. ring_kernel<STEP>: inverse FFTs, u x omega + Coriolis + j x B, u x B,
+//      -u.grad T (velocity.jl:512-527,664-670; magnetic.jl:399-401;
+//      scalar_field_common.jl:401-407), forward FFTs -> GB
+//   4. ana_kernel: vector analysis of the force and of u x B (with the curl
+//      factors of magnetic.jl:437-440), scalar analysis of -u.grad T -> nl
+//   5. cnab2_step_kernel: temperature Dirichlet rows, AB2 (AB1 at step 0,
+//      simulation.jl:1153), rhs + FP64 banded solve per column
+//      (timestep.jl:610-730); the nl history rolls by buffer swap
+//      (simulation.jl:1312-1329).
+#include <cmath>
+
+#include "gd_plan.hpp"
+#include "gd_radial.hpp"
+
+struct gd_step {
+    gd_ctx *ctx = nullptr;
+    gd_step_desc d{};
+    gd_sht_plan sht;
+    int nlm = 0, nl = 0, W = 0;
+    size_t fs = 0;                 // nlm * nr
+    DevBuf state, nlbuf[2];        // m-major FP64 [field][re/im][r][idx]
+    DevBuf Lt, LUt;                // [nu][(col*W+row)][l], nu = velocity, magnetic, temperature
+    DevBuf D1, D2, rtab;
+    DevBuf synV, synS, fb, gb;
+    DevBuf part, eout, scratch;
+    DevBuf status;
+    int step = 0;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+};
+
+// ---------------------------------------------------------------- spectral preparation
+// One thread per (idx, level).  synV columns per level (c = 2k + re/im):
+//   k: 0 u_pol 1 u_tor 2 w_pol 3 w_tor 4 B_pol 5 B_tor 6 j_pol 7 j_tor 8 T/r
+// synS columns per level: k: 0 u_r 1 w_r 2 B_r 3 j_r 4 dT/dr
+template <typename T>
+__global__ void __launch_bounds__(256) spectral_pre_kernel(const double *__restrict__ st, const int *__restrict__ l_of_idx,
+                                                           const double *__restrict__ D1, const double *__restrict__ D2,
+                                                           const double *__restrict__ rtab, int nlm, int nr, int bw,
+                                                           int radials, T *__restrict__ synV, T *__restrict__ synS)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+    if (idx >= nlm) return;
+    const size_t fs = (size_t)nlm * nr;
+    const int W = 2 * bw + 1;
+    const double lf = (double)l_of_idx[idx] * (l_of_idx[idx] + 1);
+    const double rinv = rtab[r + nr * 2], rinv2 = rtab[r + nr * 1];
+    const int j0 = r - bw < 0 ? 0 : r - bw, j1 = r + bw > nr - 1 ? nr - 1 : r + bw;
+#pragma unroll
+    for (int reim = 0; reim < 2; ++reim) {
+        const double *utor = st + (size_t)(0 + reim) * fs, *upol = st + (size_t)(2 + reim) * fs;
+        const double *Btor = st + (size_t)(4 + reim) * fs, *Bpol = st + (size_t)(6 + reim) * fs;
+        const double *Tt = st + (size_t)(8 + reim) * fs;
+        double d1u = 0, d2u = 0, d1B = 0, d2B = 0, d1T = 0;
+        for (int j = j0; j <= j1; ++j) {
+            const int b = bw + r - j + W * j;
+            const size_t o = (size_t)j * nlm + idx;
+            const double c1 = D1[b], c2 = D2[b];
+            const double up = upol[o], Bp = Bpol[o];
+            d1u += c1 * up;
+            d2u += c2 * up;
+            d1B += c1 * Bp;
+            d2B += c2 * Bp;
+            d1T += c1 * Tt[o];
+        }
+        const size_t o = (size_t)r * nlm + idx;
+        const double up = upol[o], ut = utor[o], Bp = Bpol[o], Bt = Btor[o], tv = Tt[o];
+        const double wtor = lf * rinv2 * up - d2u - 2.0 * rinv * d1u;
+        const double wpol = -lf * rinv2 * ut;
+        const double jtor = lf * rinv2 * Bp - d2B - 2.0 * rinv * d1B;
+        const double jpol = -lf * rinv2 * Bt;
+        T *v = synV + (size_t)r * 18 * nlm + idx;
+        v[(0 + reim) * (size_t)nlm] = (T)up;
+        v[(2 + reim) * (size_t)nlm] = (T)ut;
+        v[(4 + reim) * (size_t)nlm] = (T)wpol;
+        v[(6 + reim) * (size_t)nlm] = (T)wtor;
+        v[(8 + reim) * (size_t)nlm] = (T)Bp;
+        v[(10 + reim) * (size_t)nlm] = (T)Bt;
+        v[(12 + reim) * (size_t)nlm] = (T)jpol;
+        v[(14 + reim) * (size_t)nlm] = (T)jtor;
+        v[(16 + reim) * (size_t)nlm] = (T)(tv * rinv);
+        T *s = synS + (size_t)r * 10 * nlm + idx;
+        const double f = radials ? 1.0 : 0.0;
+        s[(0 + reim) * (size_t)nlm] = (T)(f * (lf * up * rinv2));
+        s[(2 + reim) * (size_t)nlm] = (T)(f * (lf * wpol * rinv2));
+        s[(4 + reim) * (size_t)nlm] = (T)(f * (lf * Bp * rinv2));
+        s[(6 + reim) * (size_t)nlm] = (T)(f * (lf * jpol * rinv2));
+        s[(8 + reim) * (size_t)nlm] = (T)d1T;
+    }
+}
+
+// ---------------------------------------------------------------- orchestration
+template <typename T> static int nonlinear_T(gd_step *s, double *nl_out)
+{
+    hipStream_t st = s->ctx->stream;
+    gd_sht_plan *p = &s->sht;
+    const int nr = s->d.nr, nlm = s->nlm;
+    hipLaunchKernelGGL((spectral_pre_kernel<T>), dim3((nlm + 255) / 256, nr), dim3(256), 0, st,
+                       s->state.as<double>(), p->d_l_of_idx.as<int>(), s->D1.as<double>(), s->D2.as<double>(),
+                       s->rtab.as<double>(), nlm, nr, s->d.bw, s->d.radials, s->synV.as<T>(), s->synS.as<T>());
+    GD_TRY(gd_check_launch("spectral_pre_kernel"));
+
+    SynthArgs a{};
+    a.moff = p->d_moff.as<int>();
+    a.ne = p->d_ne.as<int>();
+    a.nlat2 = p->nlat2;
+    a.nr = nr;
+    a.nlm = nlm;
+    a.mmax = p->mmax;
+    a.out = s->fb.p;
+    a.nslot = 60;
+    // vector fields u, omega, B, j (S,T) and grad T (S only)
+    a.in = s->synV.p;
+    a.cin = 18;
+    a.nf = 5;
+    static const int vmap[20] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, -1, -1};
+    for (int k = 0; k < 20; ++k) a.colmap[k] = vmap[k];
+    a.tab0 = gd_sht_tab(p, 1);
+    a.tab1 = gd_sht_tab(p, 2);
+    a.slot0 = 0;
+    GD_TRY(gd_launch_synth_vec<T>(st, a));
+    a.in = s->synS.p;
+    a.cin = 10;
+    a.nf = 5;
+    for (int k = 0; k < 20; ++k) a.colmap[k] = k < 10 ? k : -1;
+    a.tab0 = gd_sht_tab(p, 0);
+    a.tab1 = nullptr;
+    a.slot0 = 40;
+    GD_TRY(gd_launch_synth_scal<T>(st, a));
+
+    RingArgs r{};
+    gd_sht_fill_ring(p, r, nr);
+    r.fb = s->fb.p;
+    r.gb = s->gb.p;
+    r.nslot_f = 60;
+    r.nslot_g = 20;
+    r.nfield = 15;
+    for (int F = 0; F < 10; ++F) {
+        const int f = F / 2, c = F % 2;
+        r.fn_off[F] = 8 * f + 2 * c;
+        r.fs_off[F] = 8 * f + 4 + 2 * c;
+    }
+    for (int F = 10; F < 15; ++F) {
+        r.fn_off[F] = 40 + 4 * (F - 10);
+        r.fs_off[F] = 40 + 4 * (F - 10) + 2;
+    }
+    r.ro = s->d.Pm / s->d.E;
+    r.coriolis_index = s->d.coriolis_index;
+    GD_TRY(gd_launch_ring<T>(st, RING_STEP, r));
+
+    AnaArgs g{};
+    g.gb = s->gb.p;
+    g.nslot_g = 20;
+    g.moff = p->d_moff.as<int>();
+    g.ne = p->d_ne.as<int>();
+    g.l_of_idx = p->d_l_of_idx.as<int>();
+    g.nlat2 = p->nlat2;
+    g.nr = nr;
+    g.nlm = nlm;
+    g.mmax = p->mmax;
+    const size_t fs = s->fs;
+    const int maxgroup = (p->lmax + 2) / 2;
+    // force -> (pol, tor) of velocity nl (velocity.jl:331); u x B -> curl (magnetic.jl:437-440)
+    g.nf = 2;
+    g.gslot0 = 0;
+    g.tab0 = gd_sht_tab(p, 1);
+    g.tab1 = gd_sht_tab(p, 2);
+    g.out[0] = nl_out + 2 * fs; // S(adv)   -> u_pol
+    g.out[1] = nl_out + 0 * fs; // T(adv)   -> u_tor
+    g.out[2] = nl_out + 4 * fs; // l(l+1) S(uxB)  -> B_tor
+    g.out[3] = nl_out + 6 * fs; // -l(l+1) T(uxB) -> B_pol
+    g.scale[0] = 0;
+    g.scale[1] = 0;
+    g.scale[2] = 1;
+    g.scale[3] = 2;
+    GD_TRY(gd_launch_ana<T>(st, g, true, maxgroup));
+    g.nf = 1;
+    g.gslot0 = 16;
+    g.tab0 = gd_sht_tab(p, 0);
+    g.tab1 = nullptr;
+    g.out[0] = nl_out + 8 * fs; // -u.grad T -> T
+    g.scale[0] = 1;
+    GD_TRY(gd_launch_ana<T>(st, g, false, maxgroup));
+    return GD_OK;
+}
+
+static int nonlinear(gd_step *s, double *nl_out)
+{
+    if (s->d.prec == GD_PREC_FP64) return nonlinear_T<double>(s, nl_out);
+    return nonlinear_T<float>(s, nl_out);
+}
+
+// one full step with the nl buffers chosen by parity; boot = AB1 bootstrap
+static int enqueue_step(gd_step *s, int parity, bool boot)
+{
+    double *cur = s->nlbuf[parity].as<double>();
+    double *prev = boot ? cur : s->nlbuf[parity ^ 1].as<double>();
+    GD_TRY(nonlinear(s, cur));
+    return gd_launch_cnab2_step(s->ctx->stream, s->Lt.as<double>(), s->LUt.as<double>(), s->sht.d_l_of_idx.as<int>(),
+                                s->nl, s->d.nr, s->d.bw, s->nlm, s->d.dt, s->d.theta, s->state.as<double>(), cur,
+                                prev);
+}
+
+static void destroy_graphs(gd_step *s)
+{
+    for (int k = 0; k < 2; ++k) {
+        if (s->gexec[k]) (void)hipGraphExecDestroy(s->gexec[k]);
+        if (s->graph[k]) (void)hipGraphDestroy(s->graph[k]);
+        s->gexec[k] = nullptr;
+        s->graph[k] = nullptr;
+    }
+}
+
+static int step_release(gd_step *s)
+{
+    destroy_graphs(s);
+    DevBuf *bufs[] = {&s->state, &s->nlbuf[0], &s->nlbuf[1], &s->Lt, &s->LUt, &s->D1, &s->D2, &s->rtab, &s->synV,
+                      &s->synS, &s->fb, &s->gb, &s->part, &s->eout, &s->scratch, &s->status};
+    for (DevBuf *b : bufs) b->release();
+    gd_sht_plan_release(&s->sht);
+    return GD_OK;
+}
+
+template <typename T> static int upload_vec(gd_ctx *ctx, DevBuf &b, const std::vector<T> &v)
+{
+    GD_TRY(b.ensure(sizeof(T) * v.size()));
+    GD_HIP(hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, ctx->stream));
+    return GD_OK;
+}
+
+static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
+{
+    s->ctx = ctx;
+    s->d = *d;
+    GD_CHECK(d->nr >= 2 && d->bw >= 1 && d->bw <= 7 && d->dt > 0 && d->E != 0.0 && d->Pr != 0.0, GD_ERR_ARG,
+             "gd_step_create: bad parameters");
+    GD_CHECK(d->prec == GD_PREC_FP64 || d->prec == GD_PREC_FP32, GD_ERR_ARG,
+             "gd_step_create: precision %d not available", d->prec);
+    GD_HIP(hipSetDevice(ctx->device));
+    GD_TRY(gd_sht_plan_build(ctx, d->lmax, d->mmax, d->nlat, d->nlon, d->prec, &s->sht));
+    const int nr = d->nr, bw = d->bw;
+    s->nlm = s->sht.nlm;
+    s->nl = d->lmax + 1;
+    s->W = 2 * bw + 1;
+    s->fs = (size_t)s->nlm * nr;
+    // radial grid and FD operators (host FP64, fields.jl:108-131, linear_algebra.jl:90-151)
+    std::vector<double> r((size_t)nr * 7), d1((size_t)s->W * nr), d2((size_t)s->W * nr), lap((size_t)s->W * nr);
+    GD_TRY(gd_radial_grid(nr, d->rratio, r.data()));
+    GD_TRY(gd_derivative_matrix(1, nr, bw, r.data(), d->fd_mode, d1.data()));
+    GD_TRY(gd_derivative_matrix(2, nr, bw, r.data(), d->fd_mode, d2.data()));
+    GD_TRY(gd_radial_laplacian(nr, bw, r.data(), d->fd_mode, lap.data()));
+    GD_TRY(upload_vec(ctx, s->rtab, r));
+    GD_TRY(upload_vec(ctx, s->D1, d1));
+    GD_TRY(upload_vec(ctx, s->D2, d2));
+    // implicit operators per diffusivity (simulation.jl:229-231), FP64 on the GPU
+    const size_t nb = (size_t)s->W * nr * s->nl;
+    GD_TRY(s->Lt.ensure(sizeof(double) * nb * 3));
+    GD_TRY(s->LUt.ensure(sizeof(double) * nb * 3));
+    DevBuf dlap, tmpL, tmpS, tmpLU;
+    int rc = upload_vec(ctx, dlap, lap);
+    if (rc == GD_OK) rc = tmpL.ensure(sizeof(double) * nb);
+    if (rc == GD_OK) rc = tmpS.ensure(sizeof(double) * nb);
+    if (rc == GD_OK) rc = tmpLU.ensure(sizeof(double) * nb);
+    if (rc == GD_OK) rc = s->status.ensure(sizeof(int));
+    if (rc == GD_OK && hipMemsetAsync(s->status.p, 0, sizeof(int), ctx->stream) != hipSuccess) rc = GD_ERR_HIP;
+    const double nus[3] = {d->Pm, 1.0, d->Pm / d->Pr};
+    for (int k = 0; k < 3 && rc == GD_OK; ++k)
+        rc = gd_cnab2_build(ctx->stream, dlap.as<double>(), s->rtab.as<double>() + nr, nus[k], d->dt, d->theta, s->nl,
+                            nr, bw, tmpL.as<double>(), tmpS.as<double>(), tmpLU.as<double>(),
+                            s->Lt.as<double>() + k * nb, s->LUt.as<double>() + k * nb, s->status.as<int>());
+    int stv = 0;
+    if (rc == GD_OK && hipMemcpyAsync(&stv, s->status.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+        rc = GD_ERR_HIP;
+    if (rc == GD_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = GD_ERR_HIP;
+    dlap.release();
+    tmpL.release();
+    tmpS.release();
+    tmpLU.release();
+    GD_TRY(rc);
+    GD_CHECK(stv == 0, GD_ERR_NUMERIC, "gd_step_create: zero or non-finite pivot in LU(S_l)");
+    // state and work buffers
+    const size_t esz = d->prec == GD_PREC_FP64 ? sizeof(double) : sizeof(float);
+    const int nm = d->mmax + 1;
+    GD_TRY(s->state.ensure(sizeof(double) * s->fs * 10));
+    GD_TRY(s->nlbuf[0].ensure(sizeof(double) * s->fs * 10));
+    GD_TRY(s->nlbuf[1].ensure(sizeof(double) * s->fs * 10));
+    GD_TRY(s->synV.ensure(esz * s->fs * 18));
+    GD_TRY(s->synS.ensure(esz * s->fs * 10));
+    GD_TRY(s->fb.ensure(esz * (size_t)s->sht.nlat2 * nr * nm * 60));
+    GD_TRY(s->gb.ensure(esz * (size_t)nm * s->sht.nlat2 * nr * 20));
+    GD_TRY(s->part.ensure(sizeof(double) * 3 * 512));
+    GD_TRY(s->eout.ensure(sizeof(double) * 4));
+    GD_HIP(hipMemsetAsync(s->state.p, 0, sizeof(double) * s->fs * 10, ctx->stream));
+    GD_HIP(hipMemsetAsync(s->nlbuf[0].p, 0, sizeof(double) * s->fs * 10, ctx->stream));
+    GD_HIP(hipMemsetAsync(s->nlbuf[1].p, 0, sizeof(double) * s->fs * 10, ctx->stream));
+    GD_HIP(hipStreamSynchronize(ctx->stream));
+    s->step = 0;
+    return GD_OK;
+}
+
+extern "C" int gd_step_create(gd_ctx *ctx, const gd_step_desc *desc, gd_step **out)
+{
+    GD_CHECK(ctx && desc && out, GD_ERR_ARG, "gd_step_create: null argument");
+    gd_step *s = new gd_step();
+    int rc = step_build(ctx, desc, s);
+    if (rc != GD_OK) {
+        step_release(s);
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return GD_OK;
+}
+
+extern "C" int gd_step_destroy(gd_step *s)
+{
+    if (!s) return GD_OK;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    step_release(s);
+    delete s;
+    return GD_OK;
+}
+
+// l-major user layout <-> m-major device planes
+static int convert_state(gd_step *s, const double *re, const double *im, double *dev, bool to_device)
+{
+    const int nr = s->d.nr, nlm = s->nlm;
+    const int *map = to_device ? s->sht.d_lmaj_of_idx.as<int>() : s->sht.d_idx_of_lmaj.as<int>();
+    for (int f = 0; f < 5; ++f) {
+        const size_t o = (size_t)f * s->fs;
+        if (to_device) {
+            GD_TRY(gd_launch_perm(s->ctx->stream, re + o, map, nlm, nr, dev + (2 * f) * s->fs));
+            GD_TRY(gd_launch_perm(s->ctx->stream, im + o, map, nlm, nr, dev + (2 * f + 1) * s->fs));
+        } else {
+            GD_TRY(gd_launch_perm(s->ctx->stream, dev + (2 * f) * s->fs, map, nlm, nr, const_cast<double *>(re) + o));
+            GD_TRY(gd_launch_perm(s->ctx->stream, dev + (2 * f + 1) * s->fs, map, nlm, nr, const_cast<double *>(im) + o));
+        }
+    }
+    return GD_OK;
+}
+
+extern "C" int gd_step_set_state(gd_step *s, const double *re, const double *im, int flags)
+{
+    GD_CHECK(s && re && im, GD_ERR_ARG, "gd_step_set_state: null argument");
+    GD_HIP(hipSetDevice(s->ctx->device));
+    const size_t n = s->fs * 5;
+    const double *a = re, *b = im;
+    if (flags & GD_HOST) {
+        GD_TRY(s->scratch.ensure(sizeof(double) * n * 2));
+        GD_HIP(hipMemcpyAsync(s->scratch.p, re, sizeof(double) * n, hipMemcpyHostToDevice, s->ctx->stream));
+        GD_HIP(hipMemcpyAsync(s->scratch.as<double>() + n, im, sizeof(double) * n, hipMemcpyHostToDevice,
+                              s->ctx->stream));
+        a = s->scratch.as<double>();
+        b = a + n;
+    }
+    GD_TRY(convert_state(s, a, b, s->state.as<double>(), true));
+    if (flags & GD_HOST) GD_HIP(hipStreamSynchronize(s->ctx->stream));
+    s->step = 0;
+    return GD_OK;
+}
+
+static int get_planes(gd_step *s, const double *dev, double *re, double *im, int flags)
+{
+    GD_HIP(hipSetDevice(s->ctx->device));
+    const size_t n = s->fs * 5;
+    double *a = re, *b = im;
+    if (flags & GD_HOST) {
+        GD_TRY(s->scratch.ensure(sizeof(double) * n * 2));
+        a = s->scratch.as<double>();
+        b = a + n;
+    }
+    GD_TRY(convert_state(s, a, b, const_cast<double *>(dev), false));
+    if (flags & GD_HOST) {
+        GD_TRY(gd_from_device(s->ctx, re, a, sizeof(double) * n));
+        GD_TRY(gd_from_device(s->ctx, im, b, sizeof(double) * n));
+    }
+    return GD_OK;
+}
+
+extern "C" int gd_step_get_state(gd_step *s, double *re, double *im, int flags)
+{
+    GD_CHECK(s && re && im, GD_ERR_ARG, "gd_step_get_state: null argument");
+    return get_planes(s, s->state.as<double>(), re, im, flags);
+}
+
+extern "C" int gd_step_nonlinear(gd_step *s, double *nl_re, double *nl_im, int flags)
+{
+    GD_CHECK(s && nl_re && nl_im, GD_ERR_ARG, "gd_step_nonlinear: null argument");
+    GD_HIP(hipSetDevice(s->ctx->device));
+    // the buffer the next step would fill; the history buffer is untouched
+    double *cur = s->nlbuf[s->step & 1].as<double>();
+    GD_TRY(nonlinear(s, cur));
+    return get_planes(s, cur, nl_re, nl_im, flags);
+}
+
+extern "C" int gd_step_run(gd_step *s, int nsteps)
+{
+    GD_CHECK(s && nsteps >= 0, GD_ERR_ARG, "gd_step_run: bad args");
+    GD_HIP(hipSetDevice(s->ctx->device));
+    for (int k = 0; k < nsteps; ++k) {
+        GD_TRY(enqueue_step(s, s->step & 1, s->step == 0));
+        ++s->step;
+    }
+    return GD_OK;
+}
+
+static int capture(gd_step *s, int parity)
+{
+    hipStream_t st = s->ctx->stream;
+    GD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_step(s, parity, false);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &g);
+    GD_TRY(rc);
+    if (e != hipSuccess) {
+        gd_set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
+        return GD_ERR_HIP;
+    }
+    s->graph[parity] = g;
+    GD_HIP(hipGraphInstantiate(&s->gexec[parity], g, nullptr, nullptr, 0));
+    return GD_OK;
+}
+
+extern "C" int gd_step_run_graph(gd_step *s, int nsteps)
+{
+    GD_CHECK(s && nsteps >= 0, GD_ERR_ARG, "gd_step_run_graph: bad args");
+    GD_HIP(hipSetDevice(s->ctx->device));
+    for (int k = 0; k < nsteps; ++k) {
+        if (s->step == 0) {
+            GD_TRY(enqueue_step(s, 0, true)); // AB1 bootstrap is not part of the graphs
+        } else {
+            const int par = s->step & 1;
+            if (!s->gexec[par]) GD_TRY(capture(s, par));
+            GD_HIP(hipGraphLaunch(s->gexec[par], s->ctx->stream));
+        }
+        ++s->step;
+    }
+    return GD_OK;
+}
+
+extern "C" int gd_step_count(const gd_step *s) { return s ? s->step : -1; }
+
+extern "C" int gd_step_energies(gd_step *s, double *out3)
+{
+    GD_CHECK(s && out3, GD_ERR_ARG, "gd_step_energies: null argument");
+    GD_HIP(hipSetDevice(s->ctx->device));
+    GD_TRY(gd_launch_energies(s->ctx->stream, s->state.as<double>(), s->sht.d_l_of_idx.as<int>(), s->nlm, s->d.nr,
+                              s->part.as<double>(), 512, s->eout.as<double>()));
+    return gd_from_device(s->ctx, out3, s->eout.p, sizeof(double) * 3);
+}

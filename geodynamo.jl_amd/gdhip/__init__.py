@@ -1,0 +1,21 @@
+"""gdhip -- MI355X-native pseudo-spectral CNAB2 step behind GeoDynamo.jl's
+transform and timestep entry points.
+
+The compute path is libgdhip.so (hand-written HIP for gfx950, C ABI in
+include/gdhip.h).  Names mirror the reference's Julia functions with the
+trailing '!' dropped.
+"""
+from ._lib import GDError, MissingLError, lib  # noqa: F401
+from .fields import (RadialDomain, SHTnsPhysicalField, SHTnsSpectralField,  # noqa: F401
+                     SHTnsVectorField, create_radial_domain)
+from .linear_algebra import (BandedLU, BandedMatrix, apply_banded_matrix, banded_to_dense,  # noqa: F401
+                             create_derivative_matrix, create_radial_laplacian, factorize_banded,
+                             solve_banded)
+from .shtnskit_field_functions import (batch_shtnskit_transforms, batch_spectral_to_physical,  # noqa: F401
+                                       shtnskit_physical_to_spectral, shtnskit_spectral_to_physical,
+                                       shtnskit_vector_analysis, shtnskit_vector_synthesis)
+from .shtnskit_transforms import SHTnsKitConfig, create_shtnskit_config  # noqa: F401
+from .step import CNAB2Step  # noqa: F401
+from .timestep import (SHTnsImplicitMatrices, apply_explicit_operator, build_rhs_cnab2,  # noqa: F401
+                       compute_timestep_error, create_shtns_timestepping_matrices,
+                       solve_implicit_step)

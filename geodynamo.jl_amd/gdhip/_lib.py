@@ -1,0 +1,115 @@
+"""ctypes binding of libgdhip.so (the C ABI declared in include/gdhip.h).
+
+The library must be built in-tree (``python geodynamo.jl_amd/gdhip/_build.py``
+or ``__graft_entry__.build()``); loading fails loudly when it is missing.
+There is no CPU fallback.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgdhip.so")
+
+GD_OK, GD_ERR_ARG, GD_ERR_HIP, GD_ERR_NUMERIC, GD_ERR_MISSING_L, GD_ERR_NOMEM = range(6)
+GD_PREC_FP64, GD_PREC_FP32, GD_PREC_BF16X3 = 0, 1, 2
+GD_HOST = 1
+GD_FD_LITERAL, GD_FD_TRANSPOSE = 0, 1
+
+# every symbol include/gdhip.h declares
+EXPORTS = [
+    "gd_last_error", "gd_version", "gd_ctx_create", "gd_ctx_set_stream", "gd_ctx_synchronize",
+    "gd_ctx_destroy", "gd_radial_grid", "gd_derivative_matrix", "gd_radial_laplacian",
+    "gd_banded_factor", "gd_banded_solve", "gd_cnab2_mats_create", "gd_cnab2_mats_get",
+    "gd_cnab2_mats_destroy", "gd_build_rhs_cnab2", "gd_solve_implicit_step", "gd_cnab2_rhs_solve",
+    "gd_sht_plan_create", "gd_sht_plan_destroy", "gd_sht_nlm", "gd_sht_grid", "gd_sht_tables",
+    "gd_sht_scalar_synth", "gd_sht_scalar_analysis", "gd_sht_vector_synth", "gd_sht_vector_analysis",
+    "gd_step_create", "gd_step_destroy", "gd_step_set_state", "gd_step_get_state",
+    "gd_step_nonlinear", "gd_step_run", "gd_step_run_graph", "gd_step_count", "gd_step_energies",
+    "gd_timestep_error",
+]
+
+
+class GDError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("gdhip error %d: %s" % (code, msg))
+        self.code = code
+
+
+class MissingLError(GDError):
+    pass
+
+
+class StepDesc(ctypes.Structure):
+    _fields_ = [("E", ctypes.c_double), ("Pm", ctypes.c_double), ("Pr", ctypes.c_double),
+                ("Ra", ctypes.c_double), ("rratio", ctypes.c_double), ("dt", ctypes.c_double),
+                ("theta", ctypes.c_double),
+                ("lmax", ctypes.c_int), ("mmax", ctypes.c_int), ("nlat", ctypes.c_int),
+                ("nlon", ctypes.c_int), ("nr", ctypes.c_int), ("bw", ctypes.c_int),
+                ("prec", ctypes.c_int), ("fd_mode", ctypes.c_int), ("radials", ctypes.c_int),
+                ("coriolis_index", ctypes.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libgdhip.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i, d, l = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_long
+        pp = ctypes.POINTER(ctypes.c_void_p)
+        sig = {
+            "gd_last_error": (ctypes.c_char_p, []),
+            "gd_version": (ctypes.c_char_p, []),
+            "gd_ctx_create": (i, [i, pp]),
+            "gd_ctx_set_stream": (i, [vp, vp]),
+            "gd_ctx_synchronize": (i, [vp]),
+            "gd_ctx_destroy": (i, [vp]),
+            "gd_radial_grid": (i, [i, d, vp]),
+            "gd_derivative_matrix": (i, [i, i, i, vp, i, vp]),
+            "gd_radial_laplacian": (i, [i, i, vp, i, vp]),
+            "gd_banded_factor": (i, [vp, i, i, i, vp, vp, i]),
+            "gd_banded_solve": (i, [vp, i, i, i, vp, vp, i, vp, i]),
+            "gd_cnab2_mats_create": (i, [vp, i, i, i, vp, d, d, d, i, pp]),
+            "gd_cnab2_mats_get": (i, [vp, vp, vp, vp]),
+            "gd_cnab2_mats_destroy": (i, [vp]),
+            "gd_build_rhs_cnab2": (i, [vp, i, i, i] + [vp] * 8 + [i]),
+            "gd_solve_implicit_step": (i, [vp, i, i, i] + [vp] * 4 + [i]),
+            "gd_cnab2_rhs_solve": (i, [vp, i, i, i] + [vp] * 6 + [i]),
+            "gd_sht_plan_create": (i, [vp, i, i, i, i, i, pp]),
+            "gd_sht_plan_destroy": (i, [vp]),
+            "gd_sht_nlm": (i, [vp]),
+            "gd_sht_grid": (i, [vp, vp, vp, vp]),
+            "gd_sht_tables": (i, [vp, vp, vp]),
+            "gd_sht_scalar_synth": (i, [vp, vp, vp, i, vp, i]),
+            "gd_sht_scalar_analysis": (i, [vp, vp, i, vp, vp, i]),
+            "gd_sht_vector_synth": (i, [vp, vp, vp, vp, vp, i, vp, vp, i]),
+            "gd_sht_vector_analysis": (i, [vp, vp, vp, i, vp, vp, vp, vp, i]),
+            "gd_step_create": (i, [vp, ctypes.POINTER(StepDesc), pp]),
+            "gd_step_destroy": (i, [vp]),
+            "gd_step_set_state": (i, [vp, vp, vp, i]),
+            "gd_step_get_state": (i, [vp, vp, vp, i]),
+            "gd_step_nonlinear": (i, [vp, vp, vp, i]),
+            "gd_step_run": (i, [vp, i]),
+            "gd_step_run_graph": (i, [vp, i]),
+            "gd_step_count": (i, [vp]),
+            "gd_step_energies": (i, [vp, vp]),
+            "gd_timestep_error": (i, [vp, l, vp, vp, vp, vp, vp, i]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != GD_OK:
+        msg = lib().gd_last_error().decode(errors="replace")
+        if rc == GD_ERR_MISSING_L:
+            raise MissingLError(rc, msg)
+        raise GDError(rc, msg)
+    return rc

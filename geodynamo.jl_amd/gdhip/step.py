@@ -1,0 +1,84 @@
+"""The device-resident CNAB2 step (the step of scripts/bench_timestepping.jl:6-18).
+
+Field order: u_tor, u_pol, B_tor, B_pol, T.  State arrays are (5, nr, nlm)
+float64 re/im in the reference's l-major spectral layout.
+"""
+import ctypes
+
+import numpy as np
+
+from ._arrays import context, prepare
+from ._lib import (GD_FD_LITERAL, GD_FD_TRANSPOSE, StepDesc, check, lib)
+from .shtnskit_transforms import PRECISIONS
+
+FIELDS = ("u_tor", "u_pol", "B_tor", "B_pol", "T")
+
+
+class CNAB2Step:
+    def __init__(self, lmax, nlat, nlon, nr, E=1e-3, Pm=5.0, Pr=1.0, Ra=100.0, rratio=0.35, dt=1e-5,
+                 theta=0.5, bw=3, mmax=None, precision="fp64", literal_fd=False, radials=True,
+                 coriolis_index=0, device=0):
+        mmax = lmax if mmax is None else mmax
+        self.desc = StepDesc(E, Pm, Pr, Ra, rratio, dt, theta, lmax, mmax, nlat, nlon, nr, bw,
+                             PRECISIONS[precision], GD_FD_LITERAL if literal_fd else GD_FD_TRANSPOSE,
+                             int(radials), int(coriolis_index))
+        self.lmax, self.mmax, self.nlat, self.nlon, self.nr = lmax, mmax, nlat, nlon, nr
+        self.nlm = sum(min(l, mmax) + 1 for l in range(lmax + 1))
+        self.device = device
+        self._h = ctypes.c_void_p()
+        check(lib().gd_step_create(context(device), ctypes.byref(self.desc), ctypes.byref(self._h)))
+
+    @property
+    def shape(self):
+        return (5, self.nr, self.nlm)
+
+    def set_state(self, re, im):
+        ctx, flags, (a, b) = prepare(re, im, device=self.device)
+        check(lib().gd_step_set_state(self._h, a, b, flags))
+
+    def set_state_complex(self, st):
+        self.set_state(np.ascontiguousarray(st.real), np.ascontiguousarray(st.imag))
+
+    def get_state(self, re=None, im=None):
+        if re is None:
+            re, im = np.zeros(self.shape), np.zeros(self.shape)
+        ctx, flags, (a, b) = prepare(re, im, device=self.device)
+        check(lib().gd_step_get_state(self._h, a, b, flags))
+        return re, im
+
+    def get_state_complex(self):
+        re, im = self.get_state()
+        return re + 1j * im
+
+    def nonlinear(self):
+        re, im = np.zeros(self.shape), np.zeros(self.shape)
+        ctx, flags, (a, b) = prepare(re, im, device=self.device)
+        check(lib().gd_step_nonlinear(self._h, a, b, flags))
+        return re + 1j * im
+
+    def run(self, nsteps=1, graph=False):
+        check((lib().gd_step_run_graph if graph else lib().gd_step_run)(self._h, nsteps))
+
+    def synchronize(self):
+        check(lib().gd_ctx_synchronize(context(self.device)))
+
+    @property
+    def step_count(self):
+        return lib().gd_step_count(self._h)
+
+    def energies(self):
+        """(KE, ME, thermal) in FP64 (magnetic.jl:504-543, thermal.jl:413-438)."""
+        out = np.zeros(3)
+        check(lib().gd_step_energies(self._h, out.ctypes.data))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().gd_step_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

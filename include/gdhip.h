@@ -1,0 +1,170 @@
+/*
+ * gdhip.h -- C ABI of the MI355X-native pseudo-spectral CNAB2 step.
+ *
+ * Plain pointers and sizes only.  Every entry point returns 0 on success or a
+ * GD_ERR_* code; gd_last_error() returns a thread-local message for the last
+ * failure on the calling thread.  Unless GD_HOST is set in `flags`, array
+ * arguments are device pointers and work is queued on the context's stream
+ * (asynchronous); with GD_HOST the arrays are host memory, copied in and out,
+ * and the call returns after the results are in host memory.  The library
+ * never retains caller pointers after a call returns.
+ *
+ * Layouts are the reference's (GeoDynamo.jl) boundary layouts:
+ *   spectral  re/im planes of (nlm, 1, nr) column-major, l-major lm order:
+ *             element (lm, r) at lm + nlm*r                 (fields.jl:6-15, shtnskit_transforms.jl:177-188)
+ *   physical  (nlat, nlon, nr) column-major, colatitude fastest, north first
+ *                                                           (fields.jl:18-24)
+ *   band      (2bw+1, N) column-major, A[i][j] = ab[bw+i-j + (2bw+1)*j]
+ *                                                           (linear_algebra.jl:3-21)
+ *   radial    r table (N, 7) column-major, column p = r^(p-4) (fields.jl:108-131)
+ *   step      5 fields in the order u_tor, u_pol, B_tor, B_pol, T, each a
+ *             spectral (nlm, nr) block: field f at offset f*nlm*nr.
+ */
+#ifndef GDHIP_H
+#define GDHIP_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    GD_OK = 0,
+    GD_ERR_ARG = 1,      /* bad argument / shape */
+    GD_ERR_HIP = 2,      /* HIP runtime failure */
+    GD_ERR_NUMERIC = 3,  /* zero or non-finite pivot in a no-pivot band LU */
+    GD_ERR_MISSING_L = 4,/* an lm column's l has no implicit matrix (timestep.jl:646) */
+    GD_ERR_NOMEM = 5
+};
+enum { GD_PREC_FP64 = 0, GD_PREC_FP32 = 1, GD_PREC_BF16X3 = 2 };
+enum { GD_HOST = 1 };
+enum { GD_FD_TRANSPOSE = 1, GD_FD_LITERAL = 0 };
+
+typedef struct gd_ctx gd_ctx;
+typedef struct gd_sht_plan gd_sht_plan;
+typedef struct gd_cnab2_mats gd_cnab2_mats;
+typedef struct gd_step gd_step;
+
+const char *gd_last_error(void);
+const char *gd_version(void);
+
+/* Device context: one HIP device and one stream (default priority). */
+int gd_ctx_create(int device, gd_ctx **out);
+/* Use an existing stream (e.g. the caller's framework stream); 0 = library's own. */
+int gd_ctx_set_stream(gd_ctx *ctx, void *hip_stream);
+int gd_ctx_synchronize(gd_ctx *ctx);
+int gd_ctx_destroy(gd_ctx *ctx);
+
+/* ---- radial domain and FD operators (host FP64 set-up) -------------------
+ * replaces create_radial_domain (fields.jl:108-131),
+ * create_derivative_matrix (linear_algebra.jl:90-132),
+ * create_radial_laplacian (linear_algebra.jl:134-151).
+ * fd_mode GD_FD_TRANSPOSE solves V^T w = k! e_k; GD_FD_LITERAL is V \ rhs. */
+int gd_radial_grid(int nr, double rratio, double *r /* nr x 7 */);
+int gd_derivative_matrix(int order, int nr, int bw, const double *r, int fd_mode, double *band);
+int gd_radial_laplacian(int nr, int bw, const double *r, int fd_mode, double *band);
+
+/* ---- banded LU without pivoting (FP64, GPU) --------------------------------
+ * replaces factorize_banded (linear_algebra.jl:23-57) for nsys systems
+ * (ab and lu: nsys consecutive (2bw+1) x n bands) and solve_banded!
+ * (linear_algebra.jl:59-88) for nrhs right-hand sides x (n x nrhs, column
+ * j uses system sys_of_rhs[j]; x may alias b).  Factor returns
+ * GD_ERR_NUMERIC if a pivot is zero or non-finite. */
+int gd_banded_factor(gd_ctx *ctx, int n, int bw, int nsys, const double *ab, double *lu, int flags);
+int gd_banded_solve(gd_ctx *ctx, int n, int bw, int nsys, const double *lu,
+                    const int *sys_of_rhs, int nrhs, double *x, int flags);
+
+/* ---- CNAB2 implicit operators (FP64, GPU) ---------------------------------
+ * replaces create_shtns_timestepping_matrices (timestep.jl:498-543): for
+ * l = 0..lmax, L_l = nu(Lap - l(l+1)/r^2), S_l = I/dt - theta L_l, LU(S_l);
+ * matrices stay device resident. */
+int gd_cnab2_mats_create(gd_ctx *ctx, int lmax, int nr, int bw, const double *r, double nu,
+                         double dt, double theta, int fd_mode, gd_cnab2_mats **out);
+int gd_cnab2_mats_get(gd_cnab2_mats *M, double *L, double *S, double *LU /* host, (lmax+1) bands each */);
+int gd_cnab2_mats_destroy(gd_cnab2_mats *M);
+/* replaces build_rhs_cnab2! (timestep.jl:610-690):
+ * rhs = u/dt + 1.5 nl - 0.5 nl_prev + (1-theta) L_l u, for spectral fields with
+ * the given (lmax, mmax); columns whose l exceeds the matrices' lmax give
+ * GD_ERR_MISSING_L, as the reference errors. */
+int gd_build_rhs_cnab2(gd_cnab2_mats *M, int lmax, int mmax, int nr,
+                       const double *u_re, const double *u_im,
+                       const double *nl_re, const double *nl_im,
+                       const double *np_re, const double *np_im,
+                       double *rhs_re, double *rhs_im, int flags);
+/* replaces solve_implicit_step! (timestep.jl:692-730): sol = S_l^{-1} rhs per
+ * (lm, re/im); columns whose l has no matrix are skipped, as the reference does. */
+int gd_solve_implicit_step(gd_cnab2_mats *M, int lmax, int mmax, int nr,
+                           const double *rhs_re, const double *rhs_im,
+                           double *sol_re, double *sol_im, int flags);
+/* fused build_rhs_cnab2! + solve_implicit_step! in place on u. */
+int gd_cnab2_rhs_solve(gd_cnab2_mats *M, int lmax, int mmax, int nr,
+                       double *u_re, double *u_im,
+                       const double *nl_re, const double *nl_im,
+                       const double *np_re, const double *np_im, int flags);
+
+/* ---- spherical-harmonic transforms ----------------------------------------
+ * replaces create_shtnskit_config (shtnskit_transforms.jl:102-200):
+ * orthonormal Y_lm with Condon-Shortley phase on a Gauss grid (nlat even,
+ * nlon > 2 mmax, nlon = 2^a 3^b), P_lm by FP64 recurrence on the GPU.
+ * prec selects the Legendre contraction / FFT arithmetic. */
+int gd_sht_plan_create(gd_ctx *ctx, int lmax, int mmax, int nlat, int nlon, int prec,
+                       gd_sht_plan **out);
+int gd_sht_plan_destroy(gd_sht_plan *p);
+int gd_sht_nlm(const gd_sht_plan *p);
+/* host copies of the grid (cos theta, Gauss weights, theta) and FP64 tables
+ * P_lm, dP_lm/dtheta in l-major (nlm x nlat) */
+int gd_sht_grid(gd_sht_plan *p, double *x, double *w, double *theta);
+int gd_sht_tables(gd_sht_plan *p, double *P, double *dP);
+/* replaces shtnskit_spectral_to_physical! (shtnskit_field_functions.jl:11-21) */
+int gd_sht_scalar_synth(gd_sht_plan *p, const double *re, const double *im, int nr,
+                        double *phys, int flags);
+/* replaces shtnskit_physical_to_spectral! (shtnskit_field_functions.jl:131-141);
+ * imag(m=0) := 0 as the reference's store does (:432-435) */
+int gd_sht_scalar_analysis(gd_sht_plan *p, const double *phys, int nr,
+                           double *re, double *im, int flags);
+/* replaces shtnskit_vector_synthesis! (shtnskit_field_functions.jl:252-282):
+ * (v_theta, v_phi) from spheroidal S := pol and toroidal T := tor */
+int gd_sht_vector_synth(gd_sht_plan *p, const double *tor_re, const double *tor_im,
+                        const double *pol_re, const double *pol_im, int nr,
+                        double *vt, double *vp, int flags);
+/* replaces shtnskit_vector_analysis! (shtnskit_field_functions.jl:291-321) */
+int gd_sht_vector_analysis(gd_sht_plan *p, const double *vt, const double *vp, int nr,
+                           double *tor_re, double *tor_im, double *pol_re, double *pol_im,
+                           int flags);
+
+/* ---- the device-resident CNAB2 step ----------------------------------------
+ * replaces the bench step of scripts/bench_timestepping.jl:6-18: the
+ * nonlinear terms of thermal.jl:149-190, velocity.jl:312-332 and
+ * magnetic.jl:131-159, then apply_master_implicit_step!
+ * (simulation.jl:1151-1330). */
+typedef struct {
+    double E, Pm, Pr, Ra, rratio, dt, theta;
+    int lmax, mmax, nlat, nlon, nr, bw;
+    int prec;           /* GD_PREC_*: Legendre/FFT/products arithmetic; solves stay FP64 */
+    int fd_mode;        /* GD_FD_TRANSPOSE (default) or GD_FD_LITERAL */
+    int radials;        /* 1: radial components from the poloidal scalars (default) */
+    int coriolis_index; /* 0: reference indexing theta_{min(j,nlat)}; 1: theta_i */
+} gd_step_desc;
+
+int gd_step_create(gd_ctx *ctx, const gd_step_desc *desc, gd_step **out);
+int gd_step_destroy(gd_step *s);
+/* state: 5 x (nlm, nr) re/im in the step layout above; resets the step counter */
+int gd_step_set_state(gd_step *s, const double *re, const double *im, int flags);
+int gd_step_get_state(gd_step *s, double *re, double *im, int flags);
+/* evaluates the nonlinear terms of the current state (no time advance) */
+int gd_step_nonlinear(gd_step *s, double *nl_re, double *nl_im, int flags);
+/* advances nsteps CNAB2 steps (AB1 bootstrap at step 0, as simulation.jl:1153) */
+int gd_step_run(gd_step *s, int nsteps);
+/* same, replaying a captured hipGraph of one step */
+int gd_step_run_graph(gd_step *s, int nsteps);
+int gd_step_count(const gd_step *s);
+/* FP64 energies: KE and ME (1/2 sum (T^2+P^2)/max(l(l+1),1), magnetic.jl:504-543)
+ * and thermal 1/2 sum |T|^2 (thermal.jl:413-438); host out[3] */
+int gd_step_energies(gd_step *s, double *out3);
+/* replaces compute_timestep_error (timestep.jl:733-753) on device arrays */
+int gd_timestep_error(gd_ctx *ctx, long n, const double *a_re, const double *a_im,
+                      const double *b_re, const double *b_im, double *out, int flags);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
