@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""CNAB2 steps/s at lmax=127, nr=65: mixed-precision vs FP64 GPU path.
+
+Workload (one MI355X): the full pseudo-spectral CNAB2 step of the rotating
+Boussinesq MHD shell, lmax=mmax=127 on a 192 x 384 Gauss grid, nr=65, radius
+ratio 0.35, E=1e-3, Pr=1, Pm=5, Ra=500, dt=1e-5, band half-width 3; seeded
+(l+1)^-2 state of amplitude 1e-3.  A "step" is one CNAB2 step of all five
+fields with the state resident in HBM.
+
+  value          mixed path (FP32 Legendre / FFT / products, FP64 radial
+                 solves and state) steps/s over --steps graph-replayed steps
+  fp64_*         the all-FP64 GPU path, same protocol
+  accuracy       per-field max relative error of the mixed state against the
+                 FP64 GPU state after --err-steps steps, and of the FP64 GPU
+                 state against the FP64 CPU oracle after one step
+  roofline       dominant kernel of the mixed step, its algorithmic bytes or
+                 flops per launch over its HIP-event duration
+  cpu_baseline   the FP64 oracle step (restatement of the reference step) on
+                 the host cores, a bounded sample of steps
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "geodynamo.jl_amd"))
+
+METRIC = "CNAB2 steps/s at lmax=127 nr=65, mixed vs FP64 GPU at ≤1e-6 rel err"
+CFG = dict(lmax=127, nlat=192, nlon=384, nr=65, E=1e-3, Pm=5.0, Pr=1.0, Ra=500.0, rratio=0.35,
+           dt=1e-5, theta=0.5, bw=3)
+SEED, AMP = 2024, 1e-3
+PEAK = {"hbm": 8000.0, "fp32": 157.3, "fp64": 78.6}  # GB/s, TFLOP/s (MI355X dense)
+
+
+def kernel_costs(prec_bytes):
+    """Algorithmic bytes and flops per launch of each kernel class of one step.
+
+    Legendre flops count 2*nlat*nlm per scalar-equivalent contraction per level
+    (the even/odd-symmetric count): synthesis 18 vector + 5 scalar, analysis
+    8 vector + 1 scalar.  Bytes are the compulsory reads and writes of each
+    kernel's HBM operands (tables excluded: they stay in L2/MALL)."""
+    L, nlat, nr = CFG["lmax"], CFG["nlat"], CFG["nr"]
+    nlm = (L + 1) * (L + 2) // 2
+    nm, nlat2 = L + 1, nlat // 2
+    es = prec_bytes
+    fs = nlm * nr
+    four = nlat2 * nr * nm  # Fourier (i, level, m) points
+    contraction = 2.0 * nlat * nlm * nr
+    return {
+        "spectral_pre": dict(bytes=5 * 16 * fs + 28 * es * fs, flops=0.0),
+        "synth_vector": dict(bytes=18 * es * fs + 40 * es * four, flops=18 * contraction),
+        "synth_scalar": dict(bytes=10 * es * fs + 20 * es * four, flops=5 * contraction),
+        "ring_fft_products": dict(bytes=60 * es * four + 20 * es * four, flops=0.0),
+        "analysis_vector": dict(bytes=16 * es * four + 4 * 16 * fs, flops=8 * contraction),
+        "analysis_scalar": dict(bytes=4 * es * four + 16 * fs, flops=1 * contraction),
+        "cnab2_solve": dict(bytes=5 * 64 * fs, flops=0.0),
+    }
+
+
+def roofline(prof_ms, prec):
+    es = 8 if prec == "fp64" else 4
+    costs = kernel_costs(es)
+    name = max(prof_ms, key=prof_ms.get)
+    t = prof_ms[name] * 1e-3
+    c = costs[name]
+    peak_f = PEAK["fp64" if prec == "fp64" else "fp32"]
+    ridge = peak_f * 1e12 / (PEAK["hbm"] * 1e9)
+    mfma = c["flops"] > 0 and c["flops"] / c["bytes"] > ridge
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", "pmc_%s_%s.json" % (name, prec))
+    if os.path.exists(pmc):
+        with open(pmc) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    if mfma:
+        ach = c["flops"] / t / 1e12
+        return name, {"bound": "mfma", "achieved": round(ach, 3), "peak": peak_f, "unit": "TFLOP/s",
+                      "frac": round(ach / peak_f, 4), "traffic": traffic, "kernel": name,
+                      "kernel_ms": round(prof_ms[name], 5), "algorithmic_flops": c["flops"],
+                      "algorithmic_bytes": c["bytes"]}
+    ach = c["bytes"] / t / 1e9
+    return name, {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK["hbm"], "unit": "GB/s",
+                  "frac": round(ach / PEAK["hbm"], 4), "traffic": traffic, "kernel": name,
+                  "kernel_ms": round(prof_ms[name], 5), "algorithmic_bytes": c["bytes"]}
+
+
+def rel_err(a, b):
+    import numpy as np
+
+    return [float(np.abs(a[k] - b[k]).max() / max(np.abs(b[k]).max(), 1e-300)) for k in range(5)]
+
+
+def timed(step, st, steps, warmup):
+    step.set_state_complex(st)
+    step.run(warmup, graph=True)
+    step.synchronize()
+    t0 = time.perf_counter()
+    step.run(steps, graph=True)
+    step.synchronize()
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(st, n_steps, fp64_one_step):
+    """FP64 oracle step on the host cores (bounded sample)."""
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    m = O.Model(CFG["lmax"], CFG["nlat"], CFG["nlon"], CFG["nr"], E=CFG["E"], Pm=CFG["Pm"], Pr=CFG["Pr"],
+                Ra=CFG["Ra"], rratio=CFG["rratio"], dt=CFG["dt"], theta=CFG["theta"], bw=CFG["bw"])
+    t0 = time.perf_counter()
+    s, nl, nlp = m.step(st, None, 0)
+    first = s
+    for k in range(1, n_steps):
+        s, nl, nlp = m.step(s, nlp, k)
+    dt = time.perf_counter() - t0
+    err = rel_err(fp64_one_step, first)
+    return {"value": round(n_steps / dt, 4), "unit": "steps/s", "cores": cores, "kind": "port",
+            "sample": "%d CNAB2 steps of the FP64 oracle at lmax=127 nr=65 (%.1f s)" % (n_steps, dt)}, err
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--err-steps", type=int, default=100)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build runs the CNAB2 step on a single MI355X "
+              "(the multi-GPU sharding is a design note only); rerun with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import gdhip
+
+    st = gdhip.seeded_state(SEED, CFG["lmax"], CFG["nr"], amp=AMP)
+    kw = dict(CFG)
+    lmax, nlat, nlon, nr = kw.pop("lmax"), kw.pop("nlat"), kw.pop("nlon"), kw.pop("nr")
+    mixed = gdhip.CNAB2Step(lmax, nlat, nlon, nr, precision="fp32", **kw)
+    fp64 = gdhip.CNAB2Step(lmax, nlat, nlon, nr, precision="fp64", **kw)
+
+    # accuracy: mixed vs FP64 GPU after err_steps; FP64 GPU after one step (oracle check below)
+    fp64.set_state_complex(st)
+    fp64.run(1)
+    one_step64 = fp64.get_state_complex()
+    fp64.run(args.err_steps - 1, graph=True)
+    ref = fp64.get_state_complex()
+    e_ref = fp64.energies()
+    mixed.set_state_complex(st)
+    mixed.run(args.err_steps, graph=True)
+    err_mixed = rel_err(mixed.get_state_complex(), ref)
+    e_mix = mixed.energies()
+
+    t64 = timed(fp64, st, args.steps, args.warmup)
+    tmx = timed(mixed, st, args.steps, args.warmup)
+    prof = mixed.profile(5)
+    prof64 = fp64.profile(5)
+    kname, roof = roofline(prof, "fp32")
+
+    out = {
+        "metric": METRIC,
+        "value": round(args.steps / tmx, 2),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(tmx / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (Legendre/FFT/products) + fp64 (state, radial solves)",
+        "data": "synthetic (seeded (l+1)^-2 state, amplitude 1e-3)",
+        "config": {"workload": "CNAB2 step lmax=mmax=127 nlat=192 nlon=384 nr=65 Ra=500 dt=1e-5, "
+                               "5 fields, mixed precision", "lmax": lmax, "nlat": nlat, "nlon": nlon,
+                   "nr": nr, "parallelism": "single GPU"},
+        "fp64_steps_per_s": round(args.steps / t64, 2),
+        "fp64_ms_per_step": round(t64 / args.steps * 1e3, 4),
+        "mixed_over_fp64": round(t64 / tmx, 3),
+        "accuracy": {"mixed_vs_fp64_max_rel_per_field_after_%d_steps" % args.err_steps:
+                     [float("%.3e" % e) for e in err_mixed],
+                     "energies_fp64": [float("%.10e" % e) for e in e_ref],
+                     "energies_mixed_rel_err": [float("%.3e" % (abs(a - b) / abs(b))) for a, b in zip(e_mix, e_ref)]},
+        "kernel_ms_mixed": {k: round(v, 5) for k, v in prof.items()},
+        "kernel_ms_fp64": {k: round(v, 5) for k, v in prof64.items()},
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        cb, err64 = cpu_baseline(st, args.cpu_steps, one_step64)
+        out["cpu_baseline"] = cb
+        out["accuracy"]["fp64_vs_oracle_max_rel_per_field_after_1_step"] = [float("%.3e" % e) for e in err64]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -276,3 +276,59 @@ extern "C" int gd_radial_laplacian(int nr, int bw, const double *r, int fd_mode,
     }
     return GD_OK;
 }
+
+// ---------------------------------------------------------------- seeded initial state
+// splitmix64 -> uniform (0,1) -> Box-Muller normals; draws in the order
+// field, level, lm (l-major), (re, im); value = amp * n * (l+1)^-power, m = 0
+// kept real, rows 0 and nr-1 zeroed when zero_boundaries (no-slip,
+// isothermal, insulating rows of the configurations).
+static unsigned long long splitmix64(unsigned long long *s)
+{
+    unsigned long long z = (*s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+extern "C" int gd_seeded_state(unsigned long long seed, int lmax, int mmax, int nr, int nfields, double amp,
+                               double power, int zero_boundaries, double *re, double *im)
+{
+    GD_CHECK(lmax >= 0 && mmax >= 0 && mmax <= lmax && nr >= 1 && nfields >= 1 && re && im, GD_ERR_ARG,
+             "gd_seeded_state: bad args");
+    const int nlm = gd_nlm_of(lmax, mmax);
+    std::vector<int> lo(nlm), mo(nlm);
+    int q = 0;
+    for (int l = 0; l <= lmax; ++l)
+        for (int m = 0; m <= (l < mmax ? l : mmax); ++m) {
+            lo[q] = l;
+            mo[q] = m;
+            ++q;
+        }
+    unsigned long long st = seed;
+    double spare = 0.0;
+    bool have = false;
+    auto normal = [&]() {
+        if (have) {
+            have = false;
+            return spare;
+        }
+        double u1 = ((double)(splitmix64(&st) >> 11) + 0.5) * 0x1p-53;
+        double u2 = ((double)(splitmix64(&st) >> 11) + 0.5) * 0x1p-53;
+        double rad = std::sqrt(-2.0 * std::log(u1));
+        spare = rad * std::sin(2.0 * M_PI * u2);
+        have = true;
+        return rad * std::cos(2.0 * M_PI * u2);
+    };
+    for (int f = 0; f < nfields; ++f)
+        for (int r = 0; r < nr; ++r)
+            for (int k = 0; k < nlm; ++k) {
+                const double sc = amp * std::pow((double)lo[k] + 1.0, -power);
+                const size_t o = (size_t)f * nlm * nr + (size_t)r * nlm + k;
+                double a = normal() * sc, b = normal() * sc;
+                if (mo[k] == 0) b = 0.0;
+                if (zero_boundaries && (r == 0 || r == nr - 1)) a = b = 0.0;
+                re[o] = a;
+                im[o] = b;
+            }
+    return GD_OK;
+}

@@ -77,8 +77,9 @@ template <typename T> struct Acc4;
 template <> struct Acc4<float> { using type = floatx4; };
 template <> struct Acc4<double> { using type = doublex4; };
 
-// 16x16x4 MFMA: lane l holds A[l%16][l/16], B[l/16][l%16];
-// D[4*(l/16)+v][l%16] in accumulator element v.
+// 16x16x4 MFMA: lane l holds A[l%16][l/16], B[l/16][l%16]; accumulator
+// element v of lane l holds D[row][l%16] with row = 4*(l/16)+v for f32 and
+// row = l/16 + 4*v for f64 (measured on gfx950).
 __device__ __forceinline__ floatx4 mfma16x16x4(float a, float b, floatx4 c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
@@ -87,6 +88,10 @@ __device__ __forceinline__ doublex4 mfma16x16x4(double a, double b, doublex4 c)
 {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
+
+template <typename T> __device__ __forceinline__ int acc_row(int lane, int v);
+template <> __device__ __forceinline__ int acc_row<float>(int lane, int v) { return 4 * (lane >> 4) + v; }
+template <> __device__ __forceinline__ int acc_row<double>(int lane, int v) { return (lane >> 4) + 4 * v; }
 
 template <typename T> struct Cplx { T re, im; };
 

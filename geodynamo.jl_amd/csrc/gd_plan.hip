@@ -513,7 +513,7 @@ extern "C" int gd_banded_solve(gd_ctx *ctx, int n, int bw, int nsys, const doubl
 }
 
 extern "C" int gd_cnab2_mats_create(gd_ctx *ctx, int lmax, int nr, int bw, const double *r, double nu, double dt,
-                                    double theta, int fd_mode, gd_cnab2_mats **out)
+                                    double theta, int fd_mode, int bc_rows, gd_cnab2_mats **out)
 {
     GD_CHECK(ctx && r && out && lmax >= 0 && nr >= 2 && bw >= 1 && bw <= 7 && dt > 0.0, GD_ERR_ARG,
              "gd_cnab2_mats_create: bad args");
@@ -529,6 +529,7 @@ extern "C" int gd_cnab2_mats_create(gd_ctx *ctx, int lmax, int nr, int bw, const
     M->nu = nu;
     M->dt = dt;
     M->theta = theta;
+    M->bc_rows = bc_rows ? 1 : 0;
     const size_t nb = (size_t)W * nr * nl;
     DevBuf dlap, dr2;
     int rc = upload(ctx, dlap, lap);
@@ -539,7 +540,7 @@ extern "C" int gd_cnab2_mats_create(gd_ctx *ctx, int lmax, int nr, int bw, const
     if (rc == GD_OK) rc = zero_status(ctx, M->status);
     if (rc == GD_OK)
         rc = gd_cnab2_build(ctx->stream, dlap.as<double>(), dr2.as<double>(), nu, dt, theta, nl, nr, bw,
-                            M->L.as<double>(), M->S.as<double>(), M->LU.as<double>(), nullptr, nullptr,
+                            M->bc_rows, M->L.as<double>(), M->S.as<double>(), M->LU.as<double>(), nullptr, nullptr,
                             M->status.as<int>());
     int st = 0;
     if (rc == GD_OK) rc = read_status(ctx, M->status, &st);
@@ -624,7 +625,8 @@ static int cnab2_call(gd_cnab2_mats *M, int lmax, int mmax, int nr, int mode, co
     }
     GD_TRY(zero_status(ctx, M->status));
     GD_TRY(gd_launch_cnab2_lmajor(ctx->stream, M->L.as<double>(), M->LU.as<double>(), M->lmax + 1, lmax, mmax, nr,
-                                  M->bw, M->dt, M->theta, mode, a, b, c, d, e, f, oa, ob, M->status.as<int>()));
+                                  M->bw, M->dt, M->theta, mode, M->bc_rows, a, b, c, d, e, f, oa, ob,
+                                  M->status.as<int>()));
     int st = 0;
     GD_TRY(read_status(ctx, M->status, &st));
     if (flags & GD_HOST) {

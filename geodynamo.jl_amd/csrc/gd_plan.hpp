@@ -19,7 +19,7 @@ struct gd_sht_plan {
 
 struct gd_cnab2_mats {
     gd_ctx *ctx = nullptr;
-    int lmax = 0, nr = 0, bw = 0;
+    int lmax = 0, nr = 0, bw = 0, bc_rows = 0;
     double nu = 0, dt = 0, theta = 0;
     DevBuf L, S, LU, status;
     DevBuf h0, h1, h2, h3, h4, h5, h6, h7;

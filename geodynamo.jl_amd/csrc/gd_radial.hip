@@ -90,10 +90,11 @@ __global__ void band_solve_kernel(const double *__restrict__ lu, const int *__re
 }
 
 // ---------------------------------------------------------------- CNAB2 assembly
-// L_l = nu*Lap; L_l[bw, n] -= nu l(l+1)/r^2; S_l = -theta L_l; S_l[bw,n] += 1/dt
+// L_l = nu*Lap; L_l[bw, n] -= nu l(l+1)/r^2; S_l = -theta L_l; S_l[bw,n] += 1/dt;
+// bc_rows: rows 0 and n-1 of S_l become Dirichlet rows (identity)
 __global__ void cnab2_assemble_kernel(const double *__restrict__ lap, const double *__restrict__ rinv2, double nu,
-                                      double dt, double theta, int nl, int n, int bw, double *__restrict__ L,
-                                      double *__restrict__ S, double *__restrict__ L_t)
+                                      double dt, double theta, int nl, int n, int bw, int bc_rows,
+                                      double *__restrict__ L, double *__restrict__ S, double *__restrict__ L_t)
 {
     const int w = 2 * bw + 1;
     long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -108,16 +109,18 @@ __global__ void cnab2_assemble_kernel(const double *__restrict__ lap, const doub
     if (L_t) L_t[(size_t)e * nl + l] = v;
     double s = v * (-theta);
     if (row == bw) s += 1.0 / dt;
+    const int i = col + row - bw; // matrix row of this band entry
+    if (bc_rows && (i == 0 || i == n - 1)) s = (i == col) ? 1.0 : 0.0;
     S[q] = s;
 }
 
 int gd_cnab2_build(hipStream_t st, const double *d_lap, const double *d_rinv2, double nu, double dt,
-                   double theta, int nl, int n, int bw, double *L, double *S, double *LU, double *L_t,
+                   double theta, int nl, int n, int bw, int bc_rows, double *L, double *S, double *LU, double *L_t,
                    double *LU_t, int *d_status)
 {
     long tot = (long)nl * (2 * bw + 1) * n;
     hipLaunchKernelGGL(cnab2_assemble_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, d_lap, d_rinv2,
-                       nu, dt, theta, nl, n, bw, L, S, L_t);
+                       nu, dt, theta, nl, n, bw, bc_rows, L, S, L_t);
     GD_TRY(gd_check_launch("cnab2_assemble_kernel"));
     return gd_launch_band_lu(st, S, LU, LU_t, n, bw, nl, d_status);
 }
@@ -128,7 +131,7 @@ int gd_cnab2_build(hipStream_t st, const double *d_lap, const double *d_rinv2, d
 // 2 fused rhs+solve in place on u.
 __global__ void __launch_bounds__(64) cnab2_lmajor_kernel(
     const double *__restrict__ Lall, const double *__restrict__ LUall, int nl_mats, int lmax, int mmax, int n,
-    int bw, int nlm, double dt, double theta, int mode, const double *u_re, const double *u_im,
+    int bw, int nlm, double dt, double theta, int mode, int bc_rows, const double *u_re, const double *u_im,
     const double *__restrict__ nl_re, const double *__restrict__ nl_im, const double *__restrict__ np_re,
     const double *__restrict__ np_im, double *out_re, double *out_im, int *__restrict__ status)
 {
@@ -175,6 +178,7 @@ __global__ void __launch_bounds__(64) cnab2_lmajor_kernel(
                 const size_t oi = base + (size_t)nlm * i;
                 double v = inv_dt * win[bw] + 1.5 * nlv[oi] - 0.5 * npv[oi];
                 if (lw != 0.0) v += lw * lin;
+                if (bc_rows && (i == 0 || i == n - 1)) v = 0.0; // Dirichlet value
                 if (mode == 2) {
                     double acc = 0.0;
                     int j0 = i - bw < 0 ? 0 : i - bw;
@@ -209,7 +213,7 @@ __global__ void __launch_bounds__(64) cnab2_lmajor_kernel(
 }
 
 int gd_launch_cnab2_lmajor(hipStream_t st, const double *L, const double *LU, int nl_mats, int lmax, int mmax,
-                           int n, int bw, double dt, double theta, int mode, const double *u_re,
+                           int n, int bw, double dt, double theta, int mode, int bc_rows, const double *u_re,
                            const double *u_im, const double *nl_re, const double *nl_im, const double *np_re,
                            const double *np_im, double *out_re, double *out_im, int *d_status)
 {
@@ -217,7 +221,8 @@ int gd_launch_cnab2_lmajor(hipStream_t st, const double *L, const double *LU, in
     size_t lds = sizeof(double) * 2 * (2 * bw + 1) * n;
     GD_CHECK(lds <= 64 * 1024 && bw <= 7, GD_ERR_ARG, "band too large (n=%d bw=%d)", n, bw);
     hipLaunchKernelGGL(cnab2_lmajor_kernel, dim3(lmax + 1), dim3(64), lds, st, L, LU, nl_mats, lmax, mmax, n, bw,
-                       nlm, dt, theta, mode, u_re, u_im, nl_re, nl_im, np_re, np_im, out_re, out_im, d_status);
+                       nlm, dt, theta, mode, bc_rows, u_re, u_im, nl_re, nl_im, np_re, np_im, out_re, out_im,
+                       d_status);
     return gd_check_launch("cnab2_lmajor_kernel");
 }
 
@@ -230,7 +235,7 @@ int gd_launch_cnab2_lmajor(hipStream_t st, const double *L, const double *LU, in
 template <int BW>
 __global__ void __launch_bounds__(256) cnab2_step_kernel(const double *__restrict__ Lt, const double *__restrict__ LUt,
                                                          const int *__restrict__ l_of_idx, int nl, int n, int nlm,
-                                                         double dt, double theta, double *__restrict__ st,
+                                                         double dt, double theta, int bc_mask, double *__restrict__ st,
                                                          const double *__restrict__ nl_arr,
                                                          const double *__restrict__ np_arr)
 {
@@ -248,6 +253,7 @@ __global__ void __launch_bounds__(256) cnab2_step_kernel(const double *__restric
     const double *nre = nl_arr + (size_t)(2 * f) * fs, *nim = nl_arr + (size_t)(2 * f + 1) * fs;
     const double *pre = np_arr + (size_t)(2 * f) * fs, *pim = np_arr + (size_t)(2 * f + 1) * fs;
     const bool tbc = (f == 4);
+    const bool bcr = (bc_mask >> f) & 1; // Dirichlet rows in this field's operator
     const double inv_dt = 1.0 / dt, lw = 1.0 - theta;
     double wr[W], wi[W];   // window of original u, rows i-BW..i+BW
     double yr[BW], yi[BW]; // last BW forward-substituted values
@@ -278,6 +284,7 @@ __global__ void __launch_bounds__(256) cnab2_step_kernel(const double *__restric
             vr += lw * lr;
             vi += lw * li;
         }
+        if (bcr && (i == 0 || i == n - 1)) vr = vi = 0.0; // Dirichlet value
         // forward substitution with the unit-lower factor
 #pragma unroll
         for (int q = 0; q < BW; ++q) {
@@ -339,7 +346,7 @@ __global__ void __launch_bounds__(256) cnab2_step_kernel(const double *__restric
 }
 
 int gd_launch_cnab2_step(hipStream_t st, const double *Lt, const double *LUt, const int *l_of_idx, int nl, int n,
-                         int bw, int nlm, double dt, double theta, double *state, const double *nl_arr,
+                         int bw, int nlm, double dt, double theta, int bc_mask, double *state, const double *nl_arr,
                          const double *np_arr)
 {
     dim3 grid((nlm + 255) / 256, 5);
@@ -347,7 +354,7 @@ int gd_launch_cnab2_step(hipStream_t st, const double *Lt, const double *LUt, co
 #define CASE(B)                                                                                             \
     case B:                                                                                                 \
         hipLaunchKernelGGL(cnab2_step_kernel<B>, grid, dim3(256), 0, st, Lt, LUt, l_of_idx, nl, n, nlm, dt, \
-                           theta, state, nl_arr, np_arr);                                                   \
+                           theta, bc_mask, state, nl_arr, np_arr);                                          \
         break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
 #undef CASE

@@ -7,14 +7,14 @@ int gd_launch_band_lu(hipStream_t st, const double *ab, double *lu, double *lu_t
 int gd_launch_band_solve(hipStream_t st, const double *lu, const int *sys_of_rhs, int n, int bw, int nsys,
                          int nrhs, double *x, int *d_status);
 int gd_cnab2_build(hipStream_t st, const double *d_lap, const double *d_rinv2, double nu, double dt,
-                   double theta, int nl, int n, int bw, double *L, double *S, double *LU, double *L_t,
+                   double theta, int nl, int n, int bw, int bc_rows, double *L, double *S, double *LU, double *L_t,
                    double *LU_t, int *d_status);
 int gd_launch_cnab2_lmajor(hipStream_t st, const double *L, const double *LU, int nl_mats, int lmax, int mmax,
-                           int n, int bw, double dt, double theta, int mode, const double *u_re,
+                           int n, int bw, double dt, double theta, int mode, int bc_rows, const double *u_re,
                            const double *u_im, const double *nl_re, const double *nl_im, const double *np_re,
                            const double *np_im, double *out_re, double *out_im, int *d_status);
 int gd_launch_cnab2_step(hipStream_t st, const double *Lt, const double *LUt, const int *l_of_idx, int nl, int n,
-                         int bw, int nlm, double dt, double theta, double *state, const double *nl_arr,
+                         int bw, int nlm, double dt, double theta, int bc_mask, double *state, const double *nl_arr,
                          const double *np_arr);
 int gd_launch_energies(hipStream_t st, const double *state, const int *l_of_idx, int nlm, int n, double *part,
                        int nblk, double *out3);

@@ -167,7 +167,7 @@ template <typename T, bool VEC> __global__ void __launch_bounds__(320) synth_ker
     for (int rb = 0; rb < 2; ++rb) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int i = i0 + rb * 16 + 4 * (lane >> 4) + v;
+            const int i = i0 + rb * 16 + acc_row<T>(lane, v);
             T on, os;
             int slot;
             if constexpr (VEC) {
@@ -280,7 +280,7 @@ template <typename T, bool VEC> __global__ void __launch_bounds__(320) ana_kerne
     for (int rb = 0; rb < 2; ++rb) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int row = row0 + rb * 16 + 4 * (lane >> 4) + v;
+            const int row = row0 + rb * 16 + acc_row<T>(lane, v);
             double val;
             int outk, reim;
             if constexpr (VEC) {

@@ -100,15 +100,23 @@ __global__ void __launch_bounds__(256) spectral_pre_kernel(const double *__restr
 }
 
 // ---------------------------------------------------------------- orchestration
-template <typename T> static int nonlinear_T(gd_step *s, double *nl_out)
+// ev (optional): events recorded after each kernel class, ev[0] before the first
+#define GD_MARK(k)                                                  \
+    do {                                                            \
+        if (ev) GD_HIP(hipEventRecord(ev[k], s->ctx->stream));      \
+    } while (0)
+
+template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEvent_t *ev)
 {
     hipStream_t st = s->ctx->stream;
+    GD_MARK(0);
     gd_sht_plan *p = &s->sht;
     const int nr = s->d.nr, nlm = s->nlm;
     hipLaunchKernelGGL((spectral_pre_kernel<T>), dim3((nlm + 255) / 256, nr), dim3(256), 0, st,
                        s->state.as<double>(), p->d_l_of_idx.as<int>(), s->D1.as<double>(), s->D2.as<double>(),
                        s->rtab.as<double>(), nlm, nr, s->d.bw, s->d.radials, s->synV.as<T>(), s->synS.as<T>());
     GD_TRY(gd_check_launch("spectral_pre_kernel"));
+    GD_MARK(1);
 
     SynthArgs a{};
     a.moff = p->d_moff.as<int>();
@@ -129,6 +137,7 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out)
     a.tab1 = gd_sht_tab(p, 2);
     a.slot0 = 0;
     GD_TRY(gd_launch_synth_vec<T>(st, a));
+    GD_MARK(2);
     a.in = s->synS.p;
     a.cin = 10;
     a.nf = 5;
@@ -137,6 +146,7 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out)
     a.tab1 = nullptr;
     a.slot0 = 40;
     GD_TRY(gd_launch_synth_scal<T>(st, a));
+    GD_MARK(3);
 
     RingArgs r{};
     gd_sht_fill_ring(p, r, nr);
@@ -157,6 +167,7 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out)
     r.ro = s->d.Pm / s->d.E;
     r.coriolis_index = s->d.coriolis_index;
     GD_TRY(gd_launch_ring<T>(st, RING_STEP, r));
+    GD_MARK(4);
 
     AnaArgs g{};
     g.gb = s->gb.p;
@@ -184,6 +195,7 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out)
     g.scale[2] = 1;
     g.scale[3] = 2;
     GD_TRY(gd_launch_ana<T>(st, g, true, maxgroup));
+    GD_MARK(5);
     g.nf = 1;
     g.gslot0 = 16;
     g.tab0 = gd_sht_tab(p, 0);
@@ -191,24 +203,35 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out)
     g.out[0] = nl_out + 8 * fs; // -u.grad T -> T
     g.scale[0] = 1;
     GD_TRY(gd_launch_ana<T>(st, g, false, maxgroup));
+    GD_MARK(6);
     return GD_OK;
 }
 
-static int nonlinear(gd_step *s, double *nl_out)
+static int nonlinear(gd_step *s, double *nl_out, hipEvent_t *ev = nullptr)
 {
-    if (s->d.prec == GD_PREC_FP64) return nonlinear_T<double>(s, nl_out);
-    return nonlinear_T<float>(s, nl_out);
+    if (s->d.prec == GD_PREC_FP64) return nonlinear_T<double>(s, nl_out, ev);
+    return nonlinear_T<float>(s, nl_out, ev);
+}
+
+// fields whose implicit operator carries Dirichlet rows (bit f for field f)
+static int bc_mask(const gd_step *s)
+{
+    if (s->d.bc_rows == 1) return 0x1f;
+    if (s->d.bc_rows == 2) return 0x10;
+    return 0;
 }
 
 // one full step with the nl buffers chosen by parity; boot = AB1 bootstrap
-static int enqueue_step(gd_step *s, int parity, bool boot)
+static int enqueue_step(gd_step *s, int parity, bool boot, hipEvent_t *ev = nullptr)
 {
     double *cur = s->nlbuf[parity].as<double>();
     double *prev = boot ? cur : s->nlbuf[parity ^ 1].as<double>();
-    GD_TRY(nonlinear(s, cur));
-    return gd_launch_cnab2_step(s->ctx->stream, s->Lt.as<double>(), s->LUt.as<double>(), s->sht.d_l_of_idx.as<int>(),
-                                s->nl, s->d.nr, s->d.bw, s->nlm, s->d.dt, s->d.theta, s->state.as<double>(), cur,
-                                prev);
+    GD_TRY(nonlinear(s, cur, ev));
+    GD_TRY(gd_launch_cnab2_step(s->ctx->stream, s->Lt.as<double>(), s->LUt.as<double>(), s->sht.d_l_of_idx.as<int>(),
+                                s->nl, s->d.nr, s->d.bw, s->nlm, s->d.dt, s->d.theta, bc_mask(s), s->state.as<double>(),
+                                cur, prev));
+    GD_MARK(7);
+    return GD_OK;
 }
 
 static void destroy_graphs(gd_step *s)
@@ -274,9 +297,10 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
     if (rc == GD_OK) rc = s->status.ensure(sizeof(int));
     if (rc == GD_OK && hipMemsetAsync(s->status.p, 0, sizeof(int), ctx->stream) != hipSuccess) rc = GD_ERR_HIP;
     const double nus[3] = {d->Pm, 1.0, d->Pm / d->Pr};
+    GD_CHECK(d->bc_rows >= 0 && d->bc_rows <= 2, GD_ERR_ARG, "gd_step_create: bc_rows=%d", d->bc_rows);
     for (int k = 0; k < 3 && rc == GD_OK; ++k)
         rc = gd_cnab2_build(ctx->stream, dlap.as<double>(), s->rtab.as<double>() + nr, nus[k], d->dt, d->theta, s->nl,
-                            nr, bw, tmpL.as<double>(), tmpS.as<double>(), tmpLU.as<double>(),
+                            nr, bw, d->bc_rows == 1 || (d->bc_rows == 2 && k == 2), tmpL.as<double>(), tmpS.as<double>(), tmpLU.as<double>(),
                             s->Lt.as<double>() + k * nb, s->LUt.as<double>() + k * nb, s->status.as<int>());
     int stv = 0;
     if (rc == GD_OK && hipMemcpyAsync(&stv, s->status.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
@@ -458,4 +482,35 @@ extern "C" int gd_step_energies(gd_step *s, double *out3)
     GD_TRY(gd_launch_energies(s->ctx->stream, s->state.as<double>(), s->sht.d_l_of_idx.as<int>(), s->nlm, s->d.nr,
                               s->part.as<double>(), 512, s->eout.as<double>()));
     return gd_from_device(s->ctx, out3, s->eout.p, sizeof(double) * 3);
+}
+
+// Per-kernel-class device time with HIP events on the step's stream:
+// out[k] = average ms per step of class k over nsteps eager steps, classes
+// 0 spectral_pre, 1 synth vector, 2 synth scalar, 3 ring (FFT + products),
+// 4 analysis vector, 5 analysis scalar, 6 CNAB2 rhs + banded solve.
+extern "C" int gd_step_profile(gd_step *s, int nsteps, double *out)
+{
+    GD_CHECK(s && out && nsteps >= 1, GD_ERR_ARG, "gd_step_profile: bad args");
+    GD_HIP(hipSetDevice(s->ctx->device));
+    hipEvent_t ev[8];
+    for (int k = 0; k < 8; ++k) GD_HIP(hipEventCreate(&ev[k]));
+    for (int k = 0; k < GD_NKCLASS; ++k) out[k] = 0.0;
+    int rc = GD_OK;
+    for (int it = 0; it < nsteps && rc == GD_OK; ++it) {
+        rc = enqueue_step(s, s->step & 1, s->step == 0, ev);
+        if (rc != GD_OK) break;
+        ++s->step;
+        if (hipEventSynchronize(ev[7]) != hipSuccess) {
+            rc = GD_ERR_HIP;
+            break;
+        }
+        for (int k = 0; k < GD_NKCLASS; ++k) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, ev[k], ev[k + 1]) != hipSuccess) rc = GD_ERR_HIP;
+            out[k] += ms / nsteps;
+        }
+    }
+    for (int k = 0; k < 8; ++k) (void)hipEventDestroy(ev[k]);
+    if (rc == GD_ERR_HIP) gd_set_error("gd_step_profile: event failure");
+    return rc;
 }

@@ -7,7 +7,7 @@ trailing '!' dropped.
 """
 from ._lib import GDError, MissingLError, lib  # noqa: F401
 from .fields import (RadialDomain, SHTnsPhysicalField, SHTnsSpectralField,  # noqa: F401
-                     SHTnsVectorField, create_radial_domain)
+                     SHTnsVectorField, create_radial_domain, seeded_state)
 from .linear_algebra import (BandedLU, BandedMatrix, apply_banded_matrix, banded_to_dense,  # noqa: F401
                              create_derivative_matrix, create_radial_laplacian, factorize_banded,
                              solve_banded)

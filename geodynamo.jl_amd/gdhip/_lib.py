@@ -18,13 +18,14 @@ GD_FD_LITERAL, GD_FD_TRANSPOSE = 0, 1
 # every symbol include/gdhip.h declares
 EXPORTS = [
     "gd_last_error", "gd_version", "gd_ctx_create", "gd_ctx_set_stream", "gd_ctx_synchronize",
-    "gd_ctx_destroy", "gd_radial_grid", "gd_derivative_matrix", "gd_radial_laplacian",
+    "gd_ctx_destroy", "gd_seeded_state", "gd_radial_grid", "gd_derivative_matrix", "gd_radial_laplacian",
     "gd_banded_factor", "gd_banded_solve", "gd_cnab2_mats_create", "gd_cnab2_mats_get",
     "gd_cnab2_mats_destroy", "gd_build_rhs_cnab2", "gd_solve_implicit_step", "gd_cnab2_rhs_solve",
     "gd_sht_plan_create", "gd_sht_plan_destroy", "gd_sht_nlm", "gd_sht_grid", "gd_sht_tables",
     "gd_sht_scalar_synth", "gd_sht_scalar_analysis", "gd_sht_vector_synth", "gd_sht_vector_analysis",
     "gd_step_create", "gd_step_destroy", "gd_step_set_state", "gd_step_get_state",
     "gd_step_nonlinear", "gd_step_run", "gd_step_run_graph", "gd_step_count", "gd_step_energies",
+    "gd_step_profile",
     "gd_timestep_error",
 ]
 
@@ -46,7 +47,7 @@ class StepDesc(ctypes.Structure):
                 ("lmax", ctypes.c_int), ("mmax", ctypes.c_int), ("nlat", ctypes.c_int),
                 ("nlon", ctypes.c_int), ("nr", ctypes.c_int), ("bw", ctypes.c_int),
                 ("prec", ctypes.c_int), ("fd_mode", ctypes.c_int), ("radials", ctypes.c_int),
-                ("coriolis_index", ctypes.c_int)]
+                ("coriolis_index", ctypes.c_int), ("bc_rows", ctypes.c_int)]
 
 
 _lib = None
@@ -68,11 +69,12 @@ def lib():
             "gd_ctx_synchronize": (i, [vp]),
             "gd_ctx_destroy": (i, [vp]),
             "gd_radial_grid": (i, [i, d, vp]),
+            "gd_seeded_state": (i, [ctypes.c_ulonglong, i, i, i, i, d, d, i, vp, vp]),
             "gd_derivative_matrix": (i, [i, i, i, vp, i, vp]),
             "gd_radial_laplacian": (i, [i, i, vp, i, vp]),
             "gd_banded_factor": (i, [vp, i, i, i, vp, vp, i]),
             "gd_banded_solve": (i, [vp, i, i, i, vp, vp, i, vp, i]),
-            "gd_cnab2_mats_create": (i, [vp, i, i, i, vp, d, d, d, i, pp]),
+            "gd_cnab2_mats_create": (i, [vp, i, i, i, vp, d, d, d, i, i, pp]),
             "gd_cnab2_mats_get": (i, [vp, vp, vp, vp]),
             "gd_cnab2_mats_destroy": (i, [vp]),
             "gd_build_rhs_cnab2": (i, [vp, i, i, i] + [vp] * 8 + [i]),
@@ -96,6 +98,7 @@ def lib():
             "gd_step_run_graph": (i, [vp, i]),
             "gd_step_count": (i, [vp]),
             "gd_step_energies": (i, [vp, vp]),
+            "gd_step_profile": (i, [vp, i, vp]),
             "gd_timestep_error": (i, [vp, l, vp, vp, vp, vp, vp, i]),
         }
         for name, (res, args) in sig.items():

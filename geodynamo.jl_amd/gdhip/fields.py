@@ -73,3 +73,14 @@ def create_radial_domain(nr, rratio=0.35):
     r = np.zeros((7, nr))
     check(lib().gd_radial_grid(nr, float(rratio), r.ctypes.data))
     return RadialDomain(N=nr, r=r)
+
+
+def seeded_state(seed, lmax, nr, nfields=5, amp=1.0, power=2.0, mmax=None, zero_boundaries=True):
+    """Seeded (l+1)^-power complex state of the configurations, shape
+    (nfields, nr, nlm) in the l-major layout (see gd_seeded_state)."""
+    mmax = lmax if mmax is None else mmax
+    nlm = sum(min(l, mmax) + 1 for l in range(lmax + 1))
+    re, im = np.zeros((nfields, nr, nlm)), np.zeros((nfields, nr, nlm))
+    check(lib().gd_seeded_state(seed, lmax, mmax, nr, nfields, float(amp), float(power),
+                                int(zero_boundaries), re.ctypes.data, im.ctypes.data))
+    return re + 1j * im

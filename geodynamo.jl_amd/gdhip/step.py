@@ -16,12 +16,12 @@ FIELDS = ("u_tor", "u_pol", "B_tor", "B_pol", "T")
 
 class CNAB2Step:
     def __init__(self, lmax, nlat, nlon, nr, E=1e-3, Pm=5.0, Pr=1.0, Ra=100.0, rratio=0.35, dt=1e-5,
-                 theta=0.5, bw=3, mmax=None, precision="fp64", literal_fd=False, radials=True,
-                 coriolis_index=0, device=0):
+                 theta=0.5, bw=3, mmax=None, precision="fp64", literal_fd=False, radials=False,
+                 coriolis_index=0, bc_rows=1, device=0):
         mmax = lmax if mmax is None else mmax
         self.desc = StepDesc(E, Pm, Pr, Ra, rratio, dt, theta, lmax, mmax, nlat, nlon, nr, bw,
                              PRECISIONS[precision], GD_FD_LITERAL if literal_fd else GD_FD_TRANSPOSE,
-                             int(radials), int(coriolis_index))
+                             int(radials), int(coriolis_index), int(bc_rows))
         self.lmax, self.mmax, self.nlat, self.nlon, self.nr = lmax, mmax, nlat, nlon, nr
         self.nlm = sum(min(l, mmax) + 1 for l in range(lmax + 1))
         self.device = device
@@ -71,6 +71,15 @@ class CNAB2Step:
         out = np.zeros(3)
         check(lib().gd_step_energies(self._h, out.ctypes.data))
         return out
+
+    KERNEL_CLASSES = ("spectral_pre", "synth_vector", "synth_scalar", "ring_fft_products",
+                      "analysis_vector", "analysis_scalar", "cnab2_solve")
+
+    def profile(self, nsteps=5):
+        """Average device ms per step of each kernel class (HIP events, eager steps)."""
+        out = np.zeros(len(self.KERNEL_CLASSES))
+        check(lib().gd_step_profile(self._h, nsteps, out.ctypes.data))
+        return dict(zip(self.KERNEL_CLASSES, out.tolist()))
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:

@@ -12,7 +12,7 @@ class SHTnsImplicitMatrices:
     """timestep.jl:489-496.  Holds L_l, S_l = I/dt - theta L_l and LU(S_l) for
     l = 0..lmax on the device; host copies on demand."""
 
-    def __init__(self, config, domain, diffusivity, dt, theta=0.5, bw=DEFAULT_BW, literal_fd=False):
+    def __init__(self, config, domain, diffusivity, dt, theta=0.5, bw=DEFAULT_BW, literal_fd=False, bc_rows=False):
         self.config, self.domain = config, domain
         self.diffusivity, self.dt, self.theta, self.bw = diffusivity, dt, theta, bw
         self.lmax = int(config.lmax)
@@ -21,7 +21,7 @@ class SHTnsImplicitMatrices:
         check(lib().gd_cnab2_mats_create(context(config.device), self.lmax, self.nr, bw,
                                          np.ascontiguousarray(domain.r).ctypes.data, float(diffusivity),
                                          float(dt), float(theta),
-                                         GD_FD_LITERAL if literal_fd else GD_FD_TRANSPOSE,
+                                         GD_FD_LITERAL if literal_fd else GD_FD_TRANSPOSE, int(bc_rows),
                                          ctypes.byref(self._h)))
         self.l_values = np.arange(self.lmax + 1)
         self.lookup = {l: l for l in range(self.lmax + 1)}
@@ -60,9 +60,9 @@ class SHTnsImplicitMatrices:
 
 
 def create_shtns_timestepping_matrices(config, domain, diffusivity, dt, theta=0.5, bw=DEFAULT_BW,
-                                       literal_fd=False):
-    """timestep.jl:498-543"""
-    return SHTnsImplicitMatrices(config, domain, diffusivity, dt, theta, bw, literal_fd)
+                                       literal_fd=False, bc_rows=False):
+    """timestep.jl:498-543 (bc_rows=True adds Dirichlet rows, zero boundary values)"""
+    return SHTnsImplicitMatrices(config, domain, diffusivity, dt, theta, bw, literal_fd, bc_rows)
 
 
 def build_rhs_cnab2(rhs, un, nl, nl_prev, dt, matrices):

@@ -1,0 +1,204 @@
+"""
+    GeoDynamoHIP
+
+`ccall` shim that puts libgdhip (hand-written HIP for MI355X, C ABI in
+include/gdhip.h) behind GeoDynamo.jl's own transform and timestep entry
+points.  Method names and argument order follow
+src/shtnskit_field_functions.jl, src/timestep.jl and src/linear_algebra.jl;
+arrays are the reference's column-major layouts, passed as host memory
+(GD_HOST) so callers need no device arrays.  Julia was not available where
+this shim was written: it is not exercised by the test suite, which drives
+the same C ABI from Python (geodynamo.jl_amd/gdhip).
+"""
+module GeoDynamoHIP
+
+const libgdhip = joinpath(@__DIR__, "..", "gdhip", "libgdhip.so")
+
+const GD_HOST = Cint(1)
+const GD_FD_TRANSPOSE = Cint(1)
+const PREC = Dict(:fp64 => Cint(0), :fp32 => Cint(1), :bf16x3 => Cint(2))
+
+struct GDError <: Exception
+    code::Cint
+    msg::String
+end
+
+function check(rc::Cint)
+    rc == 0 && return nothing
+    msg = unsafe_string(ccall((:gd_last_error, libgdhip), Cstring, ()))
+    rc == 4 && error(msg)                      # timestep.jl:646 raises for a missing l
+    throw(GDError(rc, msg))
+end
+
+const _ctx = Ref{Ptr{Cvoid}}(C_NULL)
+function context(device::Integer=0)
+    if _ctx[] == C_NULL
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:gd_ctx_create, libgdhip), Cint, (Cint, Ref{Ptr{Cvoid}}), device, h))
+        _ctx[] = h[]
+    end
+    return _ctx[]
+end
+
+# ---------------------------------------------------------------- transforms
+mutable struct HIPSHTPlan
+    ptr::Ptr{Cvoid}
+    lmax::Int; mmax::Int; nlat::Int; nlon::Int; nlm::Int
+end
+
+"create_shtnskit_config (shtnskit_transforms.jl:102) -> device plan"
+function create_plan(; lmax::Int, mmax::Int=lmax, nlat::Int, nlon::Int, precision::Symbol=:fp64)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:gd_sht_plan_create, libgdhip), Cint,
+                (Ptr{Cvoid}, Cint, Cint, Cint, Cint, Cint, Ref{Ptr{Cvoid}}),
+                context(), lmax, mmax, nlat, nlon, PREC[precision], h))
+    nlm = ccall((:gd_sht_nlm, libgdhip), Cint, (Ptr{Cvoid},), h[])
+    p = HIPSHTPlan(h[], lmax, mmax, nlat, nlon, nlm)
+    finalizer(p) do q
+        ccall((:gd_sht_plan_destroy, libgdhip), Cint, (Ptr{Cvoid},), q.ptr)
+    end
+    return p
+end
+
+# spec: data_real/data_imag :: Array{Float64,3} (nlm, 1, nr); phys :: (nlat, nlon, nr)
+"shtnskit_spectral_to_physical! (shtnskit_field_functions.jl:11)"
+function shtnskit_spectral_to_physical!(p::HIPSHTPlan, re::Array{Float64,3}, im::Array{Float64,3},
+                                        phys::Array{Float64,3})
+    nr = size(re, 3)
+    GC.@preserve re im phys check(ccall((:gd_sht_scalar_synth, libgdhip), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint, Ptr{Float64}, Cint),
+        p.ptr, re, im, nr, phys, GD_HOST))
+    return nothing
+end
+
+"shtnskit_physical_to_spectral! (shtnskit_field_functions.jl:131)"
+function shtnskit_physical_to_spectral!(p::HIPSHTPlan, phys::Array{Float64,3}, re::Array{Float64,3},
+                                        im::Array{Float64,3})
+    nr = size(re, 3)
+    GC.@preserve re im phys check(ccall((:gd_sht_scalar_analysis, libgdhip), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Cint, Ptr{Float64}, Ptr{Float64}, Cint),
+        p.ptr, phys, nr, re, im, GD_HOST))
+    return nothing
+end
+
+"shtnskit_vector_synthesis! (shtnskit_field_functions.jl:252): S := pol, T := tor"
+function shtnskit_vector_synthesis!(p::HIPSHTPlan, tor_re, tor_im, pol_re, pol_im, vt, vp)
+    nr = size(tor_re, 3)
+    GC.@preserve tor_re tor_im pol_re pol_im vt vp check(ccall((:gd_sht_vector_synth, libgdhip), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint, Ptr{Float64},
+         Ptr{Float64}, Cint),
+        p.ptr, tor_re, tor_im, pol_re, pol_im, nr, vt, vp, GD_HOST))
+    return nothing
+end
+
+"shtnskit_vector_analysis! (shtnskit_field_functions.jl:291)"
+function shtnskit_vector_analysis!(p::HIPSHTPlan, vt, vp, tor_re, tor_im, pol_re, pol_im)
+    nr = size(tor_re, 3)
+    GC.@preserve vt vp tor_re tor_im pol_re pol_im check(ccall((:gd_sht_vector_analysis, libgdhip), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+         Ptr{Float64}, Cint),
+        p.ptr, vt, vp, nr, tor_re, tor_im, pol_re, pol_im, GD_HOST))
+    return nothing
+end
+
+# ---------------------------------------------------------------- banded LU
+"factorize_banded (linear_algebra.jl:23): data is the (2bw+1, N) band"
+function factorize_banded(data::Matrix{Float64}, bw::Integer)
+    N = size(data, 2)
+    lu = similar(data)
+    GC.@preserve data lu check(ccall((:gd_banded_factor, libgdhip), Cint,
+        (Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Float64}, Ptr{Float64}, Cint),
+        context(), N, bw, 1, data, lu, GD_HOST))
+    return lu
+end
+
+"solve_banded! (linear_algebra.jl:59)"
+function solve_banded!(x::Vector{Float64}, lu::Matrix{Float64}, bw::Integer, b::Vector{Float64})
+    x === b || copyto!(x, b)
+    GC.@preserve lu x check(ccall((:gd_banded_solve, libgdhip), Cint,
+        (Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Float64}, Ptr{Cint}, Cint, Ptr{Float64}, Cint),
+        context(), length(x), bw, 1, lu, C_NULL, 1, x, GD_HOST))
+    return x
+end
+
+# ---------------------------------------------------------------- CNAB2 operators
+mutable struct HIPImplicitMatrices
+    ptr::Ptr{Cvoid}
+    lmax::Int; nr::Int; dt::Float64; theta::Float64
+end
+
+"create_shtns_timestepping_matrices (timestep.jl:498); r is domain.r (N, 7)"
+function create_shtns_timestepping_matrices(lmax::Integer, r::Matrix{Float64}, diffusivity::Float64,
+                                            dt::Float64; theta::Float64=0.5, bw::Integer=3,
+                                            bc_rows::Bool=false)
+    nr = size(r, 1)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve r check(ccall((:gd_cnab2_mats_create, libgdhip), Cint,
+        (Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Float64}, Cdouble, Cdouble, Cdouble, Cint, Cint, Ref{Ptr{Cvoid}}),
+        context(), lmax, nr, bw, r, diffusivity, dt, theta, GD_FD_TRANSPOSE, bc_rows, h))
+    M = HIPImplicitMatrices(h[], lmax, nr, dt, theta)
+    finalizer(M) do q
+        ccall((:gd_cnab2_mats_destroy, libgdhip), Cint, (Ptr{Cvoid},), q.ptr)
+    end
+    return M
+end
+
+"build_rhs_cnab2! (timestep.jl:610)"
+function build_rhs_cnab2!(rhs_re, rhs_im, u_re, u_im, nl_re, nl_im, np_re, np_im, dt::Float64,
+                          M::HIPImplicitMatrices; lmax::Integer, mmax::Integer=lmax)
+    dt == M.dt || error("dt differs from the implicit matrices' dt")
+    nr = size(u_re, 3)
+    GC.@preserve rhs_re rhs_im u_re u_im nl_re nl_im np_re np_im check(ccall((:gd_build_rhs_cnab2, libgdhip), Cint,
+        (Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+         Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint),
+        M.ptr, lmax, mmax, nr, u_re, u_im, nl_re, nl_im, np_re, np_im, rhs_re, rhs_im, GD_HOST))
+    return rhs_re, rhs_im
+end
+
+"solve_implicit_step! (timestep.jl:692)"
+function solve_implicit_step!(sol_re, sol_im, rhs_re, rhs_im, M::HIPImplicitMatrices; lmax::Integer,
+                              mmax::Integer=lmax)
+    nr = size(rhs_re, 3)
+    GC.@preserve sol_re sol_im rhs_re rhs_im check(ccall((:gd_solve_implicit_step, libgdhip), Cint,
+        (Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint),
+        M.ptr, lmax, mmax, nr, rhs_re, rhs_im, sol_re, sol_im, GD_HOST))
+    return sol_re, sol_im
+end
+
+# ---------------------------------------------------------------- device-resident step
+struct GDStepDesc
+    E::Cdouble; Pm::Cdouble; Pr::Cdouble; Ra::Cdouble; rratio::Cdouble; dt::Cdouble; theta::Cdouble
+    lmax::Cint; mmax::Cint; nlat::Cint; nlon::Cint; nr::Cint; bw::Cint
+    prec::Cint; fd_mode::Cint; radials::Cint; coriolis_index::Cint; bc_rows::Cint
+end
+
+mutable struct HIPStep
+    ptr::Ptr{Cvoid}
+end
+
+"The bench step of scripts/bench_timestepping.jl:6-18 on the GPU."
+function create_step(; lmax, nlat, nlon, nr, E=1e-3, Pm=5.0, Pr=1.0, Ra=100.0, rratio=0.35, dt=1e-5,
+                     theta=0.5, bw=3, precision::Symbol=:fp32)
+    d = GDStepDesc(E, Pm, Pr, Ra, rratio, dt, theta, lmax, lmax, nlat, nlon, nr, bw, PREC[precision],
+                   GD_FD_TRANSPOSE, 0, 0, 1)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:gd_step_create, libgdhip), Cint, (Ptr{Cvoid}, Ref{GDStepDesc}, Ref{Ptr{Cvoid}}),
+                context(), d, h))
+    s = HIPStep(h[])
+    finalizer(q -> ccall((:gd_step_destroy, libgdhip), Cint, (Ptr{Cvoid},), q.ptr), s)
+    return s
+end
+
+set_state!(s::HIPStep, re::Array{Float64}, im::Array{Float64}) = GC.@preserve re im check(
+    ccall((:gd_step_set_state, libgdhip), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint), s.ptr, re, im, GD_HOST))
+get_state!(s::HIPStep, re::Array{Float64}, im::Array{Float64}) = GC.@preserve re im check(
+    ccall((:gd_step_get_state, libgdhip), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint), s.ptr, re, im, GD_HOST))
+step_once!(s::HIPStep, n::Integer=1) = check(ccall((:gd_step_run_graph, libgdhip), Cint, (Ptr{Cvoid}, Cint), s.ptr, n))
+
+function energies(s::HIPStep)
+    out = zeros(3)
+    check(ccall((:gd_step_energies, libgdhip), Cint, (Ptr{Cvoid}, Ptr{Float64}), s.ptr, out))
+    return out
+end
+
+end # module

@@ -63,6 +63,15 @@ int gd_radial_grid(int nr, double rratio, double *r /* nr x 7 */);
 int gd_derivative_matrix(int order, int nr, int bw, const double *r, int fd_mode, double *band);
 int gd_radial_laplacian(int nr, int bw, const double *r, int fd_mode, double *band);
 
+/* Seeded spectral state of the configurations (host): splitmix64 -> Box-Muller
+ * normals drawn in the order field, level, lm, (re, im); value
+ * amp * n * (l+1)^-power, m = 0 real, rows 0 and nr-1 zeroed when
+ * zero_boundaries.  No counterpart in the reference (its initial conditions
+ * use Julia's RNG stream, InitialConditions.jl); the configurations define
+ * their states this way.  re/im: nfields blocks of (nlm, nr). */
+int gd_seeded_state(unsigned long long seed, int lmax, int mmax, int nr, int nfields, double amp,
+                    double power, int zero_boundaries, double *re, double *im);
+
 /* ---- banded LU without pivoting (FP64, GPU) --------------------------------
  * replaces factorize_banded (linear_algebra.jl:23-57) for nsys systems
  * (ab and lu: nsys consecutive (2bw+1) x n bands) and solve_banded!
@@ -76,9 +85,11 @@ int gd_banded_solve(gd_ctx *ctx, int n, int bw, int nsys, const double *lu,
 /* ---- CNAB2 implicit operators (FP64, GPU) ---------------------------------
  * replaces create_shtns_timestepping_matrices (timestep.jl:498-543): for
  * l = 0..lmax, L_l = nu(Lap - l(l+1)/r^2), S_l = I/dt - theta L_l, LU(S_l);
- * matrices stay device resident. */
+ * matrices stay device resident.  bc_rows = 0 is the reference operator (no
+ * boundary rows); 1 replaces rows 0 and nr-1 of S_l by Dirichlet rows and
+ * sets those rhs rows to 0 (zero boundary values). */
 int gd_cnab2_mats_create(gd_ctx *ctx, int lmax, int nr, int bw, const double *r, double nu,
-                         double dt, double theta, int fd_mode, gd_cnab2_mats **out);
+                         double dt, double theta, int fd_mode, int bc_rows, gd_cnab2_mats **out);
 int gd_cnab2_mats_get(gd_cnab2_mats *M, double *L, double *S, double *LU /* host, (lmax+1) bands each */);
 int gd_cnab2_mats_destroy(gd_cnab2_mats *M);
 /* replaces build_rhs_cnab2! (timestep.jl:610-690):
@@ -141,8 +152,11 @@ typedef struct {
     int lmax, mmax, nlat, nlon, nr, bw;
     int prec;           /* GD_PREC_*: Legendre/FFT/products arithmetic; solves stay FP64 */
     int fd_mode;        /* GD_FD_TRANSPOSE (default) or GD_FD_LITERAL */
-    int radials;        /* 1: radial components from the poloidal scalars (default) */
+    int radials;        /* 0: radial components stay 0 as in the reference (default);
+                           1: X_r = l(l+1) P_X / r^2 from the poloidal scalars */
     int coriolis_index; /* 0: reference indexing theta_{min(j,nlat)}; 1: theta_i */
+    int bc_rows;        /* Dirichlet rows (value 0) in S_l: 1 all fields (default), 2 temperature
+                           only, 0 none (the reference operator, unstable at these parameters) */
 } gd_step_desc;
 
 int gd_step_create(gd_ctx *ctx, const gd_step_desc *desc, gd_step **out);
@@ -160,6 +174,13 @@ int gd_step_count(const gd_step *s);
 /* FP64 energies: KE and ME (1/2 sum (T^2+P^2)/max(l(l+1),1), magnetic.jl:504-543)
  * and thermal 1/2 sum |T|^2 (thermal.jl:413-438); host out[3] */
 int gd_step_energies(gd_step *s, double *out3);
+/* Measurement hook with no counterpart in the reference: average device ms per
+ * step of each kernel class over nsteps eager steps, timed with HIP events on
+ * the step's stream.  out[GD_NKCLASS]: 0 spectral prep, 1 Legendre synthesis
+ * (vector), 2 Legendre synthesis (scalar), 3 ring FFT + products, 4 Legendre
+ * analysis (vector), 5 Legendre analysis (scalar), 6 CNAB2 rhs + banded solve. */
+#define GD_NKCLASS 7
+int gd_step_profile(gd_step *s, int nsteps, double *out);
 /* replaces compute_timestep_error (timestep.jl:733-753) on device arrays */
 int gd_timestep_error(gd_ctx *ctx, long n, const double *a_re, const double *a_im,
                       const double *b_re, const double *b_im, double *out, int flags);

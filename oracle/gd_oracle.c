@@ -175,7 +175,7 @@ void gdo_band_apply(int N, int bw, const double *ab, const double *v, double *ou
 
 /* timestep.jl:498-543 */
 int gdo_cnab2_matrices(int N, int bw, const double *r, double nu, double dt, double theta,
-                       int lmax, int fd_transpose, double *Lmat, double *Smat, double *LU)
+                       int lmax, int fd_transpose, int bc_rows, double *Lmat, double *Smat, double *LU)
 {
     int bs = (2 * bw + 1) * N;
     double *lap = (double *)malloc(sizeof(double) * bs);
@@ -188,6 +188,12 @@ int gdo_cnab2_matrices(int N, int bw, const double *r, double nu, double dt, dou
         for (int n = 0; n < N; ++n) BAND(Ll, bw, bw, n) -= nu * lf * r[n + N * 1];
         for (int q = 0; q < bs; ++q) Sl[q] = Ll[q] * (-theta);
         for (int n = 0; n < N; ++n) BAND(Sl, bw, bw, n) += 1.0 / dt;
+        if (bc_rows) {
+            /* Dirichlet rows: S[0][j] = S[N-1][j] = delta (boundary values 0) */
+            for (int j = 0; j <= bw && j < N; ++j) BAND(Sl, bw, bw + 0 - j, j) = (j == 0) ? 1.0 : 0.0;
+            for (int j = N - 1 - bw; j < N; ++j)
+                if (j >= 0) BAND(Sl, bw, bw + (N - 1) - j, j) = (j == N - 1) ? 1.0 : 0.0;
+        }
         memcpy(LUl, Sl, sizeof(double) * bs);
         if (gdo_band_factor(N, bw, LUl)) rc = -1;
     }
@@ -197,7 +203,7 @@ int gdo_cnab2_matrices(int N, int bw, const double *r, double nu, double dt, dou
 
 /* rhs = u/dt + 1.5 nl - 0.5 nl_prev + (1-theta) L_l u: timestep.jl:610-690 */
 void gdo_cnab2_rhs(int nlm, int N, int bw, const int *l_of_lm, const double *Lmat,
-                   double dt, double theta,
+                   double dt, double theta, int bc_rows,
                    const double *u_re, const double *u_im,
                    const double *nl_re, const double *nl_im,
                    const double *np_re, const double *np_im,
@@ -219,6 +225,7 @@ void gdo_cnab2_rhs(int nlm, int N, int bw, const int *l_of_lm, const double *Lma
                 double vr = inv_dt * u_re[o] + 1.5 * nl_re[o] - 0.5 * np_re[o];
                 double vi = inv_dt * u_im[o] + 1.5 * nl_im[o] - 0.5 * np_im[o];
                 if (lw != 0.0) { vr += lw * lr[r]; vi += lw * li[r]; }
+                if (bc_rows && (r == 0 || r == N - 1)) { vr = 0.0; vi = 0.0; }
                 rhs_re[o] = vr; rhs_im[o] = vi;
             }
         }
@@ -712,7 +719,8 @@ gdo_model *gdo_model_create(const gdo_params *p)
         M->Lm[k] = (double *)malloc(sizeof(double) * bs * (p->lmax + 1));
         M->Sm[k] = (double *)malloc(sizeof(double) * bs * (p->lmax + 1));
         M->LUm[k] = (double *)malloc(sizeof(double) * bs * (p->lmax + 1));
-        gdo_cnab2_matrices(N, bw, M->r, nus[k], p->dt, p->theta, p->lmax, p->fd_transpose,
+        int bc = (p->bc_rows == 1) || (p->bc_rows == 2 && k == 2);
+        gdo_cnab2_matrices(N, bw, M->r, nus[k], p->dt, p->theta, p->lmax, p->fd_transpose, bc,
                            M->Lm[k], M->Sm[k], M->LUm[k]);
     }
     M->l_of_lm = M->sht->l_of_lm; M->m_of_lm = M->sht->m_of_lm;
@@ -928,7 +936,8 @@ void gdo_step(gdo_model *M, int step_index, double *st_re, double *st_im,
     for (int f = 0; f < 5; ++f) {
         int k = nu_of_field[f];
         size_t o = (size_t)f * fs;
-        gdo_cnab2_rhs(nlm, N, bw, M->l_of_lm, M->Lm[k], M->p.dt, M->p.theta,
+        int bc = (M->p.bc_rows == 1) || (M->p.bc_rows == 2 && k == 2);
+        gdo_cnab2_rhs(nlm, N, bw, M->l_of_lm, M->Lm[k], M->p.dt, M->p.theta, bc,
                       st_re + o, st_im + o, nl_re + o, nl_im + o, np_re + o, np_im + o, rr, ri);
         gdo_cnab2_solve(nlm, N, bw, M->l_of_lm, M->LUm[k], rr, ri, st_re + o, st_im + o);
     }

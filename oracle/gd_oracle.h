@@ -28,10 +28,10 @@ void gdo_band_apply(int N, int bw, const double *ab, const double *v, double *ou
 /* CNAB2 per-l operators for l = 0..lmax (timestep.jl:498-543).
  * Lmat, Smat, LU: (lmax+1) blocks of (2bw+1)*N. */
 int  gdo_cnab2_matrices(int N, int bw, const double *r, double nu, double dt,
-                        double theta, int lmax, int fd_transpose,
+                        double theta, int lmax, int fd_transpose, int bc_rows,
                         double *Lmat, double *Smat, double *LU);
 void gdo_cnab2_rhs(int nlm, int N, int bw, const int *l_of_lm, const double *Lmat,
-                   double dt, double theta,
+                   double dt, double theta, int bc_rows,
                    const double *u_re, const double *u_im,
                    const double *nl_re, const double *nl_im,
                    const double *np_re, const double *np_im,
@@ -71,8 +71,10 @@ typedef struct {
     double E, Pm, Pr, Ra, rratio, dt, theta;
     int lmax, mmax, nlat, nlon, nr, bw;
     int fd_transpose;    /* 1: V^T w = k! e_k (well-posed); 0: literal V \ rhs */
-    int radials;         /* 1: synthesise radial components from the poloidal scalars */
+    int radials;         /* 1: synthesise X_r = l(l+1) P_X / r^2; 0: radials stay 0 (reference data flow) */
     int coriolis_index;  /* 0: reference indexing theta_{min(j,nlat)}; 1: theta_i */
+    int bc_rows;         /* Dirichlet rows (value 0) in S_l and the rhs: 1 all fields, 2 temperature
+                            only, 0 none (the reference operator) */
 } gdo_params;
 
 typedef struct gdo_model gdo_model;

@@ -114,10 +114,10 @@ class NpModel:
     """Field order: u_tor, u_pol, B_tor, B_pol, T; state shape (5, nr, nlm) complex."""
 
     def __init__(self, lmax, nlat, nlon, nr, E=1e-3, Pm=5.0, Pr=1.0, Ra=100.0, rratio=0.35,
-                 dt=1e-5, theta=0.5, bw=3, radials=True, coriolis_index=0):
+                 dt=1e-5, theta=0.5, bw=3, radials=False, coriolis_index=0, bc_rows=1):
         self.sht = NpSHT(lmax, lmax, nlat, nlon)
         self.nr, self.E, self.Pm, self.Pr, self.dt, self.th = nr, E, Pm, Pr, dt, theta
-        self.radials, self.coriolis_index = radials, coriolis_index
+        self.radials, self.coriolis_index, self.bc_rows = radials, coriolis_index, bc_rows
         self.r = radial_grid(nr, rratio)
         self.D1 = fd_dense(1, self.r, bw)
         self.D2 = fd_dense(2, self.r, bw)
@@ -129,7 +129,12 @@ class NpModel:
             for l in range(lmax + 1):
                 Ll = nu * self.lap - nu * l * (l + 1) * np.diag(self.r ** -2.0)
                 self.L[k, l] = Ll
-                self.S[k, l] = np.eye(nr) / dt - theta * Ll
+                S = np.eye(nr) / dt - theta * Ll
+                if bc_rows == 1 or (bc_rows == 2 and k == 2):  # Dirichlet rows, value 0
+                    S[0, :] = 0.0
+                    S[-1, :] = 0.0
+                    S[0, 0] = S[-1, -1] = 1.0
+                self.S[k, l] = S
 
     def nonlinear(self, st):
         sht, r, ll = self.sht, self.r, self.ll
@@ -190,5 +195,7 @@ class NpModel:
             for q, l in enumerate(self.sht.l):
                 u = st[f, :, q]
                 rhs = u / self.dt + 1.5 * nl[f, :, q] - 0.5 * nl_prev[f, :, q] + (1 - self.th) * (self.L[k, l] @ u)
+                if self.bc_rows == 1 or (self.bc_rows == 2 and k == 2):
+                    rhs[0] = rhs[-1] = 0.0
                 out[f, :, q] = np.linalg.solve(self.S[k, l], rhs)
         return out, nl, nl.copy()

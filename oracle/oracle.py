@@ -45,8 +45,8 @@ def lib():
             "gdo_band_solve": (None, [c_int, c_int, _dp, _dp]),
             "gdo_band_apply": (None, [c_int, c_int, _dp, _dp, _dp]),
             "gdo_cnab2_matrices": (c_int, [c_int, c_int, _dp, c_double, c_double, c_double,
-                                           c_int, c_int, _dp, _dp, _dp]),
-            "gdo_cnab2_rhs": (None, [c_int, c_int, c_int, _ip, _dp, c_double, c_double,
+                                           c_int, c_int, c_int, _dp, _dp, _dp]),
+            "gdo_cnab2_rhs": (None, [c_int, c_int, c_int, _ip, _dp, c_double, c_double, c_int,
                                      _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
             "gdo_cnab2_solve": (None, [c_int, c_int, c_int, _ip, _dp, _dp, _dp, _dp, _dp]),
             "gdo_nlm": (c_int, [c_int, c_int]),
@@ -127,20 +127,20 @@ def band_apply(ab, v):
     return out
 
 
-def cnab2_matrices(r, bw, nu, dt, theta, lmax, fd_transpose=True):
+def cnab2_matrices(r, bw, nu, dt, theta, lmax, fd_transpose=True, bc_rows=False):
     N = r.shape[1]
     shape = (lmax + 1, N, 2 * bw + 1)
     L, S, LU = np.zeros(shape), np.zeros(shape), np.zeros(shape)
-    rc = lib().gdo_cnab2_matrices(N, bw, _c(r), nu, dt, theta, lmax, int(fd_transpose), L, S, LU)
+    rc = lib().gdo_cnab2_matrices(N, bw, _c(r), nu, dt, theta, lmax, int(fd_transpose), int(bc_rows), L, S, LU)
     return L, S, LU, rc
 
 
-def cnab2_rhs(l_of_lm, Lmat, dt, theta, u, nl, nlp):
+def cnab2_rhs(l_of_lm, Lmat, dt, theta, u, nl, nlp, bc_rows=False):
     """u, nl, nlp: complex (nr, nlm)."""
     nr, nlm = u.shape
     bw = (Lmat.shape[2] - 1) // 2
     rr, ri = np.zeros((nr, nlm)), np.zeros((nr, nlm))
-    lib().gdo_cnab2_rhs(nlm, nr, bw, np.ascontiguousarray(l_of_lm, np.int32), _c(Lmat), dt, theta,
+    lib().gdo_cnab2_rhs(nlm, nr, bw, np.ascontiguousarray(l_of_lm, np.int32), _c(Lmat), dt, theta, int(bc_rows),
                         _c(u.real), _c(u.imag), _c(nl.real), _c(nl.imag), _c(nlp.real), _c(nlp.imag), rr, ri)
     return rr + 1j * ri
 
@@ -250,18 +250,18 @@ class _Params(ctypes.Structure):
                 ("lmax", ctypes.c_int), ("mmax", ctypes.c_int), ("nlat", ctypes.c_int),
                 ("nlon", ctypes.c_int), ("nr", ctypes.c_int), ("bw", ctypes.c_int),
                 ("fd_transpose", ctypes.c_int), ("radials", ctypes.c_int),
-                ("coriolis_index", ctypes.c_int)]
+                ("coriolis_index", ctypes.c_int), ("bc_rows", ctypes.c_int)]
 
 
 class Model:
     """Oracle CNAB2 model: field order u_tor, u_pol, B_tor, B_pol, T."""
 
     def __init__(self, lmax, nlat, nlon, nr, E=1e-3, Pm=5.0, Pr=1.0, Ra=100.0, rratio=0.35,
-                 dt=1e-5, theta=0.5, bw=3, mmax=None, fd_transpose=True, radials=True,
-                 coriolis_index=0):
+                 dt=1e-5, theta=0.5, bw=3, mmax=None, fd_transpose=True, radials=False,
+                 coriolis_index=0, bc_rows=1):
         mmax = lmax if mmax is None else mmax
         self.p = _Params(E, Pm, Pr, Ra, rratio, dt, theta, lmax, mmax, nlat, nlon, nr, bw,
-                         int(fd_transpose), int(radials), int(coriolis_index))
+                         int(fd_transpose), int(radials), int(coriolis_index), int(bc_rows))
         self.lmax, self.mmax, self.nlat, self.nlon, self.nr = lmax, mmax, nlat, nlon, nr
         self.nlm = nlm(lmax, mmax)
         self._h = lib().gdo_model_create(ctypes.addressof(self.p))

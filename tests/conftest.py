@@ -9,6 +9,12 @@ for p in (ROOT, os.path.join(ROOT, "geodynamo.jl_amd")):
         sys.path.insert(0, p)
 
 
+try:  # torch's bundled HIP runtime must be loaded before libgdhip's
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover
+    pass
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
 

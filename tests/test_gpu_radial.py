@@ -71,20 +71,23 @@ def test_cnab2_matrices_match_oracle(oracle, lmax, nr):
         assert _rel(LUg, LU) < 1e-12
 
 
-@pytest.mark.parametrize("lmax,nr", [(31, 33), (127, 65)])
-def test_cnab2_rhs_solve_match_oracle(oracle, lmax, nr):
+@pytest.mark.parametrize("lmax,nr,bc", [(31, 33, False), (127, 65, False), (31, 33, True), (127, 65, True)])
+def test_cnab2_rhs_solve_match_oracle(oracle, lmax, nr, bc):
     import gdhip
 
-    print("case: cnab2 rhs/solve lmax=%d nr=%d" % (lmax, nr), flush=True)
+    print("case: cnab2 rhs/solve lmax=%d nr=%d bc_rows=%d" % (lmax, nr, bc), flush=True)
     cfg = gdhip.create_shtnskit_config(lmax, nlat=3 * (lmax + 1) // 2, nlon=3 * (lmax + 1))
     dom = gdhip.create_radial_domain(nr, 0.35)
     dt, nu = 1e-5, 5.0
-    M = gdhip.create_shtns_timestepping_matrices(cfg, dom, nu, dt)
+    M = gdhip.create_shtns_timestepping_matrices(cfg, dom, nu, dt, bc_rows=bc)
     st = oracle.seeded_state(3, lmax, lmax, nr, nfields=3, amp=1e-3)
     u, nl, nlp = st[0], st[1] * 100, st[2] * 100
     lo, _ = oracle.lm_tables(lmax, lmax)
-    L, S, LU, _ = oracle.cnab2_matrices(oracle.radial_grid(nr, 0.35), 3, nu, dt, 0.5, lmax)
-    rhs_ref = oracle.cnab2_rhs(lo, L, dt, 0.5, u, nl, nlp)
+    L, S, LU, _ = oracle.cnab2_matrices(oracle.radial_grid(nr, 0.35), 3, nu, dt, 0.5, lmax, bc_rows=bc)
+    if bc:
+        Sg = M._fetch()[1]
+        assert _rel(Sg, S) < 1e-14
+    rhs_ref = oracle.cnab2_rhs(lo, L, dt, 0.5, u, nl, nlp, bc_rows=bc)
     sol_ref = oracle.cnab2_solve(lo, LU, rhs_ref)
 
     def F(z):

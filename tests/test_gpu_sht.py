@@ -35,7 +35,8 @@ def test_plm_tables_match_oracle(oracle, lmax, nlat):
     assert np.abs(cfg.cos_theta - x).max() < 1e-15
     assert np.abs(cfg.gauss_weights - w).max() < 1e-15
     Po, dPo = oracle.plm(lmax, lmax, x)
-    assert np.abs(P - Po).max() < 1e-12
+    # recurrence round-off grows with l: 2e-14 * (lmax+1) relative to max|P|
+    assert np.abs(P - Po).max() < 2e-14 * (lmax + 1) * np.abs(Po).max()
     assert np.abs(dP - dPo).max() < 1e-10 * max(1.0, np.abs(dPo).max())
 
 

@@ -37,10 +37,10 @@ def test_state_roundtrip_and_energies(oracle):
     assert np.abs(e - eo).max() < 1e-13 * np.abs(eo).max()
 
 
-@pytest.mark.parametrize("coriolis_index", [0, 1])
-def test_nonlinear_fp64_matches_oracle(oracle, coriolis_index):
-    print("case: nonlinear fp64 C0 coriolis_index=%d" % coriolis_index, flush=True)
-    st, g, o = _make(oracle, "fp64", coriolis_index=coriolis_index, **C0)
+@pytest.mark.parametrize("coriolis_index,radials", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_nonlinear_fp64_matches_oracle(oracle, coriolis_index, radials):
+    print("case: nonlinear fp64 C0 coriolis_index=%d radials=%d" % (coriolis_index, radials), flush=True)
+    st, g, o = _make(oracle, "fp64", coriolis_index=coriolis_index, radials=radials, **C0)
     nl = g.nonlinear()
     ref = o.nonlinear(st)
     err = _relf(nl, ref)
@@ -66,6 +66,19 @@ def test_steps_fp64_match_oracle_c0(oracle):
     print("state rel err", err)
     assert max(err) < 1e-10
     assert g.step_count == 3
+
+
+@pytest.mark.parametrize("bc_rows,radials", [(0, 0), (2, 0), (1, 1), (0, 1)])
+def test_step_variants_fp64_match_oracle(oracle, bc_rows, radials):
+    """The alternative readings (reference operator without boundary rows,
+    temperature rows only, synthesised radial components) for one step."""
+    print("case: 1 step fp64 C0 bc_rows=%d radials=%d" % (bc_rows, radials), flush=True)
+    st, g, o = _make(oracle, "fp64", bc_rows=bc_rows, radials=radials, **C0)
+    s, nl, nlp = o.step(st, None, 0)
+    g.run(1)
+    err = _relf(g.get_state_complex(), s)
+    print("state rel err", err)
+    assert max(err) < 1e-10
 
 
 def test_graph_replay_matches_eager(oracle):
