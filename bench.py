@@ -70,11 +70,15 @@ def roofline(prof_ms, prec):
     peak_f = PEAK["fp64" if prec == "fp64" else "fp32"]
     ridge = peak_f * 1e12 / (PEAK["hbm"] * 1e9)
     mfma = c["flops"] > 0 and c["flops"] / c["bytes"] > ridge
+    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # runs (scaled by a copy kernel of known size), when they cover this kernel
     traffic = None
-    pmc = os.path.join(ROOT, "profiles", "pmc_%s_%s.json" % (name, prec))
+    pmc = os.path.join(ROOT, "profiles", "pmc_step_lmax127_nr65.json")
     if os.path.exists(pmc):
         with open(pmc) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            rec = json.load(f).get(prec, {}).get(name)
+        if rec:
+            traffic = round(rec["hbm_bytes_per_launch"])
     if mfma:
         ach = c["flops"] / t / 1e12
         return name, {"bound": "mfma", "achieved": round(ach, 3), "peak": peak_f, "unit": "TFLOP/s",
@@ -123,6 +127,23 @@ def cpu_baseline(st, n_steps, fp64_one_step):
             "sample": "%d CNAB2 steps of the FP64 oracle at lmax=127 nr=65 (%.1f s)" % (n_steps, dt)}, err
 
 
+def copy_probe(nbytes):
+    """One plain copy kernel (copy_kernel) of nbytes, outside the timed region."""
+    import ctypes
+
+    from gdhip._arrays import context
+    from gdhip._lib import check, lib
+
+    ctx = context(0)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    check(lib().gd_device_alloc(ctx, nbytes, ctypes.byref(a)))
+    check(lib().gd_device_alloc(ctx, nbytes, ctypes.byref(b)))
+    check(lib().gd_device_copy(ctx, b, a, nbytes))
+    check(lib().gd_ctx_synchronize(ctx))
+    check(lib().gd_device_free(ctx, a))
+    check(lib().gd_device_free(ctx, b))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +152,8 @@ def main():
     ap.add_argument("--err-steps", type=int, default=100)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="also run one plain device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs the CNAB2 step on a single MI355X "
@@ -159,6 +182,8 @@ def main():
     err_mixed = rel_err(mixed.get_state_complex(), ref)
     e_mix = mixed.energies()
 
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
     t64 = timed(fp64, st, args.steps, args.warmup)
     tmx = timed(mixed, st, args.steps, args.warmup)
     prof = mixed.profile(5)

@@ -227,15 +227,33 @@ int gd_launch_cnab2_lmajor(hipStream_t st, const double *L, const double *LU, in
 }
 
 // ---------------------------------------------------------------- m-major fused CNAB2 for the step
-// One thread per (device idx, field) column pair (re, im).  Band values come
-// from the l-fastest transposed tables L_t / LU_t[(col*w + row)*nl + l] so a
-// wavefront's lanes (consecutive idx of one m, l stepping by 2) read nearly
-// contiguous words.  nu index of field f: {0,0,1,1,2}.  Field 4 (temperature)
-// has its boundary rows set to 0 before the rhs (thermal.jl:262-313).
-template <int BW>
-__global__ void __launch_bounds__(256) cnab2_step_kernel(const double *__restrict__ Lt, const double *__restrict__ LUt,
+// One thread per (device idx, field): the re and im columns share the band
+// values.  Inputs are read from st_in and the result written to st_out
+// (distinct buffers), so the loads of a block of D levels are independent of
+// the stores and are issued a block ahead (software pipelining; a column is a
+// sequential recurrence in r and a wavefront holds few columns, so memory
+// parallelism must come from within the thread).
+//   explicit part: L_l u = nu Lap u - nu l(l+1)/r^2 u; nu*Lap is the same for
+//                  every l and is read with uniform (scalar) loads
+//   LU(S_l):       from the l-fastest transposed table LU_t[(col*W + row)*nl + l]
+// nu index of field f: {0,0,1,1,2}.  Field 4 (temperature) has its boundary
+// rows set to 0 before the rhs (thermal.jl:262-313); fields in bc_mask have
+// Dirichlet rows (rhs rows 0 and n-1 := 0).
+namespace {
+template <int BW, int D> struct FwdBlk {
+    double ur[D], ui[D], nr_[D], ni[D], pr[D], pi[D], fl[D][BW];
+};
+template <int BW, int D> struct BwdBlk {
+    double yr[D], yi[D], bu[D][BW], dg[D];
+};
+} // namespace
+
+template <int BW, int D>
+__global__ void __launch_bounds__(256) cnab2_step_kernel(const double *__restrict__ nuLap, const double *__restrict__ rinv2,
+                                                         const double *__restrict__ nus, const double *__restrict__ LUt,
                                                          const int *__restrict__ l_of_idx, int nl, int n, int nlm,
-                                                         double dt, double theta, int bc_mask, double *__restrict__ st,
+                                                         double dt, double theta, int bc_mask,
+                                                         const double *__restrict__ st_in, double *__restrict__ st_out,
                                                          const double *__restrict__ nl_arr,
                                                          const double *__restrict__ np_arr)
 {
@@ -243,118 +261,175 @@ __global__ void __launch_bounds__(256) cnab2_step_kernel(const double *__restric
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     const int f = blockIdx.y;
     if (idx >= nlm) return;
-    const int nu_of[5] = {0, 0, 1, 1, 2};
-    const int k = nu_of[f];
+    const int k = f < 2 ? 0 : (f < 4 ? 1 : 2);
     const int l = l_of_idx[idx];
-    const size_t mat = (size_t)k * W * n * nl;
-    const double *Lk = Lt + mat, *LUk = LUt + mat;
+    const double nulf = nus[k] * ((double)l * (double)(l + 1));
+    const double *Lap = nuLap + (size_t)k * W * n;
+    const double *LU = LUt + (size_t)k * W * n * nl + l;
     const size_t fs = (size_t)nlm * n;
-    double *ure = st + (size_t)(2 * f) * fs, *uim = st + (size_t)(2 * f + 1) * fs;
-    const double *nre = nl_arr + (size_t)(2 * f) * fs, *nim = nl_arr + (size_t)(2 * f + 1) * fs;
-    const double *pre = np_arr + (size_t)(2 * f) * fs, *pim = np_arr + (size_t)(2 * f + 1) * fs;
+    const double *ure = st_in + (size_t)(2 * f) * fs + idx, *uim = ure + fs;
+    double *ore = st_out + (size_t)(2 * f) * fs + idx, *oim = ore + fs;
+    const double *nre = nl_arr + (size_t)(2 * f) * fs + idx, *nim = nre + fs;
+    const double *pre = np_arr + (size_t)(2 * f) * fs + idx, *pim = pre + fs;
     const bool tbc = (f == 4);
-    const bool bcr = (bc_mask >> f) & 1; // Dirichlet rows in this field's operator
+    const bool bcr = (bc_mask >> f) & 1;
     const double inv_dt = 1.0 / dt, lw = 1.0 - theta;
-    double wr[W], wi[W];   // window of original u, rows i-BW..i+BW
-    double yr[BW], yi[BW]; // last BW forward-substituted values
+    auto urow = [&](int j) { return j >= 0 && j < n && !(tbc && (j == 0 || j == n - 1)); };
+
+    double wr[W], wi[W]; // window of (boundary-adjusted) u, rows i-BW..i+BW
 #pragma unroll
     for (int q = 0; q < W; ++q) {
-        int j = q - BW;
-        bool ok = j >= 0 && j < n && !(tbc && (j == 0 || j == n - 1));
-        wr[q] = ok ? ure[idx + (size_t)nlm * j] : 0.0;
-        wi[q] = ok ? uim[idx + (size_t)nlm * j] : 0.0;
+        const int j = q - BW;
+        wr[q] = urow(j) ? ure[(size_t)nlm * j] : 0.0;
+        wi[q] = urow(j) ? uim[(size_t)nlm * j] : 0.0;
     }
+    double yr[BW], yi[BW];
 #pragma unroll
     for (int q = 0; q < BW; ++q) yr[q] = yi[q] = 0.0;
-    for (int i = 0; i < n; ++i) {
-        double lr = 0.0, li = 0.0;
+
+    auto fload = [&](FwdBlk<BW, D> &b, int i0) {
 #pragma unroll
-        for (int q = 0; q < W; ++q) {
-            int j = i + q - BW;
-            if (j >= 0 && j < n) {
-                double c = Lk[((size_t)j * W + (BW + i - j)) * nl + l];
-                lr += c * wr[q];
-                li += c * wi[q];
+        for (int d = 0; d < D; ++d) {
+            const int i = i0 + d, jn = i + BW + 1;
+            b.ur[d] = urow(jn) ? ure[(size_t)nlm * jn] : 0.0;
+            b.ui[d] = urow(jn) ? uim[(size_t)nlm * jn] : 0.0;
+            if (i < n) {
+                b.nr_[d] = nre[(size_t)nlm * i];
+                b.ni[d] = nim[(size_t)nlm * i];
+                b.pr[d] = pre[(size_t)nlm * i];
+                b.pi[d] = pim[(size_t)nlm * i];
+#pragma unroll
+                for (int q = 0; q < BW; ++q) {
+                    const int j = i - BW + q;
+                    b.fl[d][q] = j >= 0 ? LU[((size_t)j * W + (BW + i - j)) * nl] : 0.0;
+                }
             }
         }
-        const size_t oi = idx + (size_t)nlm * i;
-        double vr = inv_dt * wr[BW] + 1.5 * nre[oi] - 0.5 * pre[oi];
-        double vi = inv_dt * wi[BW] + 1.5 * nim[oi] - 0.5 * pim[oi];
-        if (lw != 0.0) {
-            vr += lw * lr;
-            vi += lw * li;
-        }
-        if (bcr && (i == 0 || i == n - 1)) vr = vi = 0.0; // Dirichlet value
-        // forward substitution with the unit-lower factor
+    };
+    auto fcompute = [&](const FwdBlk<BW, D> &b, int i0) {
 #pragma unroll
-        for (int q = 0; q < BW; ++q) {
-            int j = i - BW + q;
-            if (j >= 0) {
-                double c = LUk[((size_t)j * W + (BW + i - j)) * nl + l];
-                vr -= c * yr[q];
-                vi -= c * yi[q];
+        for (int d = 0; d < D; ++d) {
+            const int i = i0 + d;
+            if (i >= n) break;
+            double lr = 0.0, li = 0.0;
+#pragma unroll
+            for (int q = 0; q < W; ++q) {
+                const int j = i + q - BW;
+                if (j >= 0 && j < n) {
+                    const double c = (q == BW) ? Lap[BW + W * i] - nulf * rinv2[i] : Lap[(BW + i - j) + W * j];
+                    lr += c * wr[q];
+                    li += c * wi[q];
+                }
             }
-        }
+            double vr = inv_dt * wr[BW] + 1.5 * b.nr_[d] - 0.5 * b.pr[d];
+            double vi = inv_dt * wi[BW] + 1.5 * b.ni[d] - 0.5 * b.pi[d];
+            if (lw != 0.0) {
+                vr += lw * lr;
+                vi += lw * li;
+            }
+            if (bcr && (i == 0 || i == n - 1)) vr = vi = 0.0; // Dirichlet value
 #pragma unroll
-        for (int q = 0; q < BW - 1; ++q) {
-            yr[q] = yr[q + 1];
-            yi[q] = yi[q + 1];
-        }
-        yr[BW - 1] = vr;
-        yi[BW - 1] = vi;
+            for (int q = 0; q < BW; ++q) {
+                vr -= b.fl[d][q] * yr[q];
+                vi -= b.fl[d][q] * yi[q];
+            }
 #pragma unroll
-        for (int q = 0; q < W - 1; ++q) {
-            wr[q] = wr[q + 1];
-            wi[q] = wi[q + 1];
+            for (int q = 0; q < BW - 1; ++q) {
+                yr[q] = yr[q + 1];
+                yi[q] = yi[q + 1];
+            }
+            yr[BW - 1] = vr;
+            yi[BW - 1] = vi;
+#pragma unroll
+            for (int q = 0; q < W - 1; ++q) {
+                wr[q] = wr[q + 1];
+                wi[q] = wi[q + 1];
+            }
+            wr[W - 1] = b.ur[d];
+            wi[W - 1] = b.ui[d];
+            ore[(size_t)nlm * i] = vr;
+            oim[(size_t)nlm * i] = vi;
         }
-        int jn = i + BW + 1;
-        bool ok = jn < n && !(tbc && jn == n - 1);
-        wr[W - 1] = ok ? ure[idx + (size_t)nlm * jn] : 0.0;
-        wi[W - 1] = ok ? uim[idx + (size_t)nlm * jn] : 0.0;
-        ure[oi] = vr;
-        uim[oi] = vi;
+    };
+    {
+        FwdBlk<BW, D> A, B;
+        fload(A, 0);
+        for (int i0 = 0; i0 < n; i0 += 2 * D) {
+            fload(B, i0 + D);
+            fcompute(A, i0);
+            fload(A, i0 + 2 * D);
+            fcompute(B, i0 + D);
+        }
     }
-    // back substitution
+
+    // back substitution, blocks of D rows going up
     double xr[BW], xi[BW];
 #pragma unroll
     for (int q = 0; q < BW; ++q) xr[q] = xi[q] = 0.0;
-    for (int i = n - 1; i >= 0; --i) {
-        const size_t oi = idx + (size_t)nlm * i;
-        double vr = ure[oi], vi = uim[oi];
+    auto bload = [&](BwdBlk<BW, D> &b, int i0) { // rows i0, i0-1, ..., i0-D+1
 #pragma unroll
-        for (int q = 0; q < BW; ++q) {
-            int j = i + 1 + q;
-            if (j < n) {
-                double c = LUk[((size_t)j * W + (BW + i - j)) * nl + l];
-                vr -= c * xr[q];
-                vi -= c * xi[q];
+        for (int d = 0; d < D; ++d) {
+            const int i = i0 - d;
+            if (i >= 0) {
+                b.yr[d] = ore[(size_t)nlm * i];
+                b.yi[d] = oim[(size_t)nlm * i];
+                b.dg[d] = LU[((size_t)i * W + BW) * nl];
+#pragma unroll
+                for (int q = 0; q < BW; ++q) {
+                    const int j = i + 1 + q;
+                    b.bu[d][q] = j < n ? LU[((size_t)j * W + (BW + i - j)) * nl] : 0.0;
+                }
             }
         }
-        double d = LUk[((size_t)i * W + BW) * nl + l];
-        vr /= d;
-        vi /= d;
+    };
+    auto bcompute = [&](const BwdBlk<BW, D> &b, int i0) {
 #pragma unroll
-        for (int q = BW - 1; q > 0; --q) {
-            xr[q] = xr[q - 1];
-            xi[q] = xi[q - 1];
+        for (int d = 0; d < D; ++d) {
+            const int i = i0 - d;
+            if (i < 0) break;
+            double vr = b.yr[d], vi = b.yi[d];
+#pragma unroll
+            for (int q = 0; q < BW; ++q) {
+                vr -= b.bu[d][q] * xr[q];
+                vi -= b.bu[d][q] * xi[q];
+            }
+            vr /= b.dg[d];
+            vi /= b.dg[d];
+#pragma unroll
+            for (int q = BW - 1; q > 0; --q) {
+                xr[q] = xr[q - 1];
+                xi[q] = xi[q - 1];
+            }
+            xr[0] = vr;
+            xi[0] = vi;
+            ore[(size_t)nlm * i] = vr;
+            oim[(size_t)nlm * i] = vi;
         }
-        xr[0] = vr;
-        xi[0] = vi;
-        ure[oi] = vr;
-        uim[oi] = vi;
+    };
+    {
+        BwdBlk<BW, D> A, B;
+        bload(A, n - 1);
+        for (int i0 = n - 1; i0 >= 0; i0 -= 2 * D) {
+            bload(B, i0 - D);
+            bcompute(A, i0);
+            bload(A, i0 - 2 * D);
+            bcompute(B, i0 - D);
+        }
     }
 }
 
-int gd_launch_cnab2_step(hipStream_t st, const double *Lt, const double *LUt, const int *l_of_idx, int nl, int n,
-                         int bw, int nlm, double dt, double theta, int bc_mask, double *state, const double *nl_arr,
+int gd_launch_cnab2_step(hipStream_t st, const double *nuLap, const double *rinv2, const double *nus,
+                         const double *LUt, const int *l_of_idx, int nl, int n, int bw, int nlm, double dt,
+                         double theta, int bc_mask, const double *st_in, double *st_out, const double *nl_arr,
                          const double *np_arr)
 {
+    GD_CHECK(st_in != st_out, GD_ERR_ARG, "cnab2_step: in and out state must differ");
     dim3 grid((nlm + 255) / 256, 5);
     switch (bw) {
-#define CASE(B)                                                                                             \
-    case B:                                                                                                 \
-        hipLaunchKernelGGL(cnab2_step_kernel<B>, grid, dim3(256), 0, st, Lt, LUt, l_of_idx, nl, n, nlm, dt, \
-                           theta, bc_mask, state, nl_arr, np_arr);                                          \
+#define CASE(B)                                                                                                \
+    case B:                                                                                                    \
+        hipLaunchKernelGGL((cnab2_step_kernel<B, 4>), grid, dim3(256), 0, st, nuLap, rinv2, nus, LUt, l_of_idx, \
+                           nl, n, nlm, dt, theta, bc_mask, st_in, st_out, nl_arr, np_arr);                      \
         break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
 #undef CASE
@@ -462,4 +537,40 @@ int gd_launch_band_solve(hipStream_t st, const double *lu, const int *sys_of_rhs
     hipLaunchKernelGGL(band_solve_kernel, dim3((nrhs + 63) / 64), dim3(64), 0, st, lu, sys_of_rhs, n, bw, nsys,
                        nrhs, x, d_status);
     return gd_check_launch("band_solve_kernel");
+}
+
+// ---------------------------------------------------------------- plain device copy
+// 16-byte vector copy (grid-stride); also the known-size reference kernel for
+// HBM byte counters.
+__global__ void __launch_bounds__(256) copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16)
+{
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n16; q += (size_t)gridDim.x * blockDim.x)
+        dst[q] = src[q];
+}
+
+extern "C" int gd_device_copy(gd_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    GD_CHECK(ctx && dst && src && bytes % 16 == 0, GD_ERR_ARG, "gd_device_copy: bytes must be a multiple of 16");
+    GD_HIP(hipSetDevice(ctx->device));
+    const size_t n16 = bytes / 16;
+    const unsigned grid = (unsigned)((n16 + 255) / 256 < 65536 ? (n16 + 255) / 256 : 65536);
+    if (n16) hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, ctx->stream, (const uint4 *)src, (uint4 *)dst, n16);
+    return gd_check_launch("copy_kernel");
+}
+
+extern "C" int gd_device_alloc(gd_ctx *ctx, size_t bytes, void **out)
+{
+    GD_CHECK(ctx && out, GD_ERR_ARG, "gd_device_alloc: null argument");
+    GD_HIP(hipSetDevice(ctx->device));
+    GD_HIP(hipMalloc(out, bytes));
+    return GD_OK;
+}
+
+extern "C" int gd_device_free(gd_ctx *ctx, void *p)
+{
+    GD_CHECK(ctx, GD_ERR_ARG, "gd_device_free: null ctx");
+    GD_HIP(hipSetDevice(ctx->device));
+    GD_HIP(hipStreamSynchronize(ctx->stream));
+    GD_HIP(hipFree(p));
+    return GD_OK;
 }

@@ -13,8 +13,9 @@ int gd_launch_cnab2_lmajor(hipStream_t st, const double *L, const double *LU, in
                            int n, int bw, double dt, double theta, int mode, int bc_rows, const double *u_re,
                            const double *u_im, const double *nl_re, const double *nl_im, const double *np_re,
                            const double *np_im, double *out_re, double *out_im, int *d_status);
-int gd_launch_cnab2_step(hipStream_t st, const double *Lt, const double *LUt, const int *l_of_idx, int nl, int n,
-                         int bw, int nlm, double dt, double theta, int bc_mask, double *state, const double *nl_arr,
+int gd_launch_cnab2_step(hipStream_t st, const double *nuLap, const double *rinv2, const double *nus,
+                         const double *LUt, const int *l_of_idx, int nl, int n, int bw, int nlm, double dt,
+                         double theta, int bc_mask, const double *st_in, double *st_out, const double *nl_arr,
                          const double *np_arr);
 int gd_launch_energies(hipStream_t st, const double *state, const int *l_of_idx, int nlm, int n, double *part,
                        int nblk, double *out3);

@@ -28,8 +28,9 @@ struct gd_step {
     gd_sht_plan sht;
     int nlm = 0, nl = 0, W = 0;
     size_t fs = 0;                 // nlm * nr
-    DevBuf state, nlbuf[2];        // m-major FP64 [field][re/im][r][idx]
+    DevBuf state[2], nlbuf[2];     // m-major FP64 [field][re/im][r][idx]; step k reads state[k&1]
     DevBuf Lt, LUt;                // [nu][(col*W+row)][l], nu = velocity, magnetic, temperature
+    DevBuf nuLap, nus;             // [nu][W*nr] nu * Laplacian band; the three diffusivities
     DevBuf D1, D2, rtab;
     DevBuf synV, synS, fb, gb;
     DevBuf part, eout, scratch;
@@ -100,6 +101,8 @@ __global__ void __launch_bounds__(256) spectral_pre_kernel(const double *__restr
 }
 
 // ---------------------------------------------------------------- orchestration
+static double *cur_state(gd_step *s) { return s->state[s->step & 1].as<double>(); }
+
 // ev (optional): events recorded after each kernel class, ev[0] before the first
 #define GD_MARK(k)                                                  \
     do {                                                            \
@@ -113,7 +116,7 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEven
     gd_sht_plan *p = &s->sht;
     const int nr = s->d.nr, nlm = s->nlm;
     hipLaunchKernelGGL((spectral_pre_kernel<T>), dim3((nlm + 255) / 256, nr), dim3(256), 0, st,
-                       s->state.as<double>(), p->d_l_of_idx.as<int>(), s->D1.as<double>(), s->D2.as<double>(),
+                       cur_state(s), p->d_l_of_idx.as<int>(), s->D1.as<double>(), s->D2.as<double>(),
                        s->rtab.as<double>(), nlm, nr, s->d.bw, s->d.radials, s->synV.as<T>(), s->synS.as<T>());
     GD_TRY(gd_check_launch("spectral_pre_kernel"));
     GD_MARK(1);
@@ -227,9 +230,10 @@ static int enqueue_step(gd_step *s, int parity, bool boot, hipEvent_t *ev = null
     double *cur = s->nlbuf[parity].as<double>();
     double *prev = boot ? cur : s->nlbuf[parity ^ 1].as<double>();
     GD_TRY(nonlinear(s, cur, ev));
-    GD_TRY(gd_launch_cnab2_step(s->ctx->stream, s->Lt.as<double>(), s->LUt.as<double>(), s->sht.d_l_of_idx.as<int>(),
-                                s->nl, s->d.nr, s->d.bw, s->nlm, s->d.dt, s->d.theta, bc_mask(s), s->state.as<double>(),
-                                cur, prev));
+    GD_TRY(gd_launch_cnab2_step(s->ctx->stream, s->nuLap.as<double>(), s->rtab.as<double>() + s->d.nr,
+                                s->nus.as<double>(), s->LUt.as<double>(), s->sht.d_l_of_idx.as<int>(), s->nl, s->d.nr,
+                                s->d.bw, s->nlm, s->d.dt, s->d.theta, bc_mask(s), s->state[parity].as<double>(),
+                                s->state[parity ^ 1].as<double>(), cur, prev));
     GD_MARK(7);
     return GD_OK;
 }
@@ -247,7 +251,8 @@ static void destroy_graphs(gd_step *s)
 static int step_release(gd_step *s)
 {
     destroy_graphs(s);
-    DevBuf *bufs[] = {&s->state, &s->nlbuf[0], &s->nlbuf[1], &s->Lt, &s->LUt, &s->D1, &s->D2, &s->rtab, &s->synV,
+    DevBuf *bufs[] = {&s->state[0], &s->state[1], &s->nlbuf[0], &s->nlbuf[1], &s->Lt, &s->LUt, &s->nuLap, &s->nus,
+                      &s->D1, &s->D2, &s->rtab, &s->synV,
                       &s->synS, &s->fb, &s->gb, &s->part, &s->eout, &s->scratch, &s->status};
     for (DevBuf *b : bufs) b->release();
     gd_sht_plan_release(&s->sht);
@@ -297,6 +302,13 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
     if (rc == GD_OK) rc = s->status.ensure(sizeof(int));
     if (rc == GD_OK && hipMemsetAsync(s->status.p, 0, sizeof(int), ctx->stream) != hipSuccess) rc = GD_ERR_HIP;
     const double nus[3] = {d->Pm, 1.0, d->Pm / d->Pr};
+    {
+        std::vector<double> nl3(nus, nus + 3), nlap((size_t)3 * s->W * nr);
+        for (int k = 0; k < 3; ++k)
+            for (size_t q = 0; q < (size_t)s->W * nr; ++q) nlap[k * s->W * nr + q] = nus[k] * lap[q];
+        if (rc == GD_OK) rc = upload_vec(ctx, s->nus, nl3);
+        if (rc == GD_OK) rc = upload_vec(ctx, s->nuLap, nlap);
+    }
     GD_CHECK(d->bc_rows >= 0 && d->bc_rows <= 2, GD_ERR_ARG, "gd_step_create: bc_rows=%d", d->bc_rows);
     for (int k = 0; k < 3 && rc == GD_OK; ++k)
         rc = gd_cnab2_build(ctx->stream, dlap.as<double>(), s->rtab.as<double>() + nr, nus[k], d->dt, d->theta, s->nl,
@@ -315,7 +327,8 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
     // state and work buffers
     const size_t esz = d->prec == GD_PREC_FP64 ? sizeof(double) : sizeof(float);
     const int nm = d->mmax + 1;
-    GD_TRY(s->state.ensure(sizeof(double) * s->fs * 10));
+    GD_TRY(s->state[0].ensure(sizeof(double) * s->fs * 10));
+    GD_TRY(s->state[1].ensure(sizeof(double) * s->fs * 10));
     GD_TRY(s->nlbuf[0].ensure(sizeof(double) * s->fs * 10));
     GD_TRY(s->nlbuf[1].ensure(sizeof(double) * s->fs * 10));
     GD_TRY(s->synV.ensure(esz * s->fs * 18));
@@ -324,7 +337,8 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
     GD_TRY(s->gb.ensure(esz * (size_t)nm * s->sht.nlat2 * nr * 20));
     GD_TRY(s->part.ensure(sizeof(double) * 3 * 512));
     GD_TRY(s->eout.ensure(sizeof(double) * 4));
-    GD_HIP(hipMemsetAsync(s->state.p, 0, sizeof(double) * s->fs * 10, ctx->stream));
+    GD_HIP(hipMemsetAsync(s->state[0].p, 0, sizeof(double) * s->fs * 10, ctx->stream));
+    GD_HIP(hipMemsetAsync(s->state[1].p, 0, sizeof(double) * s->fs * 10, ctx->stream));
     GD_HIP(hipMemsetAsync(s->nlbuf[0].p, 0, sizeof(double) * s->fs * 10, ctx->stream));
     GD_HIP(hipMemsetAsync(s->nlbuf[1].p, 0, sizeof(double) * s->fs * 10, ctx->stream));
     GD_HIP(hipStreamSynchronize(ctx->stream));
@@ -388,9 +402,9 @@ extern "C" int gd_step_set_state(gd_step *s, const double *re, const double *im,
         a = s->scratch.as<double>();
         b = a + n;
     }
-    GD_TRY(convert_state(s, a, b, s->state.as<double>(), true));
-    if (flags & GD_HOST) GD_HIP(hipStreamSynchronize(s->ctx->stream));
     s->step = 0;
+    GD_TRY(convert_state(s, a, b, cur_state(s), true));
+    if (flags & GD_HOST) GD_HIP(hipStreamSynchronize(s->ctx->stream));
     return GD_OK;
 }
 
@@ -415,7 +429,7 @@ static int get_planes(gd_step *s, const double *dev, double *re, double *im, int
 extern "C" int gd_step_get_state(gd_step *s, double *re, double *im, int flags)
 {
     GD_CHECK(s && re && im, GD_ERR_ARG, "gd_step_get_state: null argument");
-    return get_planes(s, s->state.as<double>(), re, im, flags);
+    return get_planes(s, cur_state(s), re, im, flags);
 }
 
 extern "C" int gd_step_nonlinear(gd_step *s, double *nl_re, double *nl_im, int flags)
@@ -479,7 +493,7 @@ extern "C" int gd_step_energies(gd_step *s, double *out3)
 {
     GD_CHECK(s && out3, GD_ERR_ARG, "gd_step_energies: null argument");
     GD_HIP(hipSetDevice(s->ctx->device));
-    GD_TRY(gd_launch_energies(s->ctx->stream, s->state.as<double>(), s->sht.d_l_of_idx.as<int>(), s->nlm, s->d.nr,
+    GD_TRY(gd_launch_energies(s->ctx->stream, cur_state(s), s->sht.d_l_of_idx.as<int>(), s->nlm, s->d.nr,
                               s->part.as<double>(), 512, s->eout.as<double>()));
     return gd_from_device(s->ctx, out3, s->eout.p, sizeof(double) * 3);
 }

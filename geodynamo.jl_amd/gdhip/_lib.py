@@ -18,7 +18,7 @@ GD_FD_LITERAL, GD_FD_TRANSPOSE = 0, 1
 # every symbol include/gdhip.h declares
 EXPORTS = [
     "gd_last_error", "gd_version", "gd_ctx_create", "gd_ctx_set_stream", "gd_ctx_synchronize",
-    "gd_ctx_destroy", "gd_seeded_state", "gd_radial_grid", "gd_derivative_matrix", "gd_radial_laplacian",
+    "gd_ctx_destroy", "gd_device_alloc", "gd_device_free", "gd_device_copy", "gd_seeded_state", "gd_radial_grid", "gd_derivative_matrix", "gd_radial_laplacian",
     "gd_banded_factor", "gd_banded_solve", "gd_cnab2_mats_create", "gd_cnab2_mats_get",
     "gd_cnab2_mats_destroy", "gd_build_rhs_cnab2", "gd_solve_implicit_step", "gd_cnab2_rhs_solve",
     "gd_sht_plan_create", "gd_sht_plan_destroy", "gd_sht_nlm", "gd_sht_grid", "gd_sht_tables",
@@ -68,6 +68,9 @@ def lib():
             "gd_ctx_set_stream": (i, [vp, vp]),
             "gd_ctx_synchronize": (i, [vp]),
             "gd_ctx_destroy": (i, [vp]),
+            "gd_device_alloc": (i, [vp, ctypes.c_size_t, pp]),
+            "gd_device_free": (i, [vp, vp]),
+            "gd_device_copy": (i, [vp, vp, vp, ctypes.c_size_t]),
             "gd_radial_grid": (i, [i, d, vp]),
             "gd_seeded_state": (i, [ctypes.c_ulonglong, i, i, i, i, d, d, i, vp, vp]),
             "gd_derivative_matrix": (i, [i, i, i, vp, i, vp]),

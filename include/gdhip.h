@@ -53,6 +53,12 @@ int gd_ctx_create(int device, gd_ctx **out);
 int gd_ctx_set_stream(gd_ctx *ctx, void *hip_stream);
 int gd_ctx_synchronize(gd_ctx *ctx);
 int gd_ctx_destroy(gd_ctx *ctx);
+/* Device memory helpers for callers without their own allocator (no
+ * counterpart in the reference): hipMalloc/hipFree and a plain 16-byte-vector
+ * copy kernel on the context's stream (bytes a multiple of 16). */
+int gd_device_alloc(gd_ctx *ctx, size_t bytes, void **out);
+int gd_device_free(gd_ctx *ctx, void *p);
+int gd_device_copy(gd_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 /* ---- radial domain and FD operators (host FP64 set-up) -------------------
  * replaces create_radial_domain (fields.jl:108-131),
