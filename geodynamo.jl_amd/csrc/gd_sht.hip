@@ -87,49 +87,90 @@ template <typename T> using acc_t = typename Acc4<T>::type;
 
 template <typename T> __device__ __forceinline__ T shfl_xor3(T v) { return __shfl_xor(v, 3); }
 
-// One parity group of the synthesis contraction for a (m, node tile, level tile).
-template <typename T, int NT>
-__device__ __forceinline__ void synth_group(const SynthArgs &a, const T *__restrict__ in, const T *__restrict__ tab0,
-                                            const T *__restrict__ tab1, T *sA, T *sB, const int *scol, int kbase,
-                                            int Kp, int i0, int r0, int VC, int NTILE, acc_t<T> (&acc)[NT][2])
+// Synthesis contraction for one (m, node tile, level tile), both parity
+// groups: the K chunks of the even group, then of the odd group, run as one
+// software-pipelined sequence -- chunk c+1 is loaded from global memory into
+// registers while the MFMAs consume chunk c from LDS (double-buffered LDS, one
+// barrier per chunk).
+template <typename T, int NT, int NF>
+__device__ __forceinline__ void synth_pipeline(const SynthArgs &a, const T *__restrict__ in, const T *__restrict__ tab0,
+                                               const T *__restrict__ tab1, T *sA, T *sB, const int *scol, int m,
+                                               int i0, int r0, int VC, int NTILE, acc_t<T> (&accE)[NT][2],
+                                               acc_t<T> (&accO)[NT][2])
 {
+    constexpr int RA = (NT * KT * TM + 64 * NF - 1) / (64 * NF); // A elements per thread (64*NF threads)
+    constexpr int RB = KT * 16 / 64;                             // B elements per thread (NTILE = 16*NF)
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int wave = tid >> 6, lane = tid & 63;
-    for (int k0 = 0; k0 < Kp; k0 += KT) {
-        for (int e = tid; e < NT * KT * TM; e += nthr) {
-            const int i = e % TM, k = (e / TM) % KT, t = e / (TM * KT);
+    const int K = a.moff[m + 1] - a.moff[m], ne = a.ne[m];
+    const int nE = (ne + KT - 1) / KT, nC = nE + (K - ne + KT - 1) / KT;
+    const int sAsz = NT * KT * TM, sBsz = KT * NTILE;
+    T ra[RA], rb[RB];
+    auto gload = [&](int c) {
+        const int p = c < nE ? 0 : 1;
+        const int k0 = (p ? c - nE : c) * KT;
+        const int Kp = p ? K - ne : ne;
+        const int kbase = a.moff[m] + (p ? ne : 0);
+#pragma unroll
+        for (int q = 0; q < RA; ++q) {
+            const int e = tid + q * nthr;
             T v = T(0);
-            if (k0 + k < Kp && i0 + i < a.nlat2) {
-                const T *tb = (t == 0) ? tab0 : tab1;
-                v = tb[(size_t)(kbase + k0 + k) * a.nlat2 + i0 + i];
+            if (e < sAsz) {
+                const int i = e % TM, k = (e / TM) % KT, t = e / (TM * KT);
+                if (k0 + k < Kp && i0 + i < a.nlat2) v = (t == 0 ? tab0 : tab1)[(size_t)(kbase + k0 + k) * a.nlat2 + i0 + i];
             }
-            sA[(t * KT + k) * TM + i] = v;
+            ra[q] = v;
         }
-        for (int e = tid; e < KT * NTILE; e += nthr) {
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+            const int e = tid + q * nthr;
             const int k = e % KT, vv = e / KT;
             const int lv = vv / VC, fv = vv - lv * VC;
-            const int c = scol[fv], r = r0 + lv;
+            const int col = scol[fv], r = r0 + lv;
             T v = T(0);
-            if (c >= 0 && r < a.nr && k0 + k < Kp) v = in[(size_t)(r * a.cin + c) * a.nlm + kbase + k0 + k];
-            sB[k * NTILE + vv] = v;
+            if (col >= 0 && r < a.nr && k0 + k < Kp) v = in[(size_t)(r * a.cin + col) * a.nlm + kbase + k0 + k];
+            rb[q] = v;
         }
-        __syncthreads();
+    };
+    auto sstore = [&](int b) {
+        T *A = sA + b * sAsz, *B = sB + b * sBsz;
+#pragma unroll
+        for (int q = 0; q < RA; ++q) {
+            const int e = tid + q * nthr;
+            if (e < sAsz) A[e] = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+            const int e = tid + q * nthr;
+            const int k = e % KT, vv = e / KT;
+            B[k * NTILE + vv] = rb[q];
+        }
+    };
+    auto mma = [&](int b, acc_t<T>(&acc)[NT][2]) {
+        const T *A = sA + b * sAsz, *B = sB + b * sBsz;
 #pragma unroll
         for (int ks = 0; ks < KT; ks += 4) {
-            const T b = sB[(ks + (lane >> 4)) * NTILE + wave * 16 + (lane & 15)];
+            const T bv = B[(ks + (lane >> 4)) * NTILE + wave * 16 + (lane & 15)];
 #pragma unroll
             for (int t = 0; t < NT; ++t)
 #pragma unroll
-                for (int rb = 0; rb < 2; ++rb) {
-                    const T av = sA[(t * KT + ks + (lane >> 4)) * TM + rb * 16 + (lane & 15)];
-                    acc[t][rb] = mfma16x16x4(av, b, acc[t][rb]);
+                for (int rb_ = 0; rb_ < 2; ++rb_) {
+                    const T av = A[(t * KT + ks + (lane >> 4)) * TM + rb_ * 16 + (lane & 15)];
+                    acc[t][rb_] = mfma16x16x4(av, bv, acc[t][rb_]);
                 }
         }
+    };
+    if (nC > 0) gload(0);
+    for (int c = 0; c < nC; ++c) {
+        sstore(c & 1);
         __syncthreads();
+        if (c + 1 < nC) gload(c + 1);
+        if (c < nE) mma(c & 1, accE);
+        else mma(c & 1, accO);
     }
 }
 
-template <typename T, bool VEC> __global__ void __launch_bounds__(320) synth_kernel(SynthArgs a)
+template <typename T, bool VEC, int NF> __global__ void __launch_bounds__(64 * NF) synth_kernel(SynthArgs a)
 {
     constexpr int NT = VEC ? 2 : 1;
     extern __shared__ char smem_raw[];
@@ -137,8 +178,8 @@ template <typename T, bool VEC> __global__ void __launch_bounds__(320) synth_ker
     const int NTILE = 16 * a.nf;
     const int LT = NTILE / VC;
     T *sA = reinterpret_cast<T *>(smem_raw);
-    T *sB = sA + NT * KT * TM;
-    int *scol = reinterpret_cast<int *>(sB + KT * NTILE);
+    T *sB = sA + 2 * NT * KT * TM;
+    int *scol = reinterpret_cast<int *>(sB + 2 * KT * NTILE);
     const int m = blockIdx.z, i0 = blockIdx.x * TM, r0 = blockIdx.y * LT;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     if (tid < 20) scol[tid] = a.colmap[tid];
@@ -146,7 +187,6 @@ template <typename T, bool VEC> __global__ void __launch_bounds__(320) synth_ker
     const T *in = static_cast<const T *>(a.in);
     const T *tab0 = static_cast<const T *>(a.tab0);
     const T *tab1 = static_cast<const T *>(a.tab1);
-    const int K = a.moff[m + 1] - a.moff[m], ne = a.ne[m];
     acc_t<T> accE[NT][2], accO[NT][2];
 #pragma unroll
     for (int t = 0; t < NT; ++t)
@@ -155,8 +195,7 @@ template <typename T, bool VEC> __global__ void __launch_bounds__(320) synth_ker
             accE[t][rb] = acc_t<T>{0, 0, 0, 0};
             accO[t][rb] = acc_t<T>{0, 0, 0, 0};
         }
-    synth_group<T, NT>(a, in, tab0, tab1, sA, sB, scol, a.moff[m], ne, i0, r0, VC, NTILE, accE);
-    synth_group<T, NT>(a, in, tab0, tab1, sA, sB, scol, a.moff[m] + ne, K - ne, i0, r0, VC, NTILE, accO);
+    synth_pipeline<T, NT, NF>(a, in, tab0, tab1, sA, sB, scol, m, i0, r0, VC, NTILE, accE, accO);
 
     T *out = static_cast<T *>(a.out);
     const int vcol = wave * 16 + (lane & 15);
@@ -205,7 +244,7 @@ template <typename T, bool VEC> __global__ void __launch_bounds__(320) synth_ker
 }
 
 // ---------------------------------------------------------------- analysis
-template <typename T, bool VEC> __global__ void __launch_bounds__(320) ana_kernel(AnaArgs a)
+template <typename T, bool VEC, int NF> __global__ void __launch_bounds__(64 * NF) ana_kernel(AnaArgs a)
 {
     constexpr int NT = VEC ? 2 : 1;
     extern __shared__ char smem_raw[];
@@ -213,7 +252,7 @@ template <typename T, bool VEC> __global__ void __launch_bounds__(320) ana_kerne
     const int NTILE = 16 * a.nf;
     const int LT = NTILE / VC;
     T *sA = reinterpret_cast<T *>(smem_raw);
-    T *sB = sA + NT * KT * TM;
+    T *sB = sA + 2 * NT * KT * TM; // A and B are double-buffered
     const int m = blockIdx.z >> 1, p = blockIdx.z & 1;
     const int K = a.moff[m + 1] - a.moff[m], ne = a.ne[m];
     const int Kp = p ? K - ne : ne;
@@ -229,48 +268,84 @@ template <typename T, bool VEC> __global__ void __launch_bounds__(320) ana_kerne
 #pragma unroll
         for (int rb = 0; rb < 2; ++rb) acc[t][rb] = acc_t<T>{0, 0, 0, 0};
 
-    for (int k0 = 0; k0 < a.nlat2; k0 += KT) {
-        for (int e = tid; e < NT * KT * TM; e += nthr) {
-            const int kk = e % KT, i = (e / KT) % TM, t = e / (KT * TM);
-            const int row = row0 + i;
+    // software-pipelined K loop over the northern nodes (see synth_pipeline)
+    constexpr int RA = (NT * KT * TM + 64 * NF - 1) / (64 * NF);
+    constexpr int RB = NT * KT * 16 / 64;
+    const int sAsz = NT * KT * TM, sBsz = NT * KT * NTILE;
+    const int nC = (a.nlat2 + KT - 1) / KT;
+    T ra[RA], rb[RB];
+    auto gload = [&](int c) {
+        const int k0 = c * KT;
+#pragma unroll
+        for (int q = 0; q < RA; ++q) {
+            const int e = tid + q * nthr;
             T v = T(0);
-            if (row < Kp && k0 + kk < a.nlat2) v = (t == 0 ? tab0 : tab1)[(size_t)(kbase + row) * a.nlat2 + k0 + kk];
-            sA[(t * KT + kk) * TM + i] = v;
-        }
-        for (int e = tid; e < NT * KT * NTILE; e += nthr) {
-            const int vv = e % NTILE, kk = (e / NTILE) % KT, t = e / (NTILE * KT);
-            const int lv = vv / VC, fv = vv - lv * VC, r = r0 + lv;
-            int comp, reim, pm;
-            if (VEC) {
-                const int V = fv >> 2, q = fv & 3;
-                comp = 2 * V + (q >> 1);
-                reim = q & 1;
-                // dP has southern sign -(-1)^(l+m): even group uses G-, odd G+; Q the reverse
-                pm = (t == 0) ? (p == 0 ? 1 : 0) : p;
-            } else {
-                comp = fv >> 1;
-                reim = fv & 1;
-                pm = p;
+            if (e < sAsz) {
+                const int kk = e % KT, i = (e / KT) % TM, t = e / (KT * TM);
+                const int row = row0 + i;
+                if (row < Kp && k0 + kk < a.nlat2) v = (t == 0 ? tab0 : tab1)[(size_t)(kbase + row) * a.nlat2 + k0 + kk];
             }
-            T v = T(0);
-            if (r < a.nr && k0 + kk < a.nlat2)
-                v = gb[(((size_t)m * a.nlat2 + k0 + kk) * a.nr + r) * a.nslot_g + a.gslot0 + 4 * comp + 2 * pm + reim];
-            sB[(t * KT + kk) * NTILE + vv] = v;
+            ra[q] = v;
         }
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+            const int e = tid + q * nthr;
+            T v = T(0);
+            if (e < sBsz) {
+                const int vv = e % NTILE, kk = (e / NTILE) % KT, t = e / (NTILE * KT);
+                const int lv = vv / VC, fv = vv - lv * VC, r = r0 + lv;
+                int comp, reim, pm;
+                if (VEC) {
+                    const int V = fv >> 2, q4 = fv & 3;
+                    comp = 2 * V + (q4 >> 1);
+                    reim = q4 & 1;
+                    // dP has southern sign -(-1)^(l+m): even group uses G-, odd G+; Q the reverse
+                    pm = (t == 0) ? (p == 0 ? 1 : 0) : p;
+                } else {
+                    comp = fv >> 1;
+                    reim = fv & 1;
+                    pm = p;
+                }
+                if (r < a.nr && k0 + kk < a.nlat2)
+                    v = gb[(((size_t)m * a.nlat2 + k0 + kk) * a.nr + r) * a.nslot_g + a.gslot0 + 4 * comp + 2 * pm + reim];
+            }
+            rb[q] = v;
+        }
+    };
+    auto sstore = [&](int b) {
+        T *A = sA + b * sAsz, *B = sB + b * sBsz;
+#pragma unroll
+        for (int q = 0; q < RA; ++q) {
+            const int e = tid + q * nthr;
+            if (e < sAsz) {
+                const int kk = e % KT, i = (e / KT) % TM, t = e / (KT * TM);
+                A[(t * KT + kk) * TM + i] = ra[q];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < RB; ++q) {
+            const int e = tid + q * nthr;
+            if (e < sBsz) B[e] = rb[q];
+        }
+    };
+    if (nC > 0) gload(0);
+    for (int c = 0; c < nC; ++c) {
+        sstore(c & 1);
         __syncthreads();
+        if (c + 1 < nC) gload(c + 1);
+        const T *A = sA + (c & 1) * sAsz, *B = sB + (c & 1) * sBsz;
 #pragma unroll
         for (int ks = 0; ks < KT; ks += 4) {
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
-                const T b = sB[(t * KT + ks + (lane >> 4)) * NTILE + wave * 16 + (lane & 15)];
+                const T b = B[(t * KT + ks + (lane >> 4)) * NTILE + wave * 16 + (lane & 15)];
 #pragma unroll
-                for (int rb = 0; rb < 2; ++rb) {
-                    const T av = sA[(t * KT + ks + (lane >> 4)) * TM + rb * 16 + (lane & 15)];
-                    acc[t][rb] = mfma16x16x4(av, b, acc[t][rb]);
+                for (int rb_ = 0; rb_ < 2; ++rb_) {
+                    const T av = A[(t * KT + ks + (lane >> 4)) * TM + rb_ * 16 + (lane & 15)];
+                    acc[t][rb_] = mfma16x16x4(av, b, acc[t][rb_]);
                 }
             }
         }
-        __syncthreads();
     }
 
     const int vcol = wave * 16 + (lane & 15);
@@ -320,8 +395,10 @@ template <typename T> int gd_launch_synth_vec(hipStream_t st, const SynthArgs &a
     GD_CHECK(a.nf >= 1 && a.nf <= 5, GD_ERR_ARG, "synth_vec: nf=%d", a.nf);
     const int LT = 4;
     dim3 grid((a.nlat2 + TM - 1) / TM, (a.nr + LT - 1) / LT, a.mmax + 1);
-    size_t lds = sizeof(T) * (2 * KT * TM + KT * 16 * a.nf) + 32 * sizeof(int);
-    hipLaunchKernelGGL((synth_kernel<T, true>), grid, dim3(64 * a.nf), lds, st, a);
+    size_t lds = sizeof(T) * 2 * (2 * KT * TM + KT * 16 * a.nf) + 32 * sizeof(int);
+    if (a.nf == 5) hipLaunchKernelGGL((synth_kernel<T, true, 5>), grid, dim3(64 * 5), lds, st, a);
+    else if (a.nf == 1) hipLaunchKernelGGL((synth_kernel<T, true, 1>), grid, dim3(64), lds, st, a);
+    else GD_CHECK(false, GD_ERR_ARG, "synth_vec: nf=%d (1 or 5)", a.nf);
     return gd_check_launch("synth_kernel<vec>");
 }
 
@@ -330,8 +407,10 @@ template <typename T> int gd_launch_synth_scal(hipStream_t st, const SynthArgs &
     GD_CHECK(a.nf >= 1 && a.nf <= 5, GD_ERR_ARG, "synth_scal: nf=%d", a.nf);
     const int LT = 8;
     dim3 grid((a.nlat2 + TM - 1) / TM, (a.nr + LT - 1) / LT, a.mmax + 1);
-    size_t lds = sizeof(T) * (KT * TM + KT * 16 * a.nf) + 32 * sizeof(int);
-    hipLaunchKernelGGL((synth_kernel<T, false>), grid, dim3(64 * a.nf), lds, st, a);
+    size_t lds = sizeof(T) * 2 * (KT * TM + KT * 16 * a.nf) + 32 * sizeof(int);
+    if (a.nf == 5) hipLaunchKernelGGL((synth_kernel<T, false, 5>), grid, dim3(64 * 5), lds, st, a);
+    else if (a.nf == 1) hipLaunchKernelGGL((synth_kernel<T, false, 1>), grid, dim3(64), lds, st, a);
+    else GD_CHECK(false, GD_ERR_ARG, "synth_scal: nf=%d (1 or 5)", a.nf);
     return gd_check_launch("synth_kernel<scal>");
 }
 
@@ -342,9 +421,11 @@ template <typename T> static int launch_ana(hipStream_t st, const AnaArgs &a, bo
     const int LT = 16 * a.nf / VC;
     dim3 grid((maxgroup + TM - 1) / TM, (a.nr + LT - 1) / LT, 2 * (a.mmax + 1));
     const int NT = vec ? 2 : 1;
-    size_t lds = sizeof(T) * (NT * KT * TM + NT * KT * 16 * a.nf);
-    if (vec) hipLaunchKernelGGL((ana_kernel<T, true>), grid, dim3(64 * a.nf), lds, st, a);
-    else hipLaunchKernelGGL((ana_kernel<T, false>), grid, dim3(64 * a.nf), lds, st, a);
+    size_t lds = sizeof(T) * 2 * (NT * KT * TM + NT * KT * 16 * a.nf);
+    if (vec && a.nf == 2) hipLaunchKernelGGL((ana_kernel<T, true, 2>), grid, dim3(128), lds, st, a);
+    else if (vec && a.nf == 1) hipLaunchKernelGGL((ana_kernel<T, true, 1>), grid, dim3(64), lds, st, a);
+    else if (!vec && a.nf == 1) hipLaunchKernelGGL((ana_kernel<T, false, 1>), grid, dim3(64), lds, st, a);
+    else GD_CHECK(false, GD_ERR_ARG, "ana: nf=%d", a.nf);
     return gd_check_launch(vec ? "ana_kernel<vec>" : "ana_kernel<scal>");
 }
 
@@ -526,15 +607,33 @@ template <typename T, int MODE> __global__ void __launch_bounds__(256) ring_kern
     if (MODE == RING_SYNTH || MODE == RING_STEP) {
         // Hermitian spectra of the north (X) and south (Y) rings packed as Z = X + iY
         const T *src = static_cast<const T *>(a.fb) + ((size_t)i * a.nr + r) * nm * a.nslot_f;
-        for (int e = tid; e < NF * nm; e += nthr) {
-            const int F = e % NF, m = e / NF;
-            const T *s = src + (size_t)m * a.nslot_f;
-            const T xr = s[sfn[F]], xi = s[sfn[F] + 1], yr = s[sfs[F]], yi = s[sfs[F] + 1];
-            if (m == 0) {
-                buf[F * n] = C2<T>{xr, yr};
-            } else {
-                buf[F * n + m] = C2<T>{xr - yi, xi + yr};
-                buf[F * n + n - m] = C2<T>{xr + yi, yr - xi};
+        // fixed trip count, fully unrolled: every thread's global loads are in
+        // flight together before the LDS stores consume them
+        constexpr int MAXI = 8;
+        T xr[MAXI], xi[MAXI], yr[MAXI], yi[MAXI];
+#pragma unroll
+        for (int it = 0; it < MAXI; ++it) {
+            const int e = tid + it * nthr;
+            if (e < NF * nm) {
+                const int F = e % NF, m = e / NF;
+                const T *s = src + (size_t)m * a.nslot_f;
+                xr[it] = s[sfn[F]];
+                xi[it] = s[sfn[F] + 1];
+                yr[it] = s[sfs[F]];
+                yi[it] = s[sfs[F] + 1];
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < MAXI; ++it) {
+            const int e = tid + it * nthr;
+            if (e < NF * nm) {
+                const int F = e % NF, m = e / NF;
+                if (m == 0) {
+                    buf[F * n] = C2<T>{xr[it], yr[it]};
+                } else {
+                    buf[F * n + m] = C2<T>{xr[it] - yi[it], xi[it] + yr[it]};
+                    buf[F * n + n - m] = C2<T>{xr[it] + yi[it], yr[it] - xi[it]};
+                }
             }
         }
         const int nz = n - 2 * nm + 1;
@@ -646,6 +745,8 @@ template <typename T> int gd_launch_ring(hipStream_t st, int mode, const RingArg
     const int nthr = 256;
     int nf = (mode == RING_STEP) ? 15 : a.nfield;
     GD_CHECK(nf >= 1 && nf <= 15, GD_ERR_ARG, "ring: nfield=%d", nf);
+    GD_CHECK((long)nf * (a.mmax + 1) <= 8L * nthr, GD_ERR_ARG, "ring: %d fields x %d orders exceed one load batch",
+             nf, a.mmax + 1);
     for (int s = 0; s < a.nstages; ++s) {
         int R = a.radix[s];
         long per = ((long)nf * (a.nlon / R) + nthr - 1) / nthr;
