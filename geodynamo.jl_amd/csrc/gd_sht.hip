@@ -523,10 +523,15 @@ template <int R, typename T> __device__ __forceinline__ void dftR(C2<T> *v, int 
 
 constexpr int MAXC = 32; // complex values a thread holds across one stage's barrier
 
-// One Stockham stage over nfft transforms of length n stored back to back in
-// buf; in place, with every butterfly read before any is written.
+// LDS index skew: one pad slot per 16 complex values, so the stride-R writes
+// of the early Stockham stages (and stride-Ns groups later) spread over banks.
+__device__ __forceinline__ int sk(int e) { return e + (e >> 4); }
+__host__ __device__ constexpr int ring_ld(int n) { return n + (n >> 4) + 1; } // per-transform LDS stride
+
+// One Stockham stage over nfft transforms of length n (stride ld in buf); in
+// place, with every butterfly read before any is written.  Runtime radix plan.
 template <int R, typename T>
-__device__ __noinline__ void fft_stage(C2<T> *buf, const C2<T> *tw, int nfft, int n, int Ns, int dir)
+__device__ __noinline__ void fft_stage(C2<T> *buf, const C2<T> *tw, int nfft, int n, int ld, int Ns, int dir)
 {
     constexpr int IT = MAXC / R;
     const int nb = n / R, total = nfft * nb;
@@ -540,7 +545,7 @@ __device__ __noinline__ void fft_stage(C2<T> *buf, const C2<T> *tw, int nfft, in
             const int f = g / nb, j = g - f * nb;
             const int k = j % Ns;
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[it][r] = buf[f * n + j + r * nb];
+            for (int r = 0; r < R; ++r) v[it][r] = buf[f * ld + sk(j + r * nb)];
 #pragma unroll
             for (int r = 1; r < R; ++r) {
                 C2<T> w = tw[k * r * twstep];
@@ -559,34 +564,109 @@ __device__ __noinline__ void fft_stage(C2<T> *buf, const C2<T> *tw, int nfft, in
             const int k = j % Ns;
             const int idxD = (j / Ns) * Ns * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[f * n + idxD + r * Ns] = v[it][r];
+            for (int r = 0; r < R; ++r) buf[f * ld + sk(idxD + r * Ns)] = v[it][r];
         }
     }
     __syncthreads();
 }
 
 template <typename T>
-__device__ void fft_batch(C2<T> *buf, const C2<T> *tw, int nfft, int n, const int *radix, int nstages, int dir)
+__device__ void fft_batch(C2<T> *buf, const C2<T> *tw, int nfft, int n, int ld, const int *radix, int nstages, int dir)
 {
     int Ns = 1;
     for (int s = 0; s < nstages; ++s) {
         const int R = radix[s];
         switch (R) {
-        case 8: fft_stage<8, T>(buf, tw, nfft, n, Ns, dir); break;
-        case 6: fft_stage<6, T>(buf, tw, nfft, n, Ns, dir); break;
-        case 4: fft_stage<4, T>(buf, tw, nfft, n, Ns, dir); break;
-        case 3: fft_stage<3, T>(buf, tw, nfft, n, Ns, dir); break;
-        default: fft_stage<2, T>(buf, tw, nfft, n, Ns, dir); break;
+        case 8: fft_stage<8, T>(buf, tw, nfft, n, ld, Ns, dir); break;
+        case 6: fft_stage<6, T>(buf, tw, nfft, n, ld, Ns, dir); break;
+        case 4: fft_stage<4, T>(buf, tw, nfft, n, ld, Ns, dir); break;
+        case 3: fft_stage<3, T>(buf, tw, nfft, n, ld, Ns, dir); break;
+        default: fft_stage<2, T>(buf, tw, nfft, n, ld, Ns, dir); break;
         }
         Ns *= R;
     }
 }
 
+// ---- compile-time plans: every divisor and twiddle stride is a constant
+constexpr int ct_next_radix(int n)
+{
+    return n % 8 == 0 ? 8 : n % 4 == 0 ? 4 : n % 6 == 0 ? 6 : n % 3 == 0 ? 3 : n % 2 == 0 ? 2 : 0;
+}
+constexpr int ct_nstages(int n)
+{
+    int s = 0;
+    while (n > 1) {
+        n /= ct_next_radix(n);
+        ++s;
+    }
+    return s;
+}
+constexpr int ct_radix(int n, int stage)
+{
+    for (int s = 0; s < stage; ++s) n /= ct_next_radix(n);
+    return ct_next_radix(n);
+}
+
+template <int R, int NS, int N, int DIR, typename T>
+__device__ __noinline__ void fft_stage_ct(C2<T> *buf, const C2<T> *tw, int nfft)
+{
+    constexpr int IT = MAXC / R;
+    constexpr int nb = N / R, twstep = N / (NS * R), LD = ring_ld(N);
+    const int total = nfft * nb;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    C2<T> v[IT][R];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int g = tid + it * nthr;
+        if (g < total) {
+            const int f = g / nb, j = g - f * nb;
+            const int k = j % NS;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[it][r] = buf[f * LD + sk(j + r * nb)];
+            if (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    C2<T> w = tw[k * r * twstep];
+                    if (DIR > 0) w.y = -w.y;
+                    v[it][r] = cmul(v[it][r], w);
+                }
+            }
+            dftR<R, T>(v[it], DIR);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int g = tid + it * nthr;
+        if (g < total) {
+            const int f = g / nb, j = g - f * nb;
+            const int k = j % NS;
+            const int idxD = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[f * LD + sk(idxD + r * NS)] = v[it][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int N, int DIR, int S = 0, int NS = 1>
+__device__ __forceinline__ void fft_ct(C2<T> *buf, const C2<T> *tw, int nfft)
+{
+    if constexpr (S < ct_nstages(N)) {
+        constexpr int R = ct_radix(N, S);
+        fft_stage_ct<R, NS, N, DIR, T>(buf, tw, nfft);
+        fft_ct<T, N, DIR, S + 1, NS * R>(buf, tw, nfft);
+    }
+}
+
 // One block per (northern node i, level r).
-template <typename T, int MODE> __global__ void __launch_bounds__(256) ring_kernel(RingArgs a)
+// NC > 0: the ring length is the compile-time constant NC (nlon); NC = 0 uses
+// the runtime plan.
+template <typename T, int MODE, int NC> __global__ void __launch_bounds__(256) ring_kernel(RingArgs a)
 {
     extern __shared__ char smem_raw[];
-    const int n = a.nlon, nm = a.mmax + 1;
+    const int n = NC > 0 ? NC : a.nlon, nm = a.mmax + 1;
+    const int ld = ring_ld(n);
     C2<T> *tw = reinterpret_cast<C2<T> *>(smem_raw);
     C2<T> *buf = tw + n;
     __shared__ int srad[12];
@@ -629,27 +709,28 @@ template <typename T, int MODE> __global__ void __launch_bounds__(256) ring_kern
             if (e < NF * nm) {
                 const int F = e % NF, m = e / NF;
                 if (m == 0) {
-                    buf[F * n] = C2<T>{xr[it], yr[it]};
+                    buf[F * ld] = C2<T>{xr[it], yr[it]};
                 } else {
-                    buf[F * n + m] = C2<T>{xr[it] - yi[it], xi[it] + yr[it]};
-                    buf[F * n + n - m] = C2<T>{xr[it] + yi[it], yr[it] - xi[it]};
+                    buf[F * ld + sk(m)] = C2<T>{xr[it] - yi[it], xi[it] + yr[it]};
+                    buf[F * ld + sk(n - m)] = C2<T>{xr[it] + yi[it], yr[it] - xi[it]};
                 }
             }
         }
         const int nz = n - 2 * nm + 1;
         for (int e = tid; e < NF * nz; e += nthr) {
             const int F = e / nz, k = nm + e % nz;
-            buf[F * n + k] = C2<T>{T(0), T(0)};
+            buf[F * ld + sk(k)] = C2<T>{T(0), T(0)};
         }
         __syncthreads();
-        fft_batch<T>(buf, tw, NF, n, srad, a.nstages, +1);
+        if constexpr (NC > 0) fft_ct<T, NC, +1>(buf, tw, NF);
+        else fft_batch<T>(buf, tw, NF, n, ld, srad, a.nstages, +1);
     }
 
     if (MODE == RING_SYNTH) {
         for (int j = tid; j < n; j += nthr)
             for (int F = 0; F < NF; ++F) {
                 double *o = F == 0 ? a.phys_out[0] : a.phys_out[1];
-                const C2<T> z = buf[F * n + j];
+                const C2<T> z = buf[F * ld + sk(j)];
                 o[((size_t)r * n + j) * a.nlat + i] = (double)z.x;
                 o[((size_t)r * n + j) * a.nlat + is] = (double)z.y;
             }
@@ -662,7 +743,7 @@ template <typename T, int MODE> __global__ void __launch_bounds__(256) ring_kern
         for (int j = tid; j < n; j += nthr)
             for (int F = 0; F < NF; ++F) {
                 const double *p = F == 0 ? a.phys_in[0] : a.phys_in[1];
-                buf[F * n + j] = C2<T>{(T)p[((size_t)r * n + j) * a.nlat + i], (T)p[((size_t)r * n + j) * a.nlat + is]};
+                buf[F * ld + sk(j)] = C2<T>{(T)p[((size_t)r * n + j) * a.nlat + i], (T)p[((size_t)r * n + j) * a.nlat + is]};
             }
         __syncthreads();
     } else {
@@ -675,7 +756,7 @@ template <typename T, int MODE> __global__ void __launch_bounds__(256) ring_kern
         for (int j = tid; j < n; j += nthr) {
             C2<T> f[15];
 #pragma unroll
-            for (int F = 0; F < 15; ++F) f[F] = buf[F * n + j];
+            for (int F = 0; F < 15; ++F) f[F] = buf[F * ld + sk(j)];
             C2<T> o[5];
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
@@ -709,25 +790,26 @@ template <typename T, int MODE> __global__ void __launch_bounds__(256) ring_kern
                 }
             }
 #pragma unroll
-            for (int P = 0; P < 5; ++P) buf[P * n + j] = o[P];
+            for (int P = 0; P < 5; ++P) buf[P * ld + sk(j)] = o[P];
         }
         __syncthreads();
     }
 
-    fft_batch<T>(buf, tw, NP, n, srad, a.nstages, -1);
+    if constexpr (NC > 0) fft_ct<T, NC, -1>(buf, tw, NP);
+    else fft_batch<T>(buf, tw, NP, n, ld, srad, a.nstages, -1);
 
     // split Z into north X and south Y, form G+- = (X +- Y) * 2 pi w_i / n
     const T c = static_cast<const T *>(a.wq)[i];
     T *gb = static_cast<T *>(a.gb);
     for (int e = tid; e < NP * nm; e += nthr) {
         const int P = e % NP, m = e / NP;
-        const C2<T> Z = buf[P * n + m];
+        const C2<T> Z = buf[P * ld + sk(m)];
         C2<T> X, Y;
         if (m == 0) {
             X = C2<T>{Z.x, T(0)};
             Y = C2<T>{Z.y, T(0)};
         } else {
-            const C2<T> Zc = buf[P * n + n - m];
+            const C2<T> Zc = buf[P * ld + sk(n - m)];
             X = C2<T>{T(0.5) * (Z.x + Zc.x), T(0.5) * (Z.y - Zc.y)};
             Y = C2<T>{T(0.5) * (Z.y + Zc.y), T(0.5) * (Zc.x - Z.x)};
         }
@@ -739,6 +821,24 @@ template <typename T, int MODE> __global__ void __launch_bounds__(256) ring_kern
     }
 }
 } // namespace
+
+template <typename T, int MODE, int NC> static void ring_launch(hipStream_t st, dim3 grid, size_t lds, const RingArgs &a)
+{
+    hipLaunchKernelGGL((ring_kernel<T, MODE, NC>), grid, dim3(256), lds, st, a);
+}
+
+template <typename T, int MODE> static void ring_dispatch(hipStream_t st, dim3 grid, size_t lds, const RingArgs &a)
+{
+    switch (a.nlon) {
+    case 48: ring_launch<T, MODE, 48>(st, grid, lds, a); break;
+    case 96: ring_launch<T, MODE, 96>(st, grid, lds, a); break;
+    case 144: ring_launch<T, MODE, 144>(st, grid, lds, a); break;
+    case 192: ring_launch<T, MODE, 192>(st, grid, lds, a); break;
+    case 288: ring_launch<T, MODE, 288>(st, grid, lds, a); break;
+    case 384: ring_launch<T, MODE, 384>(st, grid, lds, a); break;
+    default: ring_launch<T, MODE, 0>(st, grid, lds, a); break;
+    }
+}
 
 template <typename T> int gd_launch_ring(hipStream_t st, int mode, const RingArgs &a)
 {
@@ -752,13 +852,13 @@ template <typename T> int gd_launch_ring(hipStream_t st, int mode, const RingArg
         long per = ((long)nf * (a.nlon / R) + nthr - 1) / nthr;
         GD_CHECK(per * R <= MAXC, GD_ERR_ARG, "ring: nlon=%d too large for %d fields", a.nlon, nf);
     }
-    size_t lds = sizeof(C2<T>) * (size_t)a.nlon * (nf + 1);
+    size_t lds = sizeof(C2<T>) * ((size_t)ring_ld(a.nlon) * nf + a.nlon);
     GD_CHECK(lds <= 160 * 1024, GD_ERR_ARG, "ring: LDS %zu too large", lds);
     dim3 grid(a.nlat2, a.nr);
     switch (mode) {
-    case RING_SYNTH: hipLaunchKernelGGL((ring_kernel<T, RING_SYNTH>), grid, dim3(nthr), lds, st, a); break;
-    case RING_ANALYSIS: hipLaunchKernelGGL((ring_kernel<T, RING_ANALYSIS>), grid, dim3(nthr), lds, st, a); break;
-    default: hipLaunchKernelGGL((ring_kernel<T, RING_STEP>), grid, dim3(nthr), lds, st, a); break;
+    case RING_SYNTH: ring_dispatch<T, RING_SYNTH>(st, grid, lds, a); break;
+    case RING_ANALYSIS: ring_dispatch<T, RING_ANALYSIS>(st, grid, lds, a); break;
+    default: ring_dispatch<T, RING_STEP>(st, grid, lds, a); break;
     }
     return gd_check_launch("ring_kernel");
 }
