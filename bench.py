@@ -36,13 +36,26 @@ SEED, AMP = 2024, 1e-3
 PEAK = {"hbm": 8000.0, "fp32": 157.3, "fp64": 78.6}  # GB/s, TFLOP/s (MI355X dense)
 
 
-def kernel_costs(prec_bytes):
+def kernel_costs(prec_bytes, tangential=True):
     """Algorithmic bytes and flops per launch of each kernel class of one step.
 
     Legendre flops count 2*nlat*nlm per scalar-equivalent contraction per level
-    (the even/odd-symmetric count): synthesis 18 vector + 5 scalar, analysis
-    8 vector + 1 scalar.  Bytes are the compulsory reads and writes of each
-    kernel's HBM operands (tables excluded: they stay in L2/MALL)."""
+    (the even/odd-symmetric count); a vector synthesis or analysis is 4 such
+    contractions (S and T against dP/dtheta and m P/sin).  Bytes are the
+    compulsory reads and writes of each kernel's HBM operands (tables
+    excluded: they stay in L2/MALL).
+
+    tangential (the default step: radial components zero, so the tangential
+    parts of u x omega, j x B, u x B and u_r dT/dr are identically zero):
+      synthesis: u (4) and grad T from T/r (2); ring: 4 fields in, 3 products
+      out; analysis: the force (4) and -u.grad T (1)
+    full products: synthesis of u, omega, B, j (16) and grad T (2) plus dT/dr
+      (1); ring: 15 fields in (60 slots per record, 16 of them zero), 5
+      products out; analysis: force and u x B (8) and -u.grad T (1)
+    The CNAB2 kernel reads u, nl, nl_prev and writes the new state (64 B per
+    complex coefficient and field) and also writes the next step's synthesis
+    columns (6 or 20 per level), so the separate spectral preparation does
+    not run in the steady state."""
     L, nlat, nr = CFG["lmax"], CFG["nlat"], CFG["nr"]
     nlm = (L + 1) * (L + 2) // 2
     nm, nlat2 = L + 1, nlat // 2
@@ -50,15 +63,66 @@ def kernel_costs(prec_bytes):
     fs = nlm * nr
     four = nlat2 * nr * nm  # Fourier (i, level, m) points
     contraction = 2.0 * nlat * nlm * nr
+    if tangential:
+        return {
+            "spectral_pre": dict(bytes=5 * 16 * fs + 20 * es * fs, flops=0.0),
+            "synth_vector": dict(bytes=6 * es * fs + 16 * es * four, flops=6 * contraction),
+            "synth_scalar": dict(bytes=0.0, flops=0.0),
+            "ring_fft_products": dict(bytes=16 * es * four + 12 * es * four, flops=0.0),
+            "analysis_vector": dict(bytes=8 * es * four + 2 * 16 * fs, flops=4 * contraction),
+            "analysis_scalar": dict(bytes=4 * es * four + 16 * fs, flops=1 * contraction),
+            "cnab2_solve": dict(bytes=5 * 64 * fs + 6 * es * fs, flops=0.0),
+        }
     return {
-        "spectral_pre": dict(bytes=5 * 16 * fs + 28 * es * fs, flops=0.0),
+        "spectral_pre": dict(bytes=5 * 16 * fs + 20 * es * fs, flops=0.0),
         "synth_vector": dict(bytes=18 * es * fs + 40 * es * four, flops=18 * contraction),
-        "synth_scalar": dict(bytes=10 * es * fs + 20 * es * four, flops=5 * contraction),
+        "synth_scalar": dict(bytes=2 * es * fs + 4 * es * four, flops=1 * contraction),
         "ring_fft_products": dict(bytes=60 * es * four + 20 * es * four, flops=0.0),
         "analysis_vector": dict(bytes=16 * es * four + 4 * 16 * fs, flops=8 * contraction),
         "analysis_scalar": dict(bytes=4 * es * four + 16 * fs, flops=1 * contraction),
-        "cnab2_solve": dict(bytes=5 * 64 * fs, flops=0.0),
+        "cnab2_solve": dict(bytes=5 * 64 * fs + 20 * es * fs, flops=0.0),
     }
+
+
+# kernel of each class on the headline (tangential) path, as rocprofv3 names it
+TANG_KERNELS = {"spectral_pre": "spectral_pre_kernel<{T}>", "synth_vector": "synth_kernel<{T}, true, 2>",
+                "ring_fft_products": "ring_kernel<{T}, 3, 384>", "analysis_vector": "ana_kernel<{T}, true, 1>",
+                "analysis_scalar": "ana_kernel<{T}, false, 1>", "cnab2_solve": "cnab2_chain_kernel<3, 8, {T}>"}
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "pmc_step_lmax127_nr65.json")
+COPY_PROBE_BYTES = 268435456
+
+
+def derive_traffic(prof_dir):
+    """HBM bytes per launch of each kernel class from two rocprofv3 --pmc runs
+    of `bench.py --no-full --copy-probe 268435456` (FETCH_SIZE in
+    prof_dir/fetch, WRITE_SIZE in prof_dir/write; KiB per dispatch), each
+    scaled by what the counter reads for copy_kernel moving a known number of
+    bytes.  Writes TRAFFIC_JSON."""
+    import collections
+    import csv
+
+    val = {}
+    for cnt in ("fetch", "write"):
+        agg = collections.defaultdict(list)
+        with open(os.path.join(prof_dir, cnt, "run_counter_collection.csv")) as f:
+            for r in csv.DictReader(f):
+                agg[r["Kernel_Name"]].append(float(r["Counter_Value"]) * 1024)
+        val[cnt] = {k: sum(v) / len(v) for k, v in agg.items()}
+    copy = [k for k in val["fetch"] if k.startswith("copy_kernel")][0]
+    fscale = COPY_PROBE_BYTES / val["fetch"][copy]
+    wscale = COPY_PROBE_BYTES / val["write"][copy]
+    out = {"fetch_scale": round(fscale, 4), "write_scale": round(wscale, 4), "fp32": {}, "fp64": {}}
+    for prec, T in (("fp32", "float"), ("fp64", "double")):
+        for cls, pat in TANG_KERNELS.items():
+            ks = [k for k in val["fetch"] if pat.format(T=T) in k]
+            if not ks:
+                continue
+            rd, wr = val["fetch"][ks[0]] * fscale, val["write"].get(ks[0], 0.0) * wscale
+            out[prec][cls] = {"kernel": ks[0], "read_bytes": round(rd), "write_bytes": round(wr),
+                              "hbm_bytes_per_launch": round(rd + wr)}
+    with open(TRAFFIC_JSON, "w") as f:
+        json.dump(out, f, indent=1)
+    return out
 
 
 def roofline(prof_ms, prec):
@@ -73,7 +137,7 @@ def roofline(prof_ms, prec):
     # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     # runs (scaled by a copy kernel of known size), when they cover this kernel
     traffic = None
-    pmc = os.path.join(ROOT, "profiles", "pmc_step_lmax127_nr65.json")
+    pmc = TRAFFIC_JSON
     if os.path.exists(pmc):
         with open(pmc) as f:
             rec = json.load(f).get(prec, {}).get(name)
@@ -154,7 +218,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run one plain device copy of this many bytes (HBM counter calibration)")
+    ap.add_argument("--no-full", action="store_true",
+                    help="skip the full-products comparison (profiling runs of the headline path)")
+    ap.add_argument("--derive-traffic", metavar="DIR",
+                    help="only turn rocprofv3 FETCH_SIZE/WRITE_SIZE runs under DIR into the traffic JSON")
     args = ap.parse_args()
+    if args.derive_traffic:
+        print(json.dumps(derive_traffic(args.derive_traffic), indent=1))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs the CNAB2 step on a single MI355X "
               "(the multi-GPU sharding is a design note only); rerun with --gpus 1", file=sys.stderr)
@@ -169,6 +240,9 @@ def main():
     lmax, nlat, nlon, nr = kw.pop("lmax"), kw.pop("nlat"), kw.pop("nlon"), kw.pop("nr")
     mixed = gdhip.CNAB2Step(lmax, nlat, nlon, nr, precision="fp32", **kw)
     fp64 = gdhip.CNAB2Step(lmax, nlat, nlon, nr, precision="fp64", **kw)
+    if not args.no_full:
+        full_mixed = gdhip.CNAB2Step(lmax, nlat, nlon, nr, precision="fp32", full_products=True, **kw)
+        full_fp64 = gdhip.CNAB2Step(lmax, nlat, nlon, nr, precision="fp64", full_products=True, **kw)
 
     # accuracy: mixed vs FP64 GPU after err_steps; FP64 GPU after one step (oracle check below)
     fp64.set_state_complex(st)
@@ -179,8 +253,13 @@ def main():
     e_ref = fp64.energies()
     mixed.set_state_complex(st)
     mixed.run(args.err_steps, graph=True)
-    err_mixed = rel_err(mixed.get_state_complex(), ref)
+    st_mixed = mixed.get_state_complex()
+    err_mixed = rel_err(st_mixed, ref)
     e_mix = mixed.energies()
+    if not args.no_full:
+        full_mixed.set_state_complex(st)
+        full_mixed.run(args.err_steps, graph=True)
+        err_full = rel_err(full_mixed.get_state_complex(), st_mixed)
 
     if args.copy_probe:
         copy_probe(args.copy_probe)
@@ -188,6 +267,11 @@ def main():
     tmx = timed(mixed, st, args.steps, args.warmup)
     prof = mixed.profile(5)
     prof64 = fp64.profile(5)
+    if not args.no_full:
+        # the same step with every product formed (no pruning of the exact zeros)
+        tfm = timed(full_mixed, st, args.steps, args.warmup)
+        tf64 = timed(full_fp64, st, args.steps, args.warmup)
+        prof_full = full_mixed.profile(5)
     kname, roof = roofline(prof, "fp32")
 
     out = {
@@ -217,6 +301,12 @@ def main():
         "kernel_ms_fp64": {k: round(v, 5) for k, v in prof64.items()},
         "roofline": roof,
     }
+    if not args.no_full:
+        out["full_products"] = {"mixed_steps_per_s": round(args.steps / tfm, 2),
+                                "vs_default_mixed_max_rel_per_field_after_%d_steps" % args.err_steps:
+                                [float("%.3e" % e) for e in err_full],
+                                "fp64_steps_per_s": round(args.steps / tf64, 2),
+                                "kernel_ms_mixed": {k: round(v, 5) for k, v in prof_full.items()}}
     if not args.no_cpu_baseline:
         cb, err64 = cpu_baseline(st, args.cpu_steps, one_step64)
         out["cpu_baseline"] = cb

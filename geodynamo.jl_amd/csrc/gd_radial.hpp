@@ -17,6 +17,25 @@ int gd_launch_cnab2_step(hipStream_t st, const double *nuLap, const double *rinv
                          const double *LUt, const int *l_of_idx, int nl, int n, int bw, int nlm, double dt,
                          double theta, int bc_mask, const double *st_in, double *st_out, const double *nl_arr,
                          const double *np_arr);
+// Arguments of the column-register CNAB2 kernel (gd_radial.hip).  synV/synS
+// (element type float or double): when set, the next step's synthesis columns
+// are written from the new state.
+struct ChainArgs {
+    const double *nuLap, *rinv2, *nus, *LUt;
+    const int *l_of_idx;
+    int nl, n, bw, nlm;
+    double dt, theta;
+    int bc_mask;
+    const double *st_in;
+    double *st_out;
+    const double *nl_arr, *np_arr;
+    const double *D1, *D2, *rtab;
+    int radials;
+    int syn_tang; // only the u and T/r synthesis columns are needed (the tangential step)
+    void *synV, *synS;
+};
+int gd_cnab2_chain_supported(int n, int bw);
+int gd_launch_cnab2_chain(hipStream_t st, const ChainArgs &c, bool syn_fp64);
 int gd_launch_energies(hipStream_t st, const double *state, const int *l_of_idx, int nlm, int n, double *part,
                        int nblk, double *out3);
 int gd_launch_sqdiff(hipStream_t st, long n, const double *ar, const double *ai, const double *br, const double *bi,

@@ -87,24 +87,42 @@ template <typename T> using acc_t = typename Acc4<T>::type;
 
 template <typename T> __device__ __forceinline__ T shfl_xor3(T v) { return __shfl_xor(v, 3); }
 
+// 16-byte vector of T (stores of the staged output tiles)
+template <typename T> struct Vec16;
+template <> struct Vec16<float> { using type = float4; };
+template <> struct Vec16<double> { using type = double2; };
+
+// Tile shape of a synthesis block: NF fields, VC virtual columns per level
+// (vector: S_re S_im T_re T_im per field, scalar: re im), NTILE = 16 NF
+// columns = LT levels; RS output slots per (node, level) record.
+template <bool VEC, int NF> struct SynthTile {
+    static constexpr int VC = VEC ? 4 * NF : 2 * NF;
+    static constexpr int NTILE = 16 * NF;
+    static constexpr int LT = NTILE / VC;
+    static constexpr int RS = VEC ? 8 * NF : 4 * NF;
+    static constexpr int NT = VEC ? 2 : 1; // tables: (dP, mP/sin) or P
+};
+
 // Synthesis contraction for one (m, node tile, level tile), both parity
 // groups: the K chunks of the even group, then of the odd group, run as one
 // software-pipelined sequence -- chunk c+1 is loaded from global memory into
 // registers while the MFMAs consume chunk c from LDS (double-buffered LDS, one
 // barrier per chunk).
-template <typename T, int NT, int NF>
+template <typename T, bool VEC, int NF>
 __device__ __forceinline__ void synth_pipeline(const SynthArgs &a, const T *__restrict__ in, const T *__restrict__ tab0,
                                                const T *__restrict__ tab1, T *sA, T *sB, const int *scol, int m,
-                                               int i0, int r0, int VC, int NTILE, acc_t<T> (&accE)[NT][2],
-                                               acc_t<T> (&accO)[NT][2])
+                                               int i0, int r0, acc_t<T> (&accE)[SynthTile<VEC, NF>::NT][2],
+                                               acc_t<T> (&accO)[SynthTile<VEC, NF>::NT][2])
 {
+    using TL = SynthTile<VEC, NF>;
+    constexpr int NT = TL::NT, VC = TL::VC, NTILE = TL::NTILE;
     constexpr int RA = (NT * KT * TM + 64 * NF - 1) / (64 * NF); // A elements per thread (64*NF threads)
-    constexpr int RB = KT * 16 / 64;                             // B elements per thread (NTILE = 16*NF)
+    constexpr int RB = KT * 16 / 64;                             // B elements per thread
+    constexpr int sAsz = NT * KT * TM, sBsz = KT * NTILE;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int K = a.moff[m + 1] - a.moff[m], ne = a.ne[m];
     const int nE = (ne + KT - 1) / KT, nC = nE + (K - ne + KT - 1) / KT;
-    const int sAsz = NT * KT * TM, sBsz = KT * NTILE;
     T ra[RA], rb[RB];
     auto gload = [&](int c) {
         const int p = c < nE ? 0 : 1;
@@ -170,16 +188,24 @@ __device__ __forceinline__ void synth_pipeline(const SynthArgs &a, const T *__re
     }
 }
 
+template <typename T, bool VEC, int NF> constexpr size_t synth_lds_bytes()
+{
+    using TL = SynthTile<VEC, NF>;
+    const size_t pipe = 2 * (TL::NT * KT * TM + KT * TL::NTILE), tile = (size_t)TM * TL::LT * TL::RS;
+    return 128 + sizeof(T) * (pipe > tile ? pipe : tile);
+}
+
+// The block's output tile -- TM nodes x LT levels, RS contiguous slots of each
+// (node, level, m) record -- is staged in LDS and stored as 16-byte vectors,
+// one record run after another.
 template <typename T, bool VEC, int NF> __global__ void __launch_bounds__(64 * NF) synth_kernel(SynthArgs a)
 {
-    constexpr int NT = VEC ? 2 : 1;
-    extern __shared__ char smem_raw[];
-    const int VC = VEC ? 4 * a.nf : 2 * a.nf;
-    const int NTILE = 16 * a.nf;
-    const int LT = NTILE / VC;
-    T *sA = reinterpret_cast<T *>(smem_raw);
+    using TL = SynthTile<VEC, NF>;
+    constexpr int NT = TL::NT, VC = TL::VC, LT = TL::LT, RS = TL::RS;
+    extern __shared__ __align__(16) char smem_raw[];
+    int *scol = reinterpret_cast<int *>(smem_raw);
+    T *sA = reinterpret_cast<T *>(smem_raw + 128);
     T *sB = sA + 2 * NT * KT * TM;
-    int *scol = reinterpret_cast<int *>(sB + 2 * KT * NTILE);
     const int m = blockIdx.z, i0 = blockIdx.x * TM, r0 = blockIdx.y * LT;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     if (tid < 20) scol[tid] = a.colmap[tid];
@@ -195,20 +221,19 @@ template <typename T, bool VEC, int NF> __global__ void __launch_bounds__(64 * N
             accE[t][rb] = acc_t<T>{0, 0, 0, 0};
             accO[t][rb] = acc_t<T>{0, 0, 0, 0};
         }
-    synth_pipeline<T, NT, NF>(a, in, tab0, tab1, sA, sB, scol, m, i0, r0, VC, NTILE, accE, accO);
+    synth_pipeline<T, VEC, NF>(a, in, tab0, tab1, sA, sB, scol, m, i0, r0, accE, accO);
+    __syncthreads(); // the tile below reuses the pipeline's LDS
 
-    T *out = static_cast<T *>(a.out);
+    T *tile = sA; // [node][level][slot]
     const int vcol = wave * 16 + (lane & 15);
     const int lv = vcol / VC, fv = vcol - lv * VC;
-    const int r = r0 + lv;
-    const int nm = a.mmax + 1;
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int i = i0 + rb * 16 + acc_row<T>(lane, v);
+            const int il = rb * 16 + acc_row<T>(lane, v);
             T on, os;
-            int slot;
+            int srel;
             if constexpr (VEC) {
                 // north / south parity combination: dP flips sign once more than P, Q
                 const T dPn = accE[0][rb][v] + accO[0][rb][v], dPs = accO[0][rb][v] - accE[0][rb][v];
@@ -228,17 +253,31 @@ template <typename T, bool VEC, int NF> __global__ void __launch_bounds__(64 * N
                     on = Qxn - dPn;
                     os = Qxs - dPs;
                 }
-                slot = a.slot0 + 8 * (fv >> 2) + q; // + 4 for south
+                srel = 8 * (fv >> 2) + q; // + 4 for south
             } else {
                 on = accE[0][rb][v] + accO[0][rb][v];
                 os = accE[0][rb][v] - accO[0][rb][v];
-                slot = a.slot0 + 4 * (fv >> 1) + (fv & 1); // + 2 for south
+                srel = 4 * (fv >> 1) + (fv & 1); // + 2 for south
             }
-            if (i < a.nlat2 && r < a.nr) {
-                T *dst = out + (((size_t)i * a.nr + r) * nm + m) * a.nslot;
-                dst[slot] = on;
-                dst[slot + (VEC ? 4 : 2)] = os;
-            }
+            T *t = tile + (il * LT + lv) * RS + srel;
+            t[0] = on;
+            t[VEC ? 4 : 2] = os;
+        }
+    }
+    __syncthreads();
+    using V = typename Vec16<T>::type;
+    constexpr int EV = 16 / sizeof(T);
+    constexpr int NV = TM * LT * RS / EV;
+    T *out = static_cast<T *>(a.out);
+    const int nm = a.mmax + 1;
+    for (int t = tid; t < NV; t += 64 * NF) {
+        const int e = t * EV;
+        const int rec = e / RS, so = e - rec * RS;
+        const int il = rec / LT, l2 = rec - il * LT;
+        const int i = i0 + il, r = r0 + l2;
+        if (i < a.nlat2 && r < a.nr) {
+            T *dst = out + (((size_t)i * a.nr + r) * nm + m) * a.nslot + a.slot0 + so;
+            *reinterpret_cast<V *>(dst) = *reinterpret_cast<const V *>(tile + e);
         }
     }
 }
@@ -247,10 +286,11 @@ template <typename T, bool VEC, int NF> __global__ void __launch_bounds__(64 * N
 template <typename T, bool VEC, int NF> __global__ void __launch_bounds__(64 * NF) ana_kernel(AnaArgs a)
 {
     constexpr int NT = VEC ? 2 : 1;
-    extern __shared__ char smem_raw[];
-    const int VC = VEC ? 4 * a.nf : 2 * a.nf;
-    const int NTILE = 16 * a.nf;
-    const int LT = NTILE / VC;
+    extern __shared__ __align__(16) char smem_raw[];
+    constexpr int VC = VEC ? 4 * NF : 2 * NF;
+    constexpr int NTILE = 16 * NF;
+    constexpr int LT = NTILE / VC;
+    static_assert(NTILE * (TM + 1) <= 2 * NT * KT * (TM + NTILE), "analysis tile exceeds the pipeline LDS");
     T *sA = reinterpret_cast<T *>(smem_raw);
     T *sB = sA + 2 * NT * KT * TM; // A and B are double-buffered
     const int m = blockIdx.z >> 1, p = blockIdx.z & 1;
@@ -271,7 +311,7 @@ template <typename T, bool VEC, int NF> __global__ void __launch_bounds__(64 * N
     // software-pipelined K loop over the northern nodes (see synth_pipeline)
     constexpr int RA = (NT * KT * TM + 64 * NF - 1) / (64 * NF);
     constexpr int RB = NT * KT * 16 / 64;
-    const int sAsz = NT * KT * TM, sBsz = NT * KT * NTILE;
+    constexpr int sAsz = NT * KT * TM, sBsz = NT * KT * NTILE;
     const int nC = (a.nlat2 + KT - 1) / KT;
     T ra[RA], rb[RB];
     auto gload = [&](int c) {
@@ -348,68 +388,95 @@ template <typename T, bool VEC, int NF> __global__ void __launch_bounds__(64 * N
         }
     }
 
+    // stage the (row x column) tile in LDS, padded against bank conflicts,
+    // then store each output column's rows (consecutive idx) contiguously
+    __syncthreads(); // the tile reuses the pipeline's LDS
+    constexpr int TMP = TM + 1;
+    T *tile = sA; // [column][row]
     const int vcol = wave * 16 + (lane & 15);
-    const int lv = vcol / VC, fv = vcol - lv * VC, r = r0 + lv;
-    const size_t fs = (size_t)a.nr * a.nlm;
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int row = row0 + rb * 16 + acc_row<T>(lane, v);
-            double val;
-            int outk, reim;
+            const int row = rb * 16 + acc_row<T>(lane, v);
+            T o;
             if constexpr (VEC) {
                 const T X = acc[0][rb][v], Y = acc[1][rb][v];
                 const T Yx = shfl_xor3(Y);
                 const int q = lane & 3;
-                T o;
                 if (q == 0) o = X + Yx;       // S_re
                 else if (q == 1) o = X - Yx;  // S_im
                 else if (q == 2) o = Yx - X;  // T_re
                 else o = -Yx - X;             // T_im
-                val = (double)o;
-                outk = 2 * (fv >> 2) + (q >> 1);
-                reim = q & 1;
             } else {
-                val = (double)acc[0][rb][v];
+                o = acc[0][rb][v];
+            }
+            tile[vcol * TMP + row] = o;
+        }
+    }
+    __syncthreads();
+    const size_t fs = (size_t)a.nr * a.nlm;
+    for (int t = tid; t < NTILE * TM; t += nthr) {
+        const int col = t / TM, row = t - col * TM;
+        const int lv = col / VC, fv = col - lv * VC, r = r0 + lv;
+        const int grow = row0 + row;
+        if (grow < Kp && r < a.nr) {
+            int outk, reim;
+            if constexpr (VEC) {
+                outk = 2 * (fv >> 2) + ((fv & 3) >> 1);
+                reim = fv & 1;
+            } else {
                 outk = fv >> 1;
                 reim = fv & 1;
             }
-            if (row < Kp && r < a.nr) {
-                const int idx = kbase + row;
-                const int l = a.l_of_idx[idx];
-                const int sc = outk == 0 ? a.scale[0] : outk == 1 ? a.scale[1] : outk == 2 ? a.scale[2] : a.scale[3];
-                double *op = outk == 0 ? a.out[0] : outk == 1 ? a.out[1] : outk == 2 ? a.out[2] : a.out[3];
-                if (sc == 0) val = (l > 0) ? val / ((double)l * (double)(l + 1)) : 0.0;
-                else if (sc == 2) val = -val;
-                if (m == 0 && reim) val = 0.0; // imag(m=0) := 0 (shtnskit_field_functions.jl:432-435)
-                op[(size_t)reim * fs + (size_t)r * a.nlm + idx] = val;
-            }
+            double val = (double)tile[col * TMP + row];
+            const int idx = kbase + grow;
+            const int l = a.l_of_idx[idx];
+            const int sc = outk == 0 ? a.scale[0] : outk == 1 ? a.scale[1] : outk == 2 ? a.scale[2] : a.scale[3];
+            double *op = outk == 0 ? a.out[0] : outk == 1 ? a.out[1] : outk == 2 ? a.out[2] : a.out[3];
+            if (sc == 0) val = (l > 0) ? val / ((double)l * (double)(l + 1)) : 0.0;
+            else if (sc == 2) val = -val;
+            if (m == 0 && reim) val = 0.0; // imag(m=0) := 0 (shtnskit_field_functions.jl:432-435)
+            op[(size_t)reim * fs + (size_t)r * a.nlm + idx] = val;
         }
     }
 }
 } // namespace
 
+template <typename T, bool VEC, int NF> static void launch_synth(hipStream_t st, const SynthArgs &a)
+{
+    using TL = SynthTile<VEC, NF>;
+    dim3 grid((a.nlat2 + TM - 1) / TM, (a.nr + TL::LT - 1) / TL::LT, a.mmax + 1);
+    constexpr size_t lds = synth_lds_bytes<T, VEC, NF>();
+    hipLaunchKernelGGL((synth_kernel<T, VEC, NF>), grid, dim3(64 * NF), lds, st, a);
+}
+
+// The staged tile is stored in 16-byte vectors: record stride and first slot
+// must be multiples of 4 elements.
+static int synth_check(const SynthArgs &a, const char *what)
+{
+    GD_CHECK(a.nslot % 4 == 0 && a.slot0 % 4 == 0, GD_ERR_ARG, "%s: nslot=%d slot0=%d not multiples of 4", what,
+             a.nslot, a.slot0);
+    return GD_OK;
+}
+
 template <typename T> int gd_launch_synth_vec(hipStream_t st, const SynthArgs &a)
 {
-    GD_CHECK(a.nf >= 1 && a.nf <= 5, GD_ERR_ARG, "synth_vec: nf=%d", a.nf);
-    const int LT = 4;
-    dim3 grid((a.nlat2 + TM - 1) / TM, (a.nr + LT - 1) / LT, a.mmax + 1);
-    size_t lds = sizeof(T) * 2 * (2 * KT * TM + KT * 16 * a.nf) + 32 * sizeof(int);
-    if (a.nf == 5) hipLaunchKernelGGL((synth_kernel<T, true, 5>), grid, dim3(64 * 5), lds, st, a);
-    else if (a.nf == 1) hipLaunchKernelGGL((synth_kernel<T, true, 1>), grid, dim3(64), lds, st, a);
-    else GD_CHECK(false, GD_ERR_ARG, "synth_vec: nf=%d (1 or 5)", a.nf);
+    GD_TRY(synth_check(a, "synth_vec"));
+    GD_CHECK(a.slot0 + 8 * a.nf <= a.nslot, GD_ERR_ARG, "synth_vec: slots %d+%d exceed %d", a.slot0, 8 * a.nf, a.nslot);
+    if (a.nf == 5) launch_synth<T, true, 5>(st, a);
+    else if (a.nf == 2) launch_synth<T, true, 2>(st, a);
+    else if (a.nf == 1) launch_synth<T, true, 1>(st, a);
+    else GD_CHECK(false, GD_ERR_ARG, "synth_vec: nf=%d (1, 2 or 5)", a.nf);
     return gd_check_launch("synth_kernel<vec>");
 }
 
 template <typename T> int gd_launch_synth_scal(hipStream_t st, const SynthArgs &a)
 {
-    GD_CHECK(a.nf >= 1 && a.nf <= 5, GD_ERR_ARG, "synth_scal: nf=%d", a.nf);
-    const int LT = 8;
-    dim3 grid((a.nlat2 + TM - 1) / TM, (a.nr + LT - 1) / LT, a.mmax + 1);
-    size_t lds = sizeof(T) * 2 * (KT * TM + KT * 16 * a.nf) + 32 * sizeof(int);
-    if (a.nf == 5) hipLaunchKernelGGL((synth_kernel<T, false, 5>), grid, dim3(64 * 5), lds, st, a);
-    else if (a.nf == 1) hipLaunchKernelGGL((synth_kernel<T, false, 1>), grid, dim3(64), lds, st, a);
+    GD_TRY(synth_check(a, "synth_scal"));
+    GD_CHECK(a.slot0 + 4 * a.nf <= a.nslot, GD_ERR_ARG, "synth_scal: slots %d+%d exceed %d", a.slot0, 4 * a.nf, a.nslot);
+    if (a.nf == 5) launch_synth<T, false, 5>(st, a);
+    else if (a.nf == 1) launch_synth<T, false, 1>(st, a);
     else GD_CHECK(false, GD_ERR_ARG, "synth_scal: nf=%d (1 or 5)", a.nf);
     return gd_check_launch("synth_kernel<scal>");
 }
@@ -684,7 +751,7 @@ template <typename T, int MODE, int NC> __global__ void __launch_bounds__(256) r
     __syncthreads();
     const int NF = a.nfield;
 
-    if (MODE == RING_SYNTH || MODE == RING_STEP) {
+    if (MODE == RING_SYNTH || MODE == RING_STEP || MODE == RING_STEP_TANG) {
         // Hermitian spectra of the north (X) and south (Y) rings packed as Z = X + iY
         const T *src = static_cast<const T *>(a.fb) + ((size_t)i * a.nr + r) * nm * a.nslot_f;
         // fixed trip count, fully unrolled: every thread's global loads are in
@@ -738,7 +805,45 @@ template <typename T, int MODE, int NC> __global__ void __launch_bounds__(256) r
     }
 
     int NP;
-    if (MODE == RING_ANALYSIS) {
+    if constexpr (MODE == RING_STEP_TANG) {
+        // the products of RING_STEP with u_r = omega_r = B_r = j_r = 0: the
+        // tangential parts of u x omega, j x B and u x B vanish and u_r dT/dr
+        // drops out; the Coriolis term and the tangential advection of T stay,
+        // evaluated in the same order as there
+        // fields: 0 u_t 1 u_p 2 gT_t 3 gT_p
+        NP = 3;
+        const T ro = (T)a.ro;
+        const T *costh = static_cast<const T *>(a.costh);
+        for (int j = tid; j < n; j += nthr) {
+            C2<T> f[4];
+#pragma unroll
+            for (int F = 0; F < 4; ++F) f[F] = buf[F * ld + sk(j)];
+            C2<T> o[3];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int ti = a.coriolis_index ? (h ? is : i) : (j < a.nlat - 1 ? j : a.nlat - 1);
+                const T cth = costh[ti];
+#define FV(F) (h ? f[F].y : f[F].x)
+                const T ut = FV(0), up = FV(1), gt = FV(2), gp = FV(3);
+#undef FV
+                const T at = -ro * (-cth * up);
+                const T ap = -ro * (cth * ut);
+                const T aT = -(ut * gt + up * gp);
+                if (h) {
+                    o[0].y = at;
+                    o[1].y = ap;
+                    o[2].y = aT;
+                } else {
+                    o[0].x = at;
+                    o[1].x = ap;
+                    o[2].x = aT;
+                }
+            }
+#pragma unroll
+            for (int P = 0; P < 3; ++P) buf[P * ld + sk(j)] = o[P];
+        }
+        __syncthreads();
+    } else if (MODE == RING_ANALYSIS) {
         NP = NF;
         for (int j = tid; j < n; j += nthr)
             for (int F = 0; F < NF; ++F) {
@@ -843,7 +948,7 @@ template <typename T, int MODE> static void ring_dispatch(hipStream_t st, dim3 g
 template <typename T> int gd_launch_ring(hipStream_t st, int mode, const RingArgs &a)
 {
     const int nthr = 256;
-    int nf = (mode == RING_STEP) ? 15 : a.nfield;
+    int nf = (mode == RING_STEP) ? 15 : (mode == RING_STEP_TANG) ? 4 : a.nfield;
     GD_CHECK(nf >= 1 && nf <= 15, GD_ERR_ARG, "ring: nfield=%d", nf);
     GD_CHECK((long)nf * (a.mmax + 1) <= 8L * nthr, GD_ERR_ARG, "ring: %d fields x %d orders exceed one load batch",
              nf, a.mmax + 1);
@@ -858,6 +963,7 @@ template <typename T> int gd_launch_ring(hipStream_t st, int mode, const RingArg
     switch (mode) {
     case RING_SYNTH: ring_dispatch<T, RING_SYNTH>(st, grid, lds, a); break;
     case RING_ANALYSIS: ring_dispatch<T, RING_ANALYSIS>(st, grid, lds, a); break;
+    case RING_STEP_TANG: ring_dispatch<T, RING_STEP_TANG>(st, grid, lds, a); break;
     default: ring_dispatch<T, RING_STEP>(st, grid, lds, a); break;
     }
     return gd_check_launch("ring_kernel");

@@ -57,7 +57,11 @@ struct RingArgs {
     int coriolis_index;
 };
 
-enum { RING_SYNTH = 0, RING_ANALYSIS = 1, RING_STEP = 2 };
+// RING_STEP: 15 fields in (u, omega, B, j, grad T tangential; u_r, omega_r,
+// B_r, j_r, dT/dr), 5 products out.  RING_STEP_TANG: the step when every
+// radial component is zero -- 4 fields in (u_theta, u_phi, (grad T)_theta,
+// (grad T)_phi), 3 products out (the force and -u.grad T; u x B vanishes).
+enum { RING_SYNTH = 0, RING_ANALYSIS = 1, RING_STEP = 2, RING_STEP_TANG = 3 };
 
 int gd_launch_plm(hipStream_t st, const double *x, const int *moff, int lmax, int mmax, int nlat2, double *tab64,
                   float *tab32, int nlm);

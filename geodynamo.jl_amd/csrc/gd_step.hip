@@ -5,15 +5,19 @@
 //   1. spectral_pre_kernel  curl of u and B (velocity.jl:427-435,
 //      magnetic.jl:335-343), radial components X_r = l(l+1) P_X / r^2
 //      (magnetic.jl:257-259), T/r and dT/dr (scalar_field_common.jl:233-327),
-//      written as Legendre-GEMM input columns in the plan's arithmetic type
+//      written as Legendre-GEMM input columns in the plan's arithmetic type.
+//      Only after set_state and on the first step: afterwards the CNAB2
+//      kernel of step 5 writes these columns for the new state.
 //   2. synth_kernel (vector: u, omega, B, j, grad T; scalar: u_r, omega_r,
-//      B_r, j_r, dT/dr) -> Fourier rings FB
+//      B_r, j_r, dT/dr, or dT/dr alone when the radial components are zero)
+//      -> Fourier rings FB
 //   3. ring_kernel<STEP>: inverse FFTs, u x omega + Coriolis + j x B, u x B,
 //      -u.grad T (velocity.jl:512-527,664-670; magnetic.jl:399-401;
 //      scalar_field_common.jl:401-407), forward FFTs -> GB
 //   4. ana_kernel: vector analysis of the force and of u x B (with the curl
 //      factors of magnetic.jl:437-440), scalar analysis of -u.grad T -> nl
-//   5. cnab2_step_kernel: temperature Dirichlet rows, AB2 (AB1 at step 0,
+//   5. cnab2_chain_kernel (cnab2_step_kernel where the column does not fit in
+//      registers): temperature Dirichlet rows, AB2 (AB1 at step 0,
 //      simulation.jl:1153), rhs + FP64 banded solve per column
 //      (timestep.jl:610-730); the nl history rolls by buffer swap
 //      (simulation.jl:1312-1329).
@@ -36,6 +40,8 @@ struct gd_step {
     DevBuf part, eout, scratch;
     DevBuf status;
     int step = 0;
+    bool chain = false;  // the column-register CNAB2 kernel applies (it also refreshes synV/synS)
+    bool syn_ok = false; // synV/synS hold the synthesis columns of the current state
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
 };
@@ -91,11 +97,12 @@ __global__ void __launch_bounds__(256) spectral_pre_kernel(const double *__restr
         v[(14 + reim) * (size_t)nlm] = (T)jtor;
         v[(16 + reim) * (size_t)nlm] = (T)(tv * rinv);
         T *s = synS + (size_t)r * 10 * nlm + idx;
-        const double f = radials ? 1.0 : 0.0;
-        s[(0 + reim) * (size_t)nlm] = (T)(f * (lf * up * rinv2));
-        s[(2 + reim) * (size_t)nlm] = (T)(f * (lf * wpol * rinv2));
-        s[(4 + reim) * (size_t)nlm] = (T)(f * (lf * Bp * rinv2));
-        s[(6 + reim) * (size_t)nlm] = (T)(f * (lf * jpol * rinv2));
+        if (radials) { // otherwise the radial components stay zero (cleared at creation)
+            s[(0 + reim) * (size_t)nlm] = (T)(lf * up * rinv2);
+            s[(2 + reim) * (size_t)nlm] = (T)(lf * wpol * rinv2);
+            s[(4 + reim) * (size_t)nlm] = (T)(lf * Bp * rinv2);
+            s[(6 + reim) * (size_t)nlm] = (T)(lf * jpol * rinv2);
+        }
         s[(8 + reim) * (size_t)nlm] = (T)d1T;
     }
 }
@@ -109,16 +116,24 @@ static double *cur_state(gd_step *s) { return s->state[s->step & 1].as<double>()
         if (ev) GD_HIP(hipEventRecord(ev[k], s->ctx->stream));      \
     } while (0)
 
+// With every radial component zero (radials = 0) the tangential parts of
+// u x omega, j x B and u x B vanish identically and u_r dT/dr drops out, so
+// only u and grad T need transforms; full_products = 1 keeps every product.
+static bool tangential(const gd_step *s) { return !s->d.radials && !s->d.full_products; }
+
 template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEvent_t *ev)
 {
     hipStream_t st = s->ctx->stream;
     GD_MARK(0);
     gd_sht_plan *p = &s->sht;
     const int nr = s->d.nr, nlm = s->nlm;
-    hipLaunchKernelGGL((spectral_pre_kernel<T>), dim3((nlm + 255) / 256, nr), dim3(256), 0, st,
-                       cur_state(s), p->d_l_of_idx.as<int>(), s->D1.as<double>(), s->D2.as<double>(),
-                       s->rtab.as<double>(), nlm, nr, s->d.bw, s->d.radials, s->synV.as<T>(), s->synS.as<T>());
-    GD_TRY(gd_check_launch("spectral_pre_kernel"));
+    if (!s->syn_ok) {
+        hipLaunchKernelGGL((spectral_pre_kernel<T>), dim3((nlm + 255) / 256, nr), dim3(256), 0, st,
+                           cur_state(s), p->d_l_of_idx.as<int>(), s->D1.as<double>(), s->D2.as<double>(),
+                           s->rtab.as<double>(), nlm, nr, s->d.bw, s->d.radials, s->synV.as<T>(), s->synS.as<T>());
+        GD_TRY(gd_check_launch("spectral_pre_kernel"));
+        s->syn_ok = true;
+    }
     GD_MARK(1);
 
     SynthArgs a{};
@@ -129,52 +144,19 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEven
     a.nlm = nlm;
     a.mmax = p->mmax;
     a.out = s->fb.p;
-    a.nslot = 60;
-    // vector fields u, omega, B, j (S,T) and grad T (S only)
     a.in = s->synV.p;
     a.cin = 18;
-    a.nf = 5;
-    static const int vmap[20] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, -1, -1};
-    for (int k = 0; k < 20; ++k) a.colmap[k] = vmap[k];
     a.tab0 = gd_sht_tab(p, 1);
     a.tab1 = gd_sht_tab(p, 2);
     a.slot0 = 0;
-    GD_TRY(gd_launch_synth_vec<T>(st, a));
-    GD_MARK(2);
-    a.in = s->synS.p;
-    a.cin = 10;
-    a.nf = 5;
-    for (int k = 0; k < 20; ++k) a.colmap[k] = k < 10 ? k : -1;
-    a.tab0 = gd_sht_tab(p, 0);
-    a.tab1 = nullptr;
-    a.slot0 = 40;
-    GD_TRY(gd_launch_synth_scal<T>(st, a));
-    GD_MARK(3);
-
     RingArgs r{};
     gd_sht_fill_ring(p, r, nr);
     r.fb = s->fb.p;
     r.gb = s->gb.p;
-    r.nslot_f = 60;
-    r.nslot_g = 20;
-    r.nfield = 15;
-    for (int F = 0; F < 10; ++F) {
-        const int f = F / 2, c = F % 2;
-        r.fn_off[F] = 8 * f + 2 * c;
-        r.fs_off[F] = 8 * f + 4 + 2 * c;
-    }
-    for (int F = 10; F < 15; ++F) {
-        r.fn_off[F] = 40 + 4 * (F - 10);
-        r.fs_off[F] = 40 + 4 * (F - 10) + 2;
-    }
     r.ro = s->d.Pm / s->d.E;
     r.coriolis_index = s->d.coriolis_index;
-    GD_TRY(gd_launch_ring<T>(st, RING_STEP, r));
-    GD_MARK(4);
-
     AnaArgs g{};
     g.gb = s->gb.p;
-    g.nslot_g = 20;
     g.moff = p->d_moff.as<int>();
     g.ne = p->d_ne.as<int>();
     g.l_of_idx = p->d_l_of_idx.as<int>();
@@ -184,23 +166,98 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEven
     g.mmax = p->mmax;
     const size_t fs = s->fs;
     const int maxgroup = (p->lmax + 2) / 2;
-    // force -> (pol, tor) of velocity nl (velocity.jl:331); u x B -> curl (magnetic.jl:437-440)
-    g.nf = 2;
-    g.gslot0 = 0;
-    g.tab0 = gd_sht_tab(p, 1);
-    g.tab1 = gd_sht_tab(p, 2);
-    g.out[0] = nl_out + 2 * fs; // S(adv)   -> u_pol
-    g.out[1] = nl_out + 0 * fs; // T(adv)   -> u_tor
-    g.out[2] = nl_out + 4 * fs; // l(l+1) S(uxB)  -> B_tor
-    g.out[3] = nl_out + 6 * fs; // -l(l+1) T(uxB) -> B_pol
-    g.scale[0] = 0;
-    g.scale[1] = 0;
-    g.scale[2] = 1;
-    g.scale[3] = 2;
-    GD_TRY(gd_launch_ana<T>(st, g, true, maxgroup));
-    GD_MARK(5);
+
+    if (tangential(s)) {
+        // u (S = u_pol, T = u_tor) and grad T (S = T/r) only; FB records of 16 slots
+        a.nslot = 16;
+        a.nf = 2;
+        static const int vmap[8] = {0, 1, 2, 3, 16, 17, -1, -1};
+        for (int k = 0; k < 20; ++k) a.colmap[k] = k < 8 ? vmap[k] : -1;
+        GD_TRY(gd_launch_synth_vec<T>(st, a));
+        GD_MARK(2);
+        GD_MARK(3);
+        r.nslot_f = 16;
+        r.nslot_g = 12;
+        r.nfield = 4;
+        static const int fn[4] = {0, 2, 8, 10}, fsou[4] = {4, 6, 12, 14};
+        for (int F = 0; F < 4; ++F) {
+            r.fn_off[F] = fn[F];
+            r.fs_off[F] = fsou[F];
+        }
+        GD_TRY(gd_launch_ring<T>(st, RING_STEP_TANG, r));
+        GD_MARK(4);
+        // force -> (pol, tor) of velocity nl (velocity.jl:331); the magnetic nl
+        // is u x B = 0 here and its planes stay zero
+        g.nslot_g = 12;
+        g.nf = 1;
+        g.gslot0 = 0;
+        g.tab0 = gd_sht_tab(p, 1);
+        g.tab1 = gd_sht_tab(p, 2);
+        g.out[0] = nl_out + 2 * fs; // S(adv) -> u_pol
+        g.out[1] = nl_out + 0 * fs; // T(adv) -> u_tor
+        g.scale[0] = 0;
+        g.scale[1] = 0;
+        GD_TRY(gd_launch_ana<T>(st, g, true, maxgroup));
+        GD_MARK(5);
+        g.gslot0 = 8;
+    } else {
+        // vector fields u, omega, B, j (S,T) and grad T (S only)
+        a.nslot = 60;
+        a.nf = 5;
+        static const int vmap[20] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, -1, -1};
+        for (int k = 0; k < 20; ++k) a.colmap[k] = vmap[k];
+        GD_TRY(gd_launch_synth_vec<T>(st, a));
+        GD_MARK(2);
+        a.in = s->synS.p;
+        a.cin = 10;
+        a.tab0 = gd_sht_tab(p, 0);
+        a.tab1 = nullptr;
+        if (s->d.radials) {
+            a.nf = 5;
+            for (int k = 0; k < 20; ++k) a.colmap[k] = k < 10 ? k : -1;
+            a.slot0 = 40;
+        } else {
+            // the radial components are identically zero (their FB slots were
+            // cleared at creation and are never written): only dT/dr is synthesised
+            a.nf = 1;
+            for (int k = 0; k < 20; ++k) a.colmap[k] = k < 2 ? 8 + k : -1;
+            a.slot0 = 56;
+        }
+        GD_TRY(gd_launch_synth_scal<T>(st, a));
+        GD_MARK(3);
+        r.nslot_f = 60;
+        r.nslot_g = 20;
+        r.nfield = 15;
+        for (int F = 0; F < 10; ++F) {
+            const int f = F / 2, c = F % 2;
+            r.fn_off[F] = 8 * f + 2 * c;
+            r.fs_off[F] = 8 * f + 4 + 2 * c;
+        }
+        for (int F = 10; F < 15; ++F) {
+            r.fn_off[F] = 40 + 4 * (F - 10);
+            r.fs_off[F] = 40 + 4 * (F - 10) + 2;
+        }
+        GD_TRY(gd_launch_ring<T>(st, RING_STEP, r));
+        GD_MARK(4);
+        // force -> (pol, tor) of velocity nl (velocity.jl:331); u x B -> curl (magnetic.jl:437-440)
+        g.nslot_g = 20;
+        g.nf = 2;
+        g.gslot0 = 0;
+        g.tab0 = gd_sht_tab(p, 1);
+        g.tab1 = gd_sht_tab(p, 2);
+        g.out[0] = nl_out + 2 * fs; // S(adv)   -> u_pol
+        g.out[1] = nl_out + 0 * fs; // T(adv)   -> u_tor
+        g.out[2] = nl_out + 4 * fs; // l(l+1) S(uxB)  -> B_tor
+        g.out[3] = nl_out + 6 * fs; // -l(l+1) T(uxB) -> B_pol
+        g.scale[0] = 0;
+        g.scale[1] = 0;
+        g.scale[2] = 1;
+        g.scale[3] = 2;
+        GD_TRY(gd_launch_ana<T>(st, g, true, maxgroup));
+        GD_MARK(5);
+        g.gslot0 = 16;
+    }
     g.nf = 1;
-    g.gslot0 = 16;
     g.tab0 = gd_sht_tab(p, 0);
     g.tab1 = nullptr;
     g.out[0] = nl_out + 8 * fs; // -u.grad T -> T
@@ -230,10 +287,40 @@ static int enqueue_step(gd_step *s, int parity, bool boot, hipEvent_t *ev = null
     double *cur = s->nlbuf[parity].as<double>();
     double *prev = boot ? cur : s->nlbuf[parity ^ 1].as<double>();
     GD_TRY(nonlinear(s, cur, ev));
-    GD_TRY(gd_launch_cnab2_step(s->ctx->stream, s->nuLap.as<double>(), s->rtab.as<double>() + s->d.nr,
-                                s->nus.as<double>(), s->LUt.as<double>(), s->sht.d_l_of_idx.as<int>(), s->nl, s->d.nr,
-                                s->d.bw, s->nlm, s->d.dt, s->d.theta, bc_mask(s), s->state[parity].as<double>(),
-                                s->state[parity ^ 1].as<double>(), cur, prev));
+    if (s->chain) {
+        ChainArgs c{};
+        c.nuLap = s->nuLap.as<double>();
+        c.rinv2 = s->rtab.as<double>() + s->d.nr;
+        c.nus = s->nus.as<double>();
+        c.LUt = s->LUt.as<double>();
+        c.l_of_idx = s->sht.d_l_of_idx.as<int>();
+        c.nl = s->nl;
+        c.n = s->d.nr;
+        c.bw = s->d.bw;
+        c.nlm = s->nlm;
+        c.dt = s->d.dt;
+        c.theta = s->d.theta;
+        c.bc_mask = bc_mask(s);
+        c.st_in = s->state[parity].as<double>();
+        c.st_out = s->state[parity ^ 1].as<double>();
+        c.nl_arr = cur;
+        c.np_arr = prev;
+        c.D1 = s->D1.as<double>();
+        c.D2 = s->D2.as<double>();
+        c.rtab = s->rtab.as<double>();
+        c.radials = s->d.radials;
+        c.synV = s->synV.p;
+        c.synS = s->synS.p;
+        c.syn_tang = tangential(s) ? 1 : 0;
+        GD_TRY(gd_launch_cnab2_chain(s->ctx->stream, c, s->d.prec == GD_PREC_FP64));
+        s->syn_ok = true; // synV/synS now describe state[parity ^ 1]
+    } else {
+        GD_TRY(gd_launch_cnab2_step(s->ctx->stream, s->nuLap.as<double>(), s->rtab.as<double>() + s->d.nr,
+                                    s->nus.as<double>(), s->LUt.as<double>(), s->sht.d_l_of_idx.as<int>(), s->nl,
+                                    s->d.nr, s->d.bw, s->nlm, s->d.dt, s->d.theta, bc_mask(s),
+                                    s->state[parity].as<double>(), s->state[parity ^ 1].as<double>(), cur, prev));
+        s->syn_ok = false;
+    }
     GD_MARK(7);
     return GD_OK;
 }
@@ -341,8 +428,12 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
     GD_HIP(hipMemsetAsync(s->state[1].p, 0, sizeof(double) * s->fs * 10, ctx->stream));
     GD_HIP(hipMemsetAsync(s->nlbuf[0].p, 0, sizeof(double) * s->fs * 10, ctx->stream));
     GD_HIP(hipMemsetAsync(s->nlbuf[1].p, 0, sizeof(double) * s->fs * 10, ctx->stream));
+    GD_HIP(hipMemsetAsync(s->synS.p, 0, esz * s->fs * 10, ctx->stream));
+    GD_HIP(hipMemsetAsync(s->fb.p, 0, esz * (size_t)s->sht.nlat2 * nr * nm * 60, ctx->stream));
     GD_HIP(hipStreamSynchronize(ctx->stream));
     s->step = 0;
+    s->chain = gd_cnab2_chain_supported(nr, bw) != 0;
+    s->syn_ok = false;
     return GD_OK;
 }
 
@@ -403,6 +494,7 @@ extern "C" int gd_step_set_state(gd_step *s, const double *re, const double *im,
         b = a + n;
     }
     s->step = 0;
+    s->syn_ok = false;
     GD_TRY(convert_state(s, a, b, cur_state(s), true));
     if (flags & GD_HOST) GD_HIP(hipStreamSynchronize(s->ctx->stream));
     return GD_OK;
@@ -501,7 +593,9 @@ extern "C" int gd_step_energies(gd_step *s, double *out3)
 // Per-kernel-class device time with HIP events on the step's stream:
 // out[k] = average ms per step of class k over nsteps eager steps, classes
 // 0 spectral_pre, 1 synth vector, 2 synth scalar, 3 ring (FFT + products),
-// 4 analysis vector, 5 analysis scalar, 6 CNAB2 rhs + banded solve.
+// 4 analysis vector, 5 analysis scalar, 6 CNAB2 rhs + banded solve.  Where
+// the column-register CNAB2 kernel runs, it also prepares the next step's
+// synthesis columns and class 0 is empty after the first step.
 extern "C" int gd_step_profile(gd_step *s, int nsteps, double *out)
 {
     GD_CHECK(s && out && nsteps >= 1, GD_ERR_ARG, "gd_step_profile: bad args");

@@ -8,7 +8,8 @@ trailing '!' dropped.
 from ._lib import GDError, MissingLError, lib  # noqa: F401
 from .fields import (RadialDomain, SHTnsPhysicalField, SHTnsSpectralField,  # noqa: F401
                      SHTnsVectorField, create_radial_domain, seeded_state)
-from .linear_algebra import (BandedLU, BandedMatrix, apply_banded_matrix, banded_to_dense,  # noqa: F401
+from .linear_algebra import (BandedLU, BandedMatrix, apply_banded_matrix, apply_derivative_matrix,  # noqa: F401
+                             banded_to_dense,
                              create_derivative_matrix, create_radial_laplacian, factorize_banded,
                              solve_banded)
 from .shtnskit_field_functions import (batch_shtnskit_transforms, batch_spectral_to_physical,  # noqa: F401

@@ -89,6 +89,12 @@ def apply_banded_matrix(out, matrix, v):
     return out
 
 
+def apply_derivative_matrix(out, matrix, v):
+    """linear_algebra.jl:203-219: y = D v for one radial vector (the same band
+    product as apply_banded_matrix)."""
+    return apply_banded_matrix(out, matrix, v)
+
+
 def banded_to_dense(matrix):
     """timestep.jl:545-559"""
     N, bw = matrix.size, matrix.bandwidth

@@ -163,6 +163,10 @@ typedef struct {
     int coriolis_index; /* 0: reference indexing theta_{min(j,nlat)}; 1: theta_i */
     int bc_rows;        /* Dirichlet rows (value 0) in S_l: 1 all fields (default), 2 temperature
                            only, 0 none (the reference operator, unstable at these parameters) */
+    int full_products;  /* 0 (default): with radials = 0 the tangential parts of u x omega,
+                           j x B and u x B and the u_r dT/dr term are identically zero, so only
+                           u and grad T are transformed (same result: the dropped terms are
+                           exact zeros); 1: transform every field and form every product */
 } gd_step_desc;
 
 int gd_step_create(gd_ctx *ctx, const gd_step_desc *desc, gd_step **out);

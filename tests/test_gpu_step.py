@@ -115,3 +115,37 @@ def test_step_fp64_matches_oracle_c2_shape(oracle):
     err = _relf(g.get_state_complex(), s)
     print("C2 state rel err", err)
     assert max(err) < 1e-10
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 1e-6)])
+def test_tangential_path_equals_full_products(oracle, prec, tol):
+    """With radials = 0 the default step transforms only u and grad T (the
+    other products are identically zero); full_products = 1 forms them all.
+    Both must give the same nonlinear terms and states."""
+    import gdhip
+
+    print("case: tangential vs full products C0 %s" % prec, flush=True)
+    st = oracle.seeded_state(2024, 31, 31, 33, amp=1e-3)
+    a = gdhip.CNAB2Step(31, 48, 96, 33, Ra=100.0, precision=prec)
+    b = gdhip.CNAB2Step(31, 48, 96, 33, Ra=100.0, precision=prec, full_products=True)
+    for g in (a, b):
+        g.set_state_complex(st)
+    na, nb = a.nonlinear(), b.nonlinear()
+    err = _relf(na, nb)
+    print("nl rel diff", err)
+    assert max(err) < tol and np.all(na[2:4] == 0)
+    for g in (a, b):
+        g.run(1)
+        g.run(3, graph=True)
+    err = _relf(a.get_state_complex(), b.get_state_complex())
+    print("state rel diff after 4 steps", err)
+    assert max(err) < tol
+    o = oracle.Model(31, 48, 96, 33, Ra=100.0)
+    s = st
+    s, nl, prev = o.step(s, None, 0)
+    for k in range(1, 4):
+        s, nl, prev = o.step(s, prev, k)
+    err = _relf(a.get_state_complex(), s)
+    print("tangential vs oracle after 4 steps", err)
+    assert max(err) < (1e-10 if prec == "fp64" else 1e-6)
+
