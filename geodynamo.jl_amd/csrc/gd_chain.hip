@@ -117,6 +117,7 @@ template <int BW, int D, typename TS> __global__ void __launch_bounds__(64) cnab
     const bool pol = (f == 1 || f == 3);
     const bool wA = !c.syn_tang || f == 0 || f == 1 || f == 4;
     const bool wB = f < 4 && !c.syn_tang, wS = !c.syn_tang && ((f == 4) || c.radials);
+    const bool need1 = (f == 4) ? wS : (pol && wB), need2 = pol && wB; // radial derivatives used
     TS *pA = nullptr, *pB = nullptr, *pS = nullptr;
     if (syn) {
         pA = static_cast<TS *>(c.synV) + (size_t)(colA[f] + ri) * nlm + idx;
@@ -159,13 +160,21 @@ template <int BW, int D, typename TS> __global__ void __launch_bounds__(64) cnab
                 const double rinv = rinv_[r], rinv2 = rinv2_[r];
                 const double x = xw[BW];
                 double d1 = 0.0, d2 = 0.0;
+                if (need1) { // uniform
 #pragma unroll
-                for (int q = 0; q < W; ++q) {
-                    const int j = r - BW + q;
-                    const int b = (2 * BW - q) + W * cl(j);
-                    const bool ok = j >= 0 && j < n;
-                    d1 += (ok ? sD1[b] : 0.0) * xw[q];
-                    d2 += (ok ? sD2[b] : 0.0) * xw[q];
+                    for (int q = 0; q < W; ++q) {
+                        const int j = r - BW + q;
+                        const bool ok = j >= 0 && j < n;
+                        d1 += (ok ? sD1[(2 * BW - q) + W * cl(j)] : 0.0) * xw[q];
+                    }
+                }
+                if (need2) {
+#pragma unroll
+                    for (int q = 0; q < W; ++q) {
+                        const int j = r - BW + q;
+                        const bool ok = j >= 0 && j < n;
+                        d2 += (ok ? sD2[(2 * BW - q) + W * cl(j)] : 0.0) * xw[q];
+                    }
                 }
                 double a, bv, sv;
                 if (f == 4) {

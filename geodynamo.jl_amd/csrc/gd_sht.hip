@@ -87,6 +87,21 @@ template <typename T> using acc_t = typename Acc4<T>::type;
 
 template <typename T> __device__ __forceinline__ T shfl_xor3(T v) { return __shfl_xor(v, 3); }
 
+// XCD-aware block order.  Workgroup b of a launch goes to XCD b % 8, and each
+// XCD has its own L2; the blocks of one order m share their Legendre table
+// tile (and their spectral / Fourier inputs), so the logical index is built
+// so that consecutive logical blocks run on one XCD: XCD x takes the x-th
+// contiguous range of the logical order.  The m dimension is ordered
+// 0, mmax, 1, mmax-1, ... so that equal ranges carry equal work (the degree
+// count of m falls linearly with m).
+__device__ __forceinline__ int xcd_logical(int b, int n)
+{
+    constexpr int NX = 8;
+    const int q = n / NX, rem = n % NX, x = b % NX, s = b / NX;
+    return (x < rem ? x * (q + 1) : rem * (q + 1) + (x - rem) * q) + s;
+}
+__device__ __forceinline__ int paired_m(int k, int nm) { return (k & 1) ? nm - 1 - (k >> 1) : (k >> 1); }
+
 // 16-byte vector of T (stores of the staged output tiles)
 template <typename T> struct Vec16;
 template <> struct Vec16<float> { using type = float4; };
@@ -206,7 +221,10 @@ template <typename T, bool VEC, int NF> __global__ void __launch_bounds__(64 * N
     int *scol = reinterpret_cast<int *>(smem_raw);
     T *sA = reinterpret_cast<T *>(smem_raw + 128);
     T *sB = sA + 2 * NT * KT * TM;
-    const int m = blockIdx.z, i0 = blockIdx.x * TM, r0 = blockIdx.y * LT;
+    const int gx = (a.nlat2 + TM - 1) / TM, gy = (a.nr + LT - 1) / LT, nm = a.mmax + 1;
+    const int L = xcd_logical(blockIdx.x, gx * gy * nm);
+    const int bx = L % gx, by = (L / gx) % gy;
+    const int m = paired_m(L / (gx * gy), nm), i0 = bx * TM, r0 = by * LT;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     if (tid < 20) scol[tid] = a.colmap[tid];
     __syncthreads();
@@ -269,7 +287,6 @@ template <typename T, bool VEC, int NF> __global__ void __launch_bounds__(64 * N
     constexpr int EV = 16 / sizeof(T);
     constexpr int NV = TM * LT * RS / EV;
     T *out = static_cast<T *>(a.out);
-    const int nm = a.mmax + 1;
     for (int t = tid; t < NV; t += 64 * NF) {
         const int e = t * EV;
         const int rec = e / RS, so = e - rec * RS;
@@ -293,11 +310,14 @@ template <typename T, bool VEC, int NF> __global__ void __launch_bounds__(64 * N
     static_assert(NTILE * (TM + 1) <= 2 * NT * KT * (TM + NTILE), "analysis tile exceeds the pipeline LDS");
     T *sA = reinterpret_cast<T *>(smem_raw);
     T *sB = sA + 2 * NT * KT * TM; // A and B are double-buffered
-    const int m = blockIdx.z >> 1, p = blockIdx.z & 1;
+    const int gx = (a.maxgroup + TM - 1) / TM, gy = (a.nr + LT - 1) / LT, nmz = a.mmax + 1;
+    const int L = xcd_logical(blockIdx.x, gx * gy * 2 * nmz);
+    const int bx = L % gx, by = (L / gx) % gy, bz = L / (gx * gy);
+    const int m = paired_m(bz >> 1, nmz), p = bz & 1;
     const int K = a.moff[m + 1] - a.moff[m], ne = a.ne[m];
     const int Kp = p ? K - ne : ne;
     const int kbase = a.moff[m] + (p ? ne : 0);
-    const int row0 = blockIdx.x * TM, r0 = blockIdx.y * LT;
+    const int row0 = bx * TM, r0 = by * LT;
     if (row0 >= Kp) return; // uniform over the block
     const int tid = threadIdx.x, nthr = blockDim.x, wave = tid >> 6, lane = tid & 63;
     const T *gb = static_cast<const T *>(a.gb);
@@ -441,12 +461,211 @@ template <typename T, bool VEC, int NF> __global__ void __launch_bounds__(64 * N
         }
     }
 }
+// Analysis, table-resident form: one block of 4 waves per (row tile of TM
+// degrees, m, parity).  The block's table tile -- NT tables x TM degrees x all
+// KN (>= nlat/2, zero-padded) northern nodes -- is read from HBM once, in one
+// batch of loads, into LDS ([t][node][degree], padded rows).  The waves then
+// take the level tiles in turn (wave w: tiles w, w+4, ...); a wave keeps the
+// Fourier operand G of its tile in registers, one element per lane and
+// 4-node step, and reloads each register for its next tile as soon as the
+// MFMA that used it has issued, so the next tile's loads overlap this tile's
+// MFMAs.  STAGE: results go through a per-wave LDS tile and each output
+// column's consecutive idx are stored contiguously.
+constexpr int ANA_WAVES = 4;
+template <typename T, bool VEC, int NF, int KN>
+__global__ void __launch_bounds__(64 * ANA_WAVES) ana_res_kernel(AnaArgs a)
+{
+    constexpr int NT = VEC ? 2 : 1;
+    constexpr int VC = VEC ? 4 * NF : 2 * NF;
+    constexpr int LT = 16 / VC; // levels per wave tile (16 columns)
+    constexpr int KS = KN / 4;
+    constexpr int TMP = TM + 1;
+    constexpr bool STAGE = sizeof(T) == 4;
+    constexpr int NA = NT * TM * KN / (64 * ANA_WAVES);
+    static_assert(NT * TM * KN % (64 * ANA_WAVES) == 0, "table tile must split evenly over the block");
+    extern __shared__ __align__(16) char smem_raw[];
+    T *sA = reinterpret_cast<T *>(smem_raw);    // [t][node][TMP]
+    T *stg = sA + NT * KN * TMP;                 // [wave][16][TMP] when STAGE
+    const int gx = (a.maxgroup + TM - 1) / TM, nmz = a.mmax + 1;
+    const int L = xcd_logical(blockIdx.x, gx * 2 * nmz);
+    const int bx = L % gx, bz = L / gx;
+    const int m = paired_m(bz >> 1, nmz), p = bz & 1;
+    const int K = a.moff[m + 1] - a.moff[m], ne = a.ne[m];
+    const int Kp = p ? K - ne : ne;
+    const int kbase = a.moff[m] + (p ? ne : 0);
+    const int row0 = bx * TM;
+    if (row0 >= Kp) return; // uniform over the block
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const T *gb = static_cast<const T *>(a.gb);
+    const T *tab0 = static_cast<const T *>(a.tab0), *tab1 = static_cast<const T *>(a.tab1);
+
+    // table tile: all loads in flight together, then into LDS
+    {
+        T ra[NA];
+#pragma unroll
+        for (int q = 0; q < NA; ++q) {
+            const int e = tid + q * 64 * ANA_WAVES;
+            const int t = e / (TM * KN), rem = e - t * (TM * KN);
+            const int i = rem / KN, k = rem - i * KN;
+            T v = T(0);
+            if (row0 + i < Kp && k < a.nlat2) v = (t == 0 ? tab0 : tab1)[(size_t)(kbase + row0 + i) * a.nlat2 + k];
+            ra[q] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < NA; ++q) {
+            const int e = tid + q * 64 * ANA_WAVES;
+            const int t = e / (TM * KN), rem = e - t * (TM * KN);
+            const int i = rem / KN, k = rem - i * KN;
+            sA[(t * KN + k) * TMP + i] = ra[q];
+        }
+    }
+
+    // this lane's column of a wave tile, and its G slot for table t
+    const int c = lane & 15, kq = lane >> 4;
+    const int lv = c / VC, fv = c - lv * VC;
+    int slot[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        int comp, reim, pm;
+        if constexpr (VEC) {
+            const int V = fv >> 2, q4 = fv & 3;
+            comp = 2 * V + (q4 >> 1);
+            reim = q4 & 1;
+            // dP has southern sign -(-1)^(l+m): even group uses G-, odd G+; Q the reverse
+            pm = (t == 0) ? (p == 0 ? 1 : 0) : p;
+        } else {
+            comp = fv >> 1;
+            reim = fv & 1;
+            pm = p;
+        }
+        slot[t] = a.gslot0 + 4 * comp + 2 * pm + reim;
+    }
+    const int ntiles = (a.nr + LT - 1) / LT;
+    const int niter = (ntiles + ANA_WAVES - 1) / ANA_WAVES;
+    auto gload = [&](int tile, int t, int ks) {
+        const int r = tile * LT + lv, k = 4 * ks + kq;
+        T v = T(0);
+        if (tile < ntiles && r < a.nr && k < a.nlat2)
+            v = gb[(((size_t)m * a.nlat2 + k) * a.nr + r) * a.nslot_g + slot[t]];
+        return v;
+    };
+    T b[NT][KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) b[t][ks] = gload(wave, t, ks);
+    __syncthreads(); // table tile in LDS
+
+    const size_t fs = (size_t)a.nr * a.nlm;
+    T *wst = stg + wave * 16 * TMP;
+    for (int it = 0; it < niter; ++it) {
+        const int tile = it * ANA_WAVES + wave, next = tile + ANA_WAVES;
+        acc_t<T> acc[NT][2];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[t][0] = acc[t][1] = acc_t<T>{0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const T *Ak = sA + (t * KN + 4 * ks + kq) * TMP;
+                acc[t][0] = mfma16x16x4(Ak[c], b[t][ks], acc[t][0]);
+                acc[t][1] = mfma16x16x4(Ak[16 + c], b[t][ks], acc[t][1]);
+                b[t][ks] = gload(next, t, ks);
+            }
+        }
+        if (tile >= ntiles) continue; // uniform over the wave
+        const int r = tile * LT + lv;
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = rb * 16 + acc_row<T>(lane, v);
+                T o;
+                if constexpr (VEC) {
+                    const T X = acc[0][rb][v], Y = acc[1][rb][v];
+                    const T Yx = shfl_xor3(Y);
+                    const int q = lane & 3;
+                    if (q == 0) o = X + Yx;       // S_re
+                    else if (q == 1) o = X - Yx;  // S_im
+                    else if (q == 2) o = Yx - X;  // T_re
+                    else o = -Yx - X;             // T_im
+                } else {
+                    o = acc[0][rb][v];
+                }
+                if constexpr (STAGE) {
+                    wst[c * TMP + row] = o;
+                } else {
+                    const int grow = row0 + row;
+                    if (grow < Kp && r < a.nr) {
+                        int outk, reim;
+                        if constexpr (VEC) {
+                            outk = 2 * (fv >> 2) + ((fv & 3) >> 1);
+                            reim = fv & 1;
+                        } else {
+                            outk = fv >> 1;
+                            reim = fv & 1;
+                        }
+                        double val = (double)o;
+                        const int idx = kbase + grow;
+                        const int l = a.l_of_idx[idx];
+                        const int sc = a.scale[outk];
+                        if (sc == 0) val = (l > 0) ? val / ((double)l * (double)(l + 1)) : 0.0;
+                        else if (sc == 2) val = -val;
+                        if (m == 0 && reim) val = 0.0; // imag(m=0) := 0 (shtnskit_field_functions.jl:432-435)
+                        a.out[outk][(size_t)reim * fs + (size_t)r * a.nlm + idx] = val;
+                    }
+                }
+            }
+        }
+        if constexpr (STAGE) {
+            // the wave reads back what its own lanes wrote: LDS executes one
+            // wave's accesses in order; the fence keeps the compiler from
+            // reordering them
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int q = 0; q < 16 * TM / 64; ++q) {
+                const int e = lane + 64 * q;
+                const int col = e / TM, row = e - col * TM;
+                const int lv2 = col / VC, fv2 = col - lv2 * VC, r2 = tile * LT + lv2;
+                const int grow = row0 + row;
+                if (grow < Kp && r2 < a.nr) {
+                    int outk, reim;
+                    if constexpr (VEC) {
+                        outk = 2 * (fv2 >> 2) + ((fv2 & 3) >> 1);
+                        reim = fv2 & 1;
+                    } else {
+                        outk = fv2 >> 1;
+                        reim = fv2 & 1;
+                    }
+                    double val = (double)wst[col * TMP + row];
+                    const int idx = kbase + grow;
+                    const int l = a.l_of_idx[idx];
+                    const int sc = a.scale[outk];
+                    if (sc == 0) val = (l > 0) ? val / ((double)l * (double)(l + 1)) : 0.0;
+                    else if (sc == 2) val = -val;
+                    if (m == 0 && reim) val = 0.0; // imag(m=0) := 0 (shtnskit_field_functions.jl:432-435)
+                    a.out[outk][(size_t)reim * fs + (size_t)r2 * a.nlm + idx] = val;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+}
+
+template <typename T, bool VEC, int NF, int KN> constexpr size_t ana_res_lds()
+{
+    constexpr int NT = VEC ? 2 : 1;
+    return sizeof(T) * ((size_t)NT * KN * (TM + 1) + (sizeof(T) == 4 ? (size_t)ANA_WAVES * 16 * (TM + 1) : 0));
+}
+
 } // namespace
 
 template <typename T, bool VEC, int NF> static void launch_synth(hipStream_t st, const SynthArgs &a)
 {
     using TL = SynthTile<VEC, NF>;
-    dim3 grid((a.nlat2 + TM - 1) / TM, (a.nr + TL::LT - 1) / TL::LT, a.mmax + 1);
+    dim3 grid(((a.nlat2 + TM - 1) / TM) * ((a.nr + TL::LT - 1) / TL::LT) * (a.mmax + 1));
     constexpr size_t lds = synth_lds_bytes<T, VEC, NF>();
     hipLaunchKernelGGL((synth_kernel<T, VEC, NF>), grid, dim3(64 * NF), lds, st, a);
 }
@@ -482,11 +701,35 @@ template <typename T> int gd_launch_synth_scal(hipStream_t st, const SynthArgs &
 }
 
 // grid rows cover the largest parity group (maxgroup = ceil((lmax+1)/2))
+// table-resident analysis for nlat/2 <= KN; false when no instance applies
+template <typename T, bool VEC, int NF, int KN> static bool try_ana_res(hipStream_t st, const AnaArgs &a)
+{
+    constexpr size_t lds = ana_res_lds<T, VEC, NF, KN>();
+    if (a.nlat2 > KN || lds > 64 * 1024) return false;
+    const int gx = (a.maxgroup + TM - 1) / TM;
+    hipLaunchKernelGGL((ana_res_kernel<T, VEC, NF, KN>), dim3(gx * 2 * (a.mmax + 1)), dim3(64 * ANA_WAVES), lds, st, a);
+    return true;
+}
+
+template <typename T, bool VEC, int NF> static bool ana_res(hipStream_t st, const AnaArgs &a)
+{
+    if (a.nlat2 <= 32) return try_ana_res<T, VEC, NF, 32>(st, a);
+    if (a.nlat2 <= 48) return try_ana_res<T, VEC, NF, 48>(st, a);
+    if (a.nlat2 <= 96) return try_ana_res<T, VEC, NF, 96>(st, a);
+    if constexpr (sizeof(T) == 4) // the FP64 operand registers of a 128-node tile would spill
+        if (a.nlat2 <= 128) return try_ana_res<T, VEC, NF, 128>(st, a);
+    return false;
+}
+
 template <typename T> static int launch_ana(hipStream_t st, const AnaArgs &a, bool vec, int maxgroup)
 {
+    if (vec && a.nf == 2 && ana_res<T, true, 2>(st, a)) return gd_check_launch("ana_res_kernel<vec>");
+    if (vec && a.nf == 1 && ana_res<T, true, 1>(st, a)) return gd_check_launch("ana_res_kernel<vec>");
+    if (!vec && a.nf == 1 && ana_res<T, false, 1>(st, a)) return gd_check_launch("ana_res_kernel<scal>");
+    // streaming form (large nlat, or LDS beyond 64 KB)
     const int VC = vec ? 4 * a.nf : 2 * a.nf;
     const int LT = 16 * a.nf / VC;
-    dim3 grid((maxgroup + TM - 1) / TM, (a.nr + LT - 1) / LT, 2 * (a.mmax + 1));
+    dim3 grid(((maxgroup + TM - 1) / TM) * ((a.nr + LT - 1) / LT) * 2 * (a.mmax + 1));
     const int NT = vec ? 2 : 1;
     size_t lds = sizeof(T) * 2 * (NT * KT * TM + NT * KT * 16 * a.nf);
     if (vec && a.nf == 2) hipLaunchKernelGGL((ana_kernel<T, true, 2>), grid, dim3(128), lds, st, a);
@@ -499,7 +742,9 @@ template <typename T> static int launch_ana(hipStream_t st, const AnaArgs &a, bo
 template <typename T> int gd_launch_ana(hipStream_t st, const AnaArgs &a, bool vec, int maxgroup)
 {
     GD_CHECK(a.nf >= 1 && a.nf <= (vec ? 2 : 5), GD_ERR_ARG, "ana: nf=%d", a.nf);
-    return launch_ana<T>(st, a, vec, maxgroup);
+    AnaArgs b = a;
+    b.maxgroup = maxgroup;
+    return launch_ana<T>(st, b, vec, maxgroup);
 }
 
 // ====================================================================== FFT ring kernel

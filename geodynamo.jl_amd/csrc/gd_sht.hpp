@@ -36,6 +36,7 @@ struct AnaArgs {
     const void *tab0, *tab1;
     const int *moff, *ne, *l_of_idx;
     int nlat2, nr, nlm, mmax;
+    int maxgroup;         // largest parity group, ceil((lmax+1)/2); set by gd_launch_ana
     // outputs, FP64 m-major [reim][level][idx]; vector field V: out[2V] gets S, out[2V+1] gets T
     double *out[4];
     int scale[4];         // 0: /l(l+1) (0 at l=0), 1: x1, 2: x(-1)
