@@ -53,9 +53,11 @@ def kernel_costs(prec_bytes, tangential=True):
       (1); ring: 15 fields in (60 slots per record, 16 of them zero), 5
       products out; analysis: force and u x B (8) and -u.grad T (1)
     The CNAB2 kernel reads u, nl, nl_prev and writes the new state (64 B per
-    complex coefficient and field) and also writes the next step's synthesis
-    columns (6 or 20 per level), so the separate spectral preparation does
-    not run in the steady state."""
+    complex coefficient and field; 32 B for the two magnetic fields of the
+    tangential step, whose nonlinear terms are identically zero and are not
+    read) and also writes the next step's synthesis columns (6 or 20 per
+    level), so the separate spectral preparation does not run in the steady
+    state."""
     L, nlat, nr = CFG["lmax"], CFG["nlat"], CFG["nr"]
     nlm = (L + 1) * (L + 2) // 2
     nm, nlat2 = L + 1, nlat // 2
@@ -71,7 +73,7 @@ def kernel_costs(prec_bytes, tangential=True):
             "ring_fft_products": dict(bytes=16 * es * four + 12 * es * four, flops=0.0),
             "analysis_vector": dict(bytes=8 * es * four + 2 * 16 * fs, flops=4 * contraction),
             "analysis_scalar": dict(bytes=4 * es * four + 16 * fs, flops=1 * contraction),
-            "cnab2_solve": dict(bytes=5 * 64 * fs + 6 * es * fs, flops=0.0),
+            "cnab2_solve": dict(bytes=(3 * 64 + 2 * 32) * fs + 6 * es * fs, flops=0.0),
         }
     return {
         "spectral_pre": dict(bytes=5 * 16 * fs + 20 * es * fs, flops=0.0),
@@ -86,8 +88,8 @@ def kernel_costs(prec_bytes, tangential=True):
 
 # kernel of each class on the headline (tangential) path, as rocprofv3 names it
 TANG_KERNELS = {"spectral_pre": "spectral_pre_kernel<{T}>", "synth_vector": "synth_kernel<{T}, true, 2>",
-                "ring_fft_products": "ring_kernel<{T}, 3, 384>", "analysis_vector": "ana_kernel<{T}, true, 1>",
-                "analysis_scalar": "ana_kernel<{T}, false, 1>", "cnab2_solve": "cnab2_chain_kernel<3, 8, {T}>"}
+                "ring_fft_products": "ring_kernel<{T}, 3, 384>", "analysis_vector": "ana_res_kernel<{T}, true, 1, 96>",
+                "analysis_scalar": "ana_res_kernel<{T}, false, 1, 96>", "cnab2_solve": "cnab2_chain_kernel<3, 8, {T}>"}
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "pmc_step_lmax127_nr65.json")
 COPY_PROBE_BYTES = 268435456
 
@@ -123,6 +125,35 @@ def derive_traffic(prof_dir):
     with open(TRAFFIC_JSON, "w") as f:
         json.dump(out, f, indent=1)
     return out
+
+
+def summarize_trace(prof_dir):
+    """Markdown table of a rocprofv3 --kernel-trace --stats run of this bench
+    (prof_dir/trace/run_kernel_stats.csv) beside the HBM traffic of the
+    headline kernels (TRAFFIC_JSON)."""
+    import csv
+
+    with open(os.path.join(prof_dir, "trace", "run_kernel_stats.csv")) as f:
+        rows = list(csv.DictReader(f))
+    traffic = {}
+    if os.path.exists(TRAFFIC_JSON):
+        with open(TRAFFIC_JSON) as f:
+            tj = json.load(f)
+        for prec in ("fp32", "fp64"):
+            for cls, rec in tj.get(prec, {}).items():
+                traffic[rec["kernel"]] = (cls, rec)
+    lines = ["| kernel | calls | avg us | min us | max us | class | HBM read MB | HBM write MB |",
+             "|---|---|---|---|---|---|---|---|"]
+    for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
+        name = r["Name"].replace("(anonymous namespace)::", "").replace("void ", "")
+        cls, rd, wr = "", "", ""
+        if r["Name"] in traffic:
+            cls, rec = traffic[r["Name"]]
+            rd, wr = "%.1f" % (rec["read_bytes"] / 1e6), "%.1f" % (rec["write_bytes"] / 1e6)
+        lines.append("| `%s` | %s | %.2f | %.2f | %.2f | %s | %s | %s |" % (
+            name.split("(")[0], r["Calls"], float(r["AverageNs"]) / 1e3, float(r["MinNs"]) / 1e3,
+            float(r["MaxNs"]) / 1e3, cls, rd, wr))
+    return "\n".join(lines)
 
 
 def roofline(prof_ms, prec):
@@ -222,9 +253,14 @@ def main():
                     help="skip the full-products comparison (profiling runs of the headline path)")
     ap.add_argument("--derive-traffic", metavar="DIR",
                     help="only turn rocprofv3 FETCH_SIZE/WRITE_SIZE runs under DIR into the traffic JSON")
+    ap.add_argument("--summarize-trace", metavar="DIR",
+                    help="only print a markdown table of the rocprofv3 kernel trace under DIR")
     args = ap.parse_args()
     if args.derive_traffic:
         print(json.dumps(derive_traffic(args.derive_traffic), indent=1))
+        return
+    if args.summarize_trace:
+        print(summarize_trace(args.summarize_trace))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs the CNAB2 step on a single MI355X "

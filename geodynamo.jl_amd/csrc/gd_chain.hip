@@ -47,7 +47,12 @@ template <int BW, int D, typename TS> __global__ void __launch_bounds__(64) cnab
     const double nulf = c.nus[k] * lf;
     const double *LU = c.LUt + (size_t)k * W * n * c.nl + l;
     const size_t fs = (size_t)nlm * n, off = (size_t)(2 * f + ri) * fs + idx;
-    const double *u = c.st_in + off, *pn = c.nl_arr + off, *pp = c.np_arr + off;
+    const double *u = c.st_in + off;
+    // a field whose nonlinear terms are known to be zero reads one zero row (L2-resident)
+    const bool nlz = (c.nl_zero_mask >> f) & 1;
+    const double *pn = nlz ? c.zero_row + idx : c.nl_arr + off;
+    const double *pp = nlz ? c.zero_row + idx : c.np_arr + off;
+    const size_t nstride = nlz ? 0 : (size_t)nlm;
     double *o = c.st_out + off;
     const bool tbc = (f == 4);
     const bool bcr = (c.bc_mask >> f) & 1;
@@ -70,8 +75,8 @@ template <int BW, int D, typename TS> __global__ void __launch_bounds__(64) cnab
         const int jn = i + BW + 1, ic = cl(i);
         const double v = u[(size_t)nlm * cl(jn)];
         su[s] = urow(jn) ? v : 0.0;
-        sn[s] = pn[(size_t)nlm * ic];
-        sp[s] = pp[(size_t)nlm * ic];
+        sn[s] = pn[nstride * ic];
+        sp[s] = pp[nstride * ic];
 #pragma unroll
         for (int q = 0; q < BW; ++q) {
             const int j = i - BW + q;
@@ -209,6 +214,7 @@ int gd_cnab2_chain_supported(int n, int bw)
 int gd_launch_cnab2_chain(hipStream_t st, const ChainArgs &c, bool syn_fp64)
 {
     GD_CHECK(c.st_in != c.st_out, GD_ERR_ARG, "cnab2_chain: in and out state must differ");
+    GD_CHECK(!c.nl_zero_mask || c.zero_row, GD_ERR_ARG, "cnab2_chain: nl_zero_mask needs zero_row");
     GD_CHECK(!c.synV || (c.synS && c.D1 && c.D2 && c.rtab), GD_ERR_ARG, "cnab2_chain: incomplete synthesis outputs");
     GD_CHECK(gd_cnab2_chain_supported(c.n, c.bw), GD_ERR_ARG, "cnab2_chain: n=%d bw=%d unsupported", c.n, c.bw);
     dim3 grid((c.nlm + 63) / 64, 5, 2);

@@ -29,6 +29,8 @@ struct ChainArgs {
     const double *st_in;
     double *st_out;
     const double *nl_arr, *np_arr;
+    int nl_zero_mask;          // bit f: nl and nl_prev of field f are identically zero
+    const double *zero_row;    // nlm zeros, read in their place
     const double *D1, *D2, *rtab;
     int radials;
     int syn_tang; // only the u and T/r synthesis columns are needed (the tangential step)

@@ -919,13 +919,17 @@ constexpr int ct_radix(int n, int stage)
     return ct_next_radix(n);
 }
 
-template <int R, int NS, int N, int DIR, typename T>
+constexpr int RING_THREADS = 256; // ring_kernel block size
+
+// NFFT > 0: the transform count is a compile-time constant (nfft is ignored)
+template <int R, int NS, int N, int DIR, typename T, int NFFT>
 __device__ __noinline__ void fft_stage_ct(C2<T> *buf, const C2<T> *tw, int nfft)
 {
-    constexpr int IT = MAXC / R;
     constexpr int nb = N / R, twstep = N / (NS * R), LD = ring_ld(N);
-    const int total = nfft * nb;
-    const int tid = threadIdx.x, nthr = blockDim.x;
+    constexpr int nthr = RING_THREADS;
+    constexpr int IT = NFFT > 0 ? (NFFT * nb + nthr - 1) / nthr : MAXC / R;
+    const int total = (NFFT > 0 ? NFFT : nfft) * nb;
+    const int tid = threadIdx.x;
     C2<T> v[IT][R];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
@@ -961,13 +965,13 @@ __device__ __noinline__ void fft_stage_ct(C2<T> *buf, const C2<T> *tw, int nfft)
     __syncthreads();
 }
 
-template <typename T, int N, int DIR, int S = 0, int NS = 1>
+template <typename T, int N, int DIR, int NFFT = 0, int S = 0, int NS = 1>
 __device__ __forceinline__ void fft_ct(C2<T> *buf, const C2<T> *tw, int nfft)
 {
     if constexpr (S < ct_nstages(N)) {
         constexpr int R = ct_radix(N, S);
-        fft_stage_ct<R, NS, N, DIR, T>(buf, tw, nfft);
-        fft_ct<T, N, DIR, S + 1, NS * R>(buf, tw, nfft);
+        fft_stage_ct<R, NS, N, DIR, T, NFFT>(buf, tw, nfft);
+        fft_ct<T, N, DIR, NFFT, S + 1, NS * R>(buf, tw, nfft);
     }
 }
 
@@ -984,7 +988,8 @@ template <typename T, int MODE, int NC> __global__ void __launch_bounds__(256) r
     __shared__ int srad[12];
     __shared__ int sfn[16], sfs[16];
     const int i = blockIdx.x, r = blockIdx.y;
-    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int tid = threadIdx.x;
+    constexpr int nthr = RING_THREADS;
     const int is = a.nlat - 1 - i; // southern mirror
     const C2<T> *twg = static_cast<const C2<T> *>(a.tw);
     for (int k = tid; k < n; k += nthr) tw[k] = twg[k];
@@ -994,7 +999,10 @@ template <typename T, int MODE, int NC> __global__ void __launch_bounds__(256) r
         sfs[tid] = a.fs_off[tid];
     }
     __syncthreads();
-    const int NF = a.nfield;
+    // fields in / products out: compile-time for the step modes
+    constexpr int NFC = MODE == RING_STEP ? 15 : MODE == RING_STEP_TANG ? 4 : 0;
+    constexpr int NPC = MODE == RING_STEP ? 5 : MODE == RING_STEP_TANG ? 3 : 0;
+    const int NF = NFC ? NFC : a.nfield;
 
     if (MODE == RING_SYNTH || MODE == RING_STEP || MODE == RING_STEP_TANG) {
         // Hermitian spectra of the north (X) and south (Y) rings packed as Z = X + iY
@@ -1028,13 +1036,11 @@ template <typename T, int MODE, int NC> __global__ void __launch_bounds__(256) r
                 }
             }
         }
-        const int nz = n - 2 * nm + 1;
-        for (int e = tid; e < NF * nz; e += nthr) {
-            const int F = e / nz, k = nm + e % nz;
-            buf[F * ld + sk(k)] = C2<T>{T(0), T(0)};
-        }
+        // zero the modes nm .. n-nm of every field
+        for (int k = nm + tid; k <= n - nm; k += nthr)
+            for (int F = 0; F < NF; ++F) buf[F * ld + sk(k)] = C2<T>{T(0), T(0)};
         __syncthreads();
-        if constexpr (NC > 0) fft_ct<T, NC, +1>(buf, tw, NF);
+        if constexpr (NC > 0) fft_ct<T, NC, +1, NFC>(buf, tw, NF);
         else fft_batch<T>(buf, tw, NF, n, ld, srad, a.nstages, +1);
     }
 
@@ -1145,14 +1151,15 @@ template <typename T, int MODE, int NC> __global__ void __launch_bounds__(256) r
         __syncthreads();
     }
 
-    if constexpr (NC > 0) fft_ct<T, NC, -1>(buf, tw, NP);
+    if constexpr (NC > 0) fft_ct<T, NC, -1, NPC>(buf, tw, NP);
     else fft_batch<T>(buf, tw, NP, n, ld, srad, a.nstages, -1);
 
     // split Z into north X and south Y, form G+- = (X +- Y) * 2 pi w_i / n
     const T c = static_cast<const T *>(a.wq)[i];
     T *gb = static_cast<T *>(a.gb);
-    for (int e = tid; e < NP * nm; e += nthr) {
-        const int P = e % NP, m = e / NP;
+    const int NPn = NPC ? NPC : NP;
+    for (int e = tid; e < NPn * nm; e += nthr) {
+        const int P = e % NPn, m = e / NPn;
         const C2<T> Z = buf[P * ld + sk(m)];
         C2<T> X, Y;
         if (m == 0) {
@@ -1174,7 +1181,7 @@ template <typename T, int MODE, int NC> __global__ void __launch_bounds__(256) r
 
 template <typename T, int MODE, int NC> static void ring_launch(hipStream_t st, dim3 grid, size_t lds, const RingArgs &a)
 {
-    hipLaunchKernelGGL((ring_kernel<T, MODE, NC>), grid, dim3(256), lds, st, a);
+    hipLaunchKernelGGL((ring_kernel<T, MODE, NC>), grid, dim3(RING_THREADS), lds, st, a);
 }
 
 template <typename T, int MODE> static void ring_dispatch(hipStream_t st, dim3 grid, size_t lds, const RingArgs &a)
@@ -1192,7 +1199,7 @@ template <typename T, int MODE> static void ring_dispatch(hipStream_t st, dim3 g
 
 template <typename T> int gd_launch_ring(hipStream_t st, int mode, const RingArgs &a)
 {
-    const int nthr = 256;
+    const int nthr = RING_THREADS;
     int nf = (mode == RING_STEP) ? 15 : (mode == RING_STEP_TANG) ? 4 : a.nfield;
     GD_CHECK(nf >= 1 && nf <= 15, GD_ERR_ARG, "ring: nfield=%d", nf);
     GD_CHECK((long)nf * (a.mmax + 1) <= 8L * nthr, GD_ERR_ARG, "ring: %d fields x %d orders exceed one load batch",
