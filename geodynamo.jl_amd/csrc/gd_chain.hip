@@ -222,8 +222,8 @@ int gd_launch_cnab2_chain(hipStream_t st, const ChainArgs &c, bool syn_fp64)
     GD_CHECK(lds <= 64 * 1024, GD_ERR_ARG, "cnab2_chain: n=%d bw=%d needs %zu B of LDS", c.n, c.bw, lds);
 #define GD_CHAIN(B)                                                                                              \
     if (c.bw == B) {                                                                                             \
-        if (syn_fp64) hipLaunchKernelGGL((cnab2_chain_kernel<B, 8, double>), grid, dim3(64), lds, st, c);        \
-        else hipLaunchKernelGGL((cnab2_chain_kernel<B, 8, float>), grid, dim3(64), lds, st, c);                  \
+        if (syn_fp64) hipLaunchKernelGGL((cnab2_chain_kernel<B, 4, double>), grid, dim3(64), lds, st, c);        \
+        else hipLaunchKernelGGL((cnab2_chain_kernel<B, 4, float>), grid, dim3(64), lds, st, c);                  \
         return gd_check_launch("cnab2_chain_kernel");                                                            \
     }
     GD_CHAIN(1) GD_CHAIN(2) GD_CHAIN(3) GD_CHAIN(4) GD_CHAIN(5) GD_CHAIN(6) GD_CHAIN(7)

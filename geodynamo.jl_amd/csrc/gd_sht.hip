@@ -108,11 +108,12 @@ template <> struct Vec16<float> { using type = float4; };
 template <> struct Vec16<double> { using type = double2; };
 
 // Tile shape of a synthesis block: NF fields, VC virtual columns per level
-// (vector: S_re S_im T_re T_im per field, scalar: re im), NTILE = 16 NF
-// columns = LT levels; RS output slots per (node, level) record.
-template <bool VEC, int NF> struct SynthTile {
+// (vector: S_re S_im T_re T_im per field, scalar: re im), NW waves of 16
+// columns each, NTILE = 16 NW columns = LT levels; RS output slots per
+// (node, level) record.
+template <bool VEC, int NF, int NW> struct SynthTile {
     static constexpr int VC = VEC ? 4 * NF : 2 * NF;
-    static constexpr int NTILE = 16 * NF;
+    static constexpr int NTILE = 16 * NW;
     static constexpr int LT = NTILE / VC;
     static constexpr int RS = VEC ? 8 * NF : 4 * NF;
     static constexpr int NT = VEC ? 2 : 1; // tables: (dP, mP/sin) or P
@@ -123,25 +124,27 @@ template <bool VEC, int NF> struct SynthTile {
 // software-pipelined sequence -- chunk c+1 is loaded from global memory into
 // registers while the MFMAs consume chunk c from LDS (double-buffered LDS, one
 // barrier per chunk).
-template <typename T, bool VEC, int NF>
+template <typename T, bool VEC, int NF, int NW, int KC>
 __device__ __forceinline__ void synth_pipeline(const SynthArgs &a, const T *__restrict__ in, const T *__restrict__ tab0,
                                                const T *__restrict__ tab1, T *sA, T *sB, const int *scol, int m,
-                                               int i0, int r0, acc_t<T> (&accE)[SynthTile<VEC, NF>::NT][2],
-                                               acc_t<T> (&accO)[SynthTile<VEC, NF>::NT][2])
+                                               int i0, int r0, acc_t<T> (&accE)[SynthTile<VEC, NF, NW>::NT][2],
+                                               acc_t<T> (&accO)[SynthTile<VEC, NF, NW>::NT][2])
 {
-    using TL = SynthTile<VEC, NF>;
+    using TL = SynthTile<VEC, NF, NW>;
+    static_assert(TL::NTILE % TL::VC == 0, "a block tile holds whole levels");
     constexpr int NT = TL::NT, VC = TL::VC, NTILE = TL::NTILE;
-    constexpr int RA = (NT * KT * TM + 64 * NF - 1) / (64 * NF); // A elements per thread (64*NF threads)
-    constexpr int RB = KT * 16 / 64;                             // B elements per thread
-    constexpr int sAsz = NT * KT * TM, sBsz = KT * NTILE;
-    const int tid = threadIdx.x, nthr = blockDim.x;
+    constexpr int RA = (NT * KC * TM + 64 * NW - 1) / (64 * NW); // A elements per thread
+    constexpr int RB = KC / 4;                                   // B elements per thread
+    constexpr int sAsz = NT * KC * TM, sBsz = KC * NTILE;
+    constexpr int nthr = 64 * NW;
+    const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int K = a.moff[m + 1] - a.moff[m], ne = a.ne[m];
-    const int nE = (ne + KT - 1) / KT, nC = nE + (K - ne + KT - 1) / KT;
+    const int nE = (ne + KC - 1) / KC, nC = nE + (K - ne + KC - 1) / KC;
     T ra[RA], rb[RB];
     auto gload = [&](int c) {
         const int p = c < nE ? 0 : 1;
-        const int k0 = (p ? c - nE : c) * KT;
+        const int k0 = (p ? c - nE : c) * KC;
         const int Kp = p ? K - ne : ne;
         const int kbase = a.moff[m] + (p ? ne : 0);
 #pragma unroll
@@ -149,7 +152,7 @@ __device__ __forceinline__ void synth_pipeline(const SynthArgs &a, const T *__re
             const int e = tid + q * nthr;
             T v = T(0);
             if (e < sAsz) {
-                const int i = e % TM, k = (e / TM) % KT, t = e / (TM * KT);
+                const int i = e % TM, k = (e / TM) % KC, t = e / (TM * KC);
                 if (k0 + k < Kp && i0 + i < a.nlat2) v = (t == 0 ? tab0 : tab1)[(size_t)(kbase + k0 + k) * a.nlat2 + i0 + i];
             }
             ra[q] = v;
@@ -157,7 +160,7 @@ __device__ __forceinline__ void synth_pipeline(const SynthArgs &a, const T *__re
 #pragma unroll
         for (int q = 0; q < RB; ++q) {
             const int e = tid + q * nthr;
-            const int k = e % KT, vv = e / KT;
+            const int k = e % KC, vv = e / KC;
             const int lv = vv / VC, fv = vv - lv * VC;
             const int col = scol[fv], r = r0 + lv;
             T v = T(0);
@@ -175,20 +178,20 @@ __device__ __forceinline__ void synth_pipeline(const SynthArgs &a, const T *__re
 #pragma unroll
         for (int q = 0; q < RB; ++q) {
             const int e = tid + q * nthr;
-            const int k = e % KT, vv = e / KT;
+            const int k = e % KC, vv = e / KC;
             B[k * NTILE + vv] = rb[q];
         }
     };
     auto mma = [&](int b, acc_t<T>(&acc)[NT][2]) {
         const T *A = sA + b * sAsz, *B = sB + b * sBsz;
 #pragma unroll
-        for (int ks = 0; ks < KT; ks += 4) {
+        for (int ks = 0; ks < KC; ks += 4) {
             const T bv = B[(ks + (lane >> 4)) * NTILE + wave * 16 + (lane & 15)];
 #pragma unroll
             for (int t = 0; t < NT; ++t)
 #pragma unroll
                 for (int rb_ = 0; rb_ < 2; ++rb_) {
-                    const T av = A[(t * KT + ks + (lane >> 4)) * TM + rb_ * 16 + (lane & 15)];
+                    const T av = A[(t * KC + ks + (lane >> 4)) * TM + rb_ * 16 + (lane & 15)];
                     acc[t][rb_] = mfma16x16x4(av, bv, acc[t][rb_]);
                 }
         }
@@ -203,24 +206,25 @@ __device__ __forceinline__ void synth_pipeline(const SynthArgs &a, const T *__re
     }
 }
 
-template <typename T, bool VEC, int NF> constexpr size_t synth_lds_bytes()
+template <typename T, bool VEC, int NF, int NW, int KC> constexpr size_t synth_lds_bytes()
 {
-    using TL = SynthTile<VEC, NF>;
-    const size_t pipe = 2 * (TL::NT * KT * TM + KT * TL::NTILE), tile = (size_t)TM * TL::LT * TL::RS;
+    using TL = SynthTile<VEC, NF, NW>;
+    const size_t pipe = 2 * (TL::NT * KC * TM + KC * TL::NTILE), tile = (size_t)TM * TL::LT * TL::RS;
     return 128 + sizeof(T) * (pipe > tile ? pipe : tile);
 }
 
 // The block's output tile -- TM nodes x LT levels, RS contiguous slots of each
 // (node, level, m) record -- is staged in LDS and stored as 16-byte vectors,
 // one record run after another.
-template <typename T, bool VEC, int NF> __global__ void __launch_bounds__(64 * NF) synth_kernel(SynthArgs a)
+template <typename T, bool VEC, int NF, int NW, int KC>
+__global__ void __launch_bounds__(64 * NW) synth_kernel(SynthArgs a)
 {
-    using TL = SynthTile<VEC, NF>;
+    using TL = SynthTile<VEC, NF, NW>;
     constexpr int NT = TL::NT, VC = TL::VC, LT = TL::LT, RS = TL::RS;
     extern __shared__ __align__(16) char smem_raw[];
     int *scol = reinterpret_cast<int *>(smem_raw);
     T *sA = reinterpret_cast<T *>(smem_raw + 128);
-    T *sB = sA + 2 * NT * KT * TM;
+    T *sB = sA + 2 * NT * KC * TM;
     const int gx = (a.nlat2 + TM - 1) / TM, gy = (a.nr + LT - 1) / LT, nm = a.mmax + 1;
     const int L = xcd_logical(blockIdx.x, gx * gy * nm);
     const int bx = L % gx, by = (L / gx) % gy;
@@ -239,7 +243,7 @@ template <typename T, bool VEC, int NF> __global__ void __launch_bounds__(64 * N
             accE[t][rb] = acc_t<T>{0, 0, 0, 0};
             accO[t][rb] = acc_t<T>{0, 0, 0, 0};
         }
-    synth_pipeline<T, VEC, NF>(a, in, tab0, tab1, sA, sB, scol, m, i0, r0, accE, accO);
+    synth_pipeline<T, VEC, NF, NW, KC>(a, in, tab0, tab1, sA, sB, scol, m, i0, r0, accE, accO);
     __syncthreads(); // the tile below reuses the pipeline's LDS
 
     T *tile = sA; // [node][level][slot]
@@ -287,7 +291,7 @@ template <typename T, bool VEC, int NF> __global__ void __launch_bounds__(64 * N
     constexpr int EV = 16 / sizeof(T);
     constexpr int NV = TM * LT * RS / EV;
     T *out = static_cast<T *>(a.out);
-    for (int t = tid; t < NV; t += 64 * NF) {
+    for (int t = tid; t < NV; t += 64 * NW) {
         const int e = t * EV;
         const int rec = e / RS, so = e - rec * RS;
         const int il = rec / LT, l2 = rec - il * LT;
@@ -662,12 +666,12 @@ template <typename T, bool VEC, int NF, int KN> constexpr size_t ana_res_lds()
 
 } // namespace
 
-template <typename T, bool VEC, int NF> static void launch_synth(hipStream_t st, const SynthArgs &a)
+template <typename T, bool VEC, int NF, int NW, int KC> static void launch_synth(hipStream_t st, const SynthArgs &a)
 {
-    using TL = SynthTile<VEC, NF>;
+    using TL = SynthTile<VEC, NF, NW>;
     dim3 grid(((a.nlat2 + TM - 1) / TM) * ((a.nr + TL::LT - 1) / TL::LT) * (a.mmax + 1));
-    constexpr size_t lds = synth_lds_bytes<T, VEC, NF>();
-    hipLaunchKernelGGL((synth_kernel<T, VEC, NF>), grid, dim3(64 * NF), lds, st, a);
+    constexpr size_t lds = synth_lds_bytes<T, VEC, NF, NW, KC>();
+    hipLaunchKernelGGL((synth_kernel<T, VEC, NF, NW, KC>), grid, dim3(64 * NW), lds, st, a);
 }
 
 // The staged tile is stored in 16-byte vectors: record stride and first slot
@@ -683,9 +687,9 @@ template <typename T> int gd_launch_synth_vec(hipStream_t st, const SynthArgs &a
 {
     GD_TRY(synth_check(a, "synth_vec"));
     GD_CHECK(a.slot0 + 8 * a.nf <= a.nslot, GD_ERR_ARG, "synth_vec: slots %d+%d exceed %d", a.slot0, 8 * a.nf, a.nslot);
-    if (a.nf == 5) launch_synth<T, true, 5>(st, a);
-    else if (a.nf == 2) launch_synth<T, true, 2>(st, a);
-    else if (a.nf == 1) launch_synth<T, true, 1>(st, a);
+    if (a.nf == 5) launch_synth<T, true, 5, 5, 16>(st, a);
+    else if (a.nf == 2) launch_synth<T, true, 2, 2, 16>(st, a);
+    else if (a.nf == 1) launch_synth<T, true, 1, 1, 16>(st, a);
     else GD_CHECK(false, GD_ERR_ARG, "synth_vec: nf=%d (1, 2 or 5)", a.nf);
     return gd_check_launch("synth_kernel<vec>");
 }
@@ -694,8 +698,8 @@ template <typename T> int gd_launch_synth_scal(hipStream_t st, const SynthArgs &
 {
     GD_TRY(synth_check(a, "synth_scal"));
     GD_CHECK(a.slot0 + 4 * a.nf <= a.nslot, GD_ERR_ARG, "synth_scal: slots %d+%d exceed %d", a.slot0, 4 * a.nf, a.nslot);
-    if (a.nf == 5) launch_synth<T, false, 5>(st, a);
-    else if (a.nf == 1) launch_synth<T, false, 1>(st, a);
+    if (a.nf == 5) launch_synth<T, false, 5, 5, 16>(st, a);
+    else if (a.nf == 1) launch_synth<T, false, 1, 1, 16>(st, a);
     else GD_CHECK(false, GD_ERR_ARG, "synth_scal: nf=%d (1 or 5)", a.nf);
     return gd_check_launch("synth_kernel<scal>");
 }
