@@ -16,6 +16,9 @@
 //                    values never leave LDS.
 #include <cmath>
 
+#include <cstdint>
+#include <type_traits>
+
 #include "gd_sht.hpp"
 
 // ====================================================================== P_lm
@@ -120,103 +123,134 @@ template <bool VEC, int NF, int NW> struct SynthTile {
 };
 
 // Synthesis contraction for one (m, node tile, level tile), both parity
-// groups: the K chunks of the even group, then of the odd group, run as one
-// software-pipelined sequence -- chunk c+1 is loaded from global memory into
-// registers while the MFMAs consume chunk c from LDS (double-buffered LDS, one
-// barrier per chunk).
-template <typename T, bool VEC, int NF, int NW, int KC>
+// groups: the K chunks (KC degrees) of the even group, then of the odd group,
+// run as one software-pipelined sequence -- chunk c+1 is loaded into
+// registers while the MFMAs consume chunk c.
+//   A (tables, shared by the NW waves): double-buffered in LDS with a padded
+//     row stride TMA = TM + 4 (the 4-row MFMA step then hits disjoint bank
+//     halves); loaded as 16-byte vectors through per-thread pointers when
+//     VL (nlat/2 and the node tile are multiples of 4), else element-wise.
+//   B (spectral columns, one wave's 16 columns): straight into registers in
+//     the MFMA operand layout.  Within a 16-degree chunk, lane group g
+//     (lane / 16) takes degrees 4g .. 4g+3 over the chunk's 4 MFMA steps (the
+//     same permutation indexes A), so each lane reads 4 consecutive degrees.
+constexpr int TMA = TM + 4;
+template <typename T, bool VEC, int NF, int NW, int KC, bool VL>
 __device__ __forceinline__ void synth_pipeline(const SynthArgs &a, const T *__restrict__ in, const T *__restrict__ tab0,
-                                               const T *__restrict__ tab1, T *sA, T *sB, const int *scol, int m,
+                                               const T *__restrict__ tab1, T *sA, const int *scol, int m,
                                                int i0, int r0, acc_t<T> (&accE)[SynthTile<VEC, NF, NW>::NT][2],
                                                acc_t<T> (&accO)[SynthTile<VEC, NF, NW>::NT][2])
 {
     using TL = SynthTile<VEC, NF, NW>;
     static_assert(TL::NTILE % TL::VC == 0, "a block tile holds whole levels");
-    constexpr int NT = TL::NT, VC = TL::VC, NTILE = TL::NTILE;
-    constexpr int RA = (NT * KC * TM + 64 * NW - 1) / (64 * NW); // A elements per thread
-    constexpr int RB = KC / 4;                                   // B elements per thread
-    constexpr int sAsz = NT * KC * TM, sBsz = KC * NTILE;
+    static_assert(KC % 16 == 0, "chunks are whole 16-degree MFMA groups");
+    constexpr int NT = TL::NT, VC = TL::VC;
     constexpr int nthr = 64 * NW;
-    const int tid = threadIdx.x;
-    const int wave = tid >> 6, lane = tid & 63;
+    constexpr int sAsz = NT * KC * TMA;
+    constexpr int EA = VL ? 16 / (int)sizeof(T) : 1;       // elements per A load
+    constexpr int NAE = NT * KC * TM / EA;                 // A loads per chunk
+    constexpr int RA = (NAE + nthr - 1) / nthr;            // per thread
+    constexpr int RB = KC / 4;                             // B registers per chunk
+    using VA = typename std::conditional<VL, typename Vec16<T>::type, T>::type;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int K = a.moff[m + 1] - a.moff[m], ne = a.ne[m];
     const int nE = (ne + KC - 1) / KC, nC = nE + (K - ne + KC - 1) / KC;
-    T ra[RA], rb[RB];
-    auto gload = [&](int c) {
+    const int kb0 = a.moff[m];
+
+    // per-thread A element: table t, degree row k of the chunk, node offset i
+    const T *pa[RA];
+    int ka[RA], sa[RA];
+    bool va[RA];
+#pragma unroll
+    for (int q = 0; q < RA; ++q) {
+        const int e = tid + q * nthr;
+        const int t = e / (KC * TM / EA), rem = e - t * (KC * TM / EA);
+        const int k = rem / (TM / EA), i = (rem - k * (TM / EA)) * EA;
+        va[q] = e < NAE && i0 + i < a.nlat2;
+        pa[q] = (t == 0 || !va[q] ? tab0 : tab1) + i0 + i;
+        ka[q] = k;
+        sa[q] = (t * KC + k) * TMA + i;
+    }
+    // this lane's B column
+    const int c16 = lane & 15, g = lane >> 4;
+    const int vcol = wave * 16 + c16;
+    const int lv = vcol / VC, fv = vcol - lv * VC;
+    const int col = scol[fv], r = r0 + lv;
+    const bool vb = col >= 0 && r < a.nr;
+    const T *pb = in + (size_t)(vb ? r * a.cin + col : 0) * a.nlm + kb0;
+
+    VA ra[RA];
+    T rb[RB];
+    auto chunk = [&](int c, int &kbase, int &k0, int &Kp) {
         const int p = c < nE ? 0 : 1;
-        const int k0 = (p ? c - nE : c) * KC;
-        const int Kp = p ? K - ne : ne;
-        const int kbase = a.moff[m] + (p ? ne : 0);
+        k0 = (p ? c - nE : c) * KC;
+        Kp = p ? K - ne : ne;
+        kbase = p ? ne : 0; // relative to moff[m]
+    };
+    auto gload = [&](int c) {
+        int kbase, k0, Kp;
+        chunk(c, kbase, k0, Kp);
 #pragma unroll
         for (int q = 0; q < RA; ++q) {
-            const int e = tid + q * nthr;
-            T v = T(0);
-            if (e < sAsz) {
-                const int i = e % TM, k = (e / TM) % KC, t = e / (TM * KC);
-                if (k0 + k < Kp && i0 + i < a.nlat2) v = (t == 0 ? tab0 : tab1)[(size_t)(kbase + k0 + k) * a.nlat2 + i0 + i];
-            }
+            VA v;
+            T *pv = reinterpret_cast<T *>(&v);
+#pragma unroll
+            for (int u = 0; u < EA; ++u) pv[u] = T(0);
+            if (va[q] && k0 + ka[q] < Kp)
+                v = *reinterpret_cast<const VA *>(pa[q] + (size_t)(kb0 + kbase + k0 + ka[q]) * a.nlat2);
             ra[q] = v;
         }
 #pragma unroll
-        for (int q = 0; q < RB; ++q) {
-            const int e = tid + q * nthr;
-            const int k = e % KC, vv = e / KC;
-            const int lv = vv / VC, fv = vv - lv * VC;
-            const int col = scol[fv], r = r0 + lv;
-            T v = T(0);
-            if (col >= 0 && r < a.nr && k0 + k < Kp) v = in[(size_t)(r * a.cin + col) * a.nlm + kbase + k0 + k];
-            rb[q] = v;
+        for (int s = 0; s < RB; ++s) {
+            // degree of step s: group h = s / 4 of 16, lane group g, step s % 4
+            const int k = (s >> 2) * 16 + 4 * g + (s & 3);
+            rb[s] = (vb && k0 + k < Kp) ? pb[kbase + k0 + k] : T(0);
         }
     };
-    auto sstore = [&](int b) {
-        T *A = sA + b * sAsz, *B = sB + b * sBsz;
+    auto sstore = [&](int buf) {
+        T *A = sA + buf * sAsz;
 #pragma unroll
-        for (int q = 0; q < RA; ++q) {
-            const int e = tid + q * nthr;
-            if (e < sAsz) A[e] = ra[q];
-        }
-#pragma unroll
-        for (int q = 0; q < RB; ++q) {
-            const int e = tid + q * nthr;
-            const int k = e % KC, vv = e / KC;
-            B[k * NTILE + vv] = rb[q];
-        }
+        for (int q = 0; q < RA; ++q)
+            if (tid + q * nthr < NAE) *reinterpret_cast<VA *>(A + sa[q]) = ra[q];
     };
-    auto mma = [&](int b, acc_t<T>(&acc)[NT][2]) {
-        const T *A = sA + b * sAsz, *B = sB + b * sBsz;
+    auto mma = [&](int buf, const T (&bv)[RB], acc_t<T>(&acc)[NT][2]) {
+        const T *A = sA + buf * sAsz;
 #pragma unroll
-        for (int ks = 0; ks < KC; ks += 4) {
-            const T bv = B[(ks + (lane >> 4)) * NTILE + wave * 16 + (lane & 15)];
+        for (int s = 0; s < RB; ++s) {
+            const int k = (s >> 2) * 16 + 4 * g + (s & 3);
 #pragma unroll
             for (int t = 0; t < NT; ++t)
 #pragma unroll
                 for (int rb_ = 0; rb_ < 2; ++rb_) {
-                    const T av = A[(t * KC + ks + (lane >> 4)) * TM + rb_ * 16 + (lane & 15)];
-                    acc[t][rb_] = mfma16x16x4(av, bv, acc[t][rb_]);
+                    const T av = A[(t * KC + k) * TMA + rb_ * 16 + c16];
+                    acc[t][rb_] = mfma16x16x4(av, bv[s], acc[t][rb_]);
                 }
         }
     };
     if (nC > 0) gload(0);
     for (int c = 0; c < nC; ++c) {
         sstore(c & 1);
+        T bcur[RB];
+#pragma unroll
+        for (int s = 0; s < RB; ++s) bcur[s] = rb[s];
         __syncthreads();
         if (c + 1 < nC) gload(c + 1);
-        if (c < nE) mma(c & 1, accE);
-        else mma(c & 1, accO);
+        if (c < nE) mma(c & 1, bcur, accE);
+        else mma(c & 1, bcur, accO);
     }
 }
 
 template <typename T, bool VEC, int NF, int NW, int KC> constexpr size_t synth_lds_bytes()
 {
     using TL = SynthTile<VEC, NF, NW>;
-    const size_t pipe = 2 * (TL::NT * KC * TM + KC * TL::NTILE), tile = (size_t)TM * TL::LT * TL::RS;
+    const size_t pipe = 2 * (size_t)TL::NT * KC * TMA, tile = (size_t)TM * TL::LT * TL::RS;
     return 128 + sizeof(T) * (pipe > tile ? pipe : tile);
 }
 
 // The block's output tile -- TM nodes x LT levels, RS contiguous slots of each
 // (node, level, m) record -- is staged in LDS and stored as 16-byte vectors,
 // one record run after another.
-template <typename T, bool VEC, int NF, int NW, int KC>
+template <typename T, bool VEC, int NF, int NW, int KC, bool VL>
 __global__ void __launch_bounds__(64 * NW) synth_kernel(SynthArgs a)
 {
     using TL = SynthTile<VEC, NF, NW>;
@@ -224,7 +258,6 @@ __global__ void __launch_bounds__(64 * NW) synth_kernel(SynthArgs a)
     extern __shared__ __align__(16) char smem_raw[];
     int *scol = reinterpret_cast<int *>(smem_raw);
     T *sA = reinterpret_cast<T *>(smem_raw + 128);
-    T *sB = sA + 2 * NT * KC * TM;
     const int gx = (a.nlat2 + TM - 1) / TM, gy = (a.nr + LT - 1) / LT, nm = a.mmax + 1;
     const int L = xcd_logical(blockIdx.x, gx * gy * nm);
     const int bx = L % gx, by = (L / gx) % gy;
@@ -243,7 +276,7 @@ __global__ void __launch_bounds__(64 * NW) synth_kernel(SynthArgs a)
             accE[t][rb] = acc_t<T>{0, 0, 0, 0};
             accO[t][rb] = acc_t<T>{0, 0, 0, 0};
         }
-    synth_pipeline<T, VEC, NF, NW, KC>(a, in, tab0, tab1, sA, sB, scol, m, i0, r0, accE, accO);
+    synth_pipeline<T, VEC, NF, NW, KC, VL>(a, in, tab0, tab1, sA, scol, m, i0, r0, accE, accO);
     __syncthreads(); // the tile below reuses the pipeline's LDS
 
     T *tile = sA; // [node][level][slot]
@@ -671,7 +704,11 @@ template <typename T, bool VEC, int NF, int NW, int KC> static void launch_synth
     using TL = SynthTile<VEC, NF, NW>;
     dim3 grid(((a.nlat2 + TM - 1) / TM) * ((a.nr + TL::LT - 1) / TL::LT) * (a.mmax + 1));
     constexpr size_t lds = synth_lds_bytes<T, VEC, NF, NW, KC>();
-    hipLaunchKernelGGL((synth_kernel<T, VEC, NF, NW, KC>), grid, dim3(64 * NW), lds, st, a);
+    // 16-byte table loads need every node row and tile start 16-byte aligned
+    if (a.nlat2 % 4 == 0 && ((uintptr_t)a.tab0 % 16) == 0 && ((uintptr_t)a.tab1 % 16) == 0)
+        hipLaunchKernelGGL((synth_kernel<T, VEC, NF, NW, KC, true>), grid, dim3(64 * NW), lds, st, a);
+    else
+        hipLaunchKernelGGL((synth_kernel<T, VEC, NF, NW, KC, false>), grid, dim3(64 * NW), lds, st, a);
 }
 
 // The staged tile is stored in 16-byte vectors: record stride and first slot
@@ -688,7 +725,7 @@ template <typename T> int gd_launch_synth_vec(hipStream_t st, const SynthArgs &a
     GD_TRY(synth_check(a, "synth_vec"));
     GD_CHECK(a.slot0 + 8 * a.nf <= a.nslot, GD_ERR_ARG, "synth_vec: slots %d+%d exceed %d", a.slot0, 8 * a.nf, a.nslot);
     if (a.nf == 5) launch_synth<T, true, 5, 5, 16>(st, a);
-    else if (a.nf == 2) launch_synth<T, true, 2, 2, 16>(st, a);
+    else if (a.nf == 2) launch_synth<T, true, 2, 4, 16>(st, a);
     else if (a.nf == 1) launch_synth<T, true, 1, 1, 16>(st, a);
     else GD_CHECK(false, GD_ERR_ARG, "synth_vec: nf=%d (1, 2 or 5)", a.nf);
     return gd_check_launch("synth_kernel<vec>");
