@@ -516,13 +516,14 @@ __global__ void __launch_bounds__(64 * ANA_WAVES) ana_res_kernel(AnaArgs a)
     constexpr int VC = VEC ? 4 * NF : 2 * NF;
     constexpr int LT = 16 / VC; // levels per wave tile (16 columns)
     constexpr int KS = KN / 4;
-    constexpr int TMP = TM + 1;
+    constexpr int TMP = TM + 1; // output staging row stride
     constexpr bool STAGE = sizeof(T) == 4;
     constexpr int NA = NT * TM * KN / (64 * ANA_WAVES);
     static_assert(NT * TM * KN % (64 * ANA_WAVES) == 0, "table tile must split evenly over the block");
+    static_assert(KN % 16 == 0, "nodes come in 16-node MFMA groups");
     extern __shared__ __align__(16) char smem_raw[];
-    T *sA = reinterpret_cast<T *>(smem_raw);    // [t][node][TMP]
-    T *stg = sA + NT * KN * TMP;                 // [wave][16][TMP] when STAGE
+    T *sA = reinterpret_cast<T *>(smem_raw);    // [t][node][TMA]
+    T *stg = sA + NT * KN * TMA;                 // [wave][16][TMP] when STAGE
     const int gx = (a.maxgroup + TM - 1) / TM, nmz = a.mmax + 1;
     const int L = xcd_logical(blockIdx.x, gx * 2 * nmz);
     const int bx = L % gx, bz = L / gx;
@@ -553,7 +554,7 @@ __global__ void __launch_bounds__(64 * ANA_WAVES) ana_res_kernel(AnaArgs a)
             const int e = tid + q * 64 * ANA_WAVES;
             const int t = e / (TM * KN), rem = e - t * (TM * KN);
             const int i = rem / KN, k = rem - i * KN;
-            sA[(t * KN + k) * TMP + i] = ra[q];
+            sA[(t * KN + k) * TMA + i] = ra[q];
         }
     }
 
@@ -579,11 +580,15 @@ __global__ void __launch_bounds__(64 * ANA_WAVES) ana_res_kernel(AnaArgs a)
     }
     const int ntiles = (a.nr + LT - 1) / LT;
     const int niter = (ntiles + ANA_WAVES - 1) / ANA_WAVES;
+    // node of MFMA step ks for this lane: within each group of 16 nodes, lane
+    // group kq takes nodes 4kq .. 4kq+3 over 4 steps (A is indexed the same way)
+    auto node = [&](int ks) { return (ks >> 2) * 16 + 4 * kq + (ks & 3); };
+    const size_t nstride = (size_t)a.nr * a.nslot_g; // one node further in GB
+    const T *gm = gb + (size_t)m * a.nlat2 * nstride;
     auto gload = [&](int tile, int t, int ks) {
-        const int r = tile * LT + lv, k = 4 * ks + kq;
+        const int r = tile * LT + lv, k = node(ks);
         T v = T(0);
-        if (tile < ntiles && r < a.nr && k < a.nlat2)
-            v = gb[(((size_t)m * a.nlat2 + k) * a.nr + r) * a.nslot_g + slot[t]];
+        if (tile < ntiles && r < a.nr && k < a.nlat2) v = gm[(size_t)k * nstride + (size_t)r * a.nslot_g + slot[t]];
         return v;
     };
     T b[NT][KS];
@@ -604,7 +609,7 @@ __global__ void __launch_bounds__(64 * ANA_WAVES) ana_res_kernel(AnaArgs a)
         for (int ks = 0; ks < KS; ++ks) {
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
-                const T *Ak = sA + (t * KN + 4 * ks + kq) * TMP;
+                const T *Ak = sA + (t * KN + node(ks)) * TMA;
                 acc[t][0] = mfma16x16x4(Ak[c], b[t][ks], acc[t][0]);
                 acc[t][1] = mfma16x16x4(Ak[16 + c], b[t][ks], acc[t][1]);
                 b[t][ks] = gload(next, t, ks);
@@ -694,7 +699,7 @@ __global__ void __launch_bounds__(64 * ANA_WAVES) ana_res_kernel(AnaArgs a)
 template <typename T, bool VEC, int NF, int KN> constexpr size_t ana_res_lds()
 {
     constexpr int NT = VEC ? 2 : 1;
-    return sizeof(T) * ((size_t)NT * KN * (TM + 1) + (sizeof(T) == 4 ? (size_t)ANA_WAVES * 16 * (TM + 1) : 0));
+    return sizeof(T) * ((size_t)NT * KN * TMA + (sizeof(T) == 4 ? (size_t)ANA_WAVES * 16 * (TM + 1) : 0));
 }
 
 } // namespace
