@@ -969,7 +969,7 @@ constexpr int RING_THREADS = 256; // ring_kernel block size
 
 // NFFT > 0: the transform count is a compile-time constant (nfft is ignored)
 template <int R, int NS, int N, int DIR, typename T, int NFFT>
-__device__ __noinline__ void fft_stage_ct(C2<T> *buf, const C2<T> *tw, int nfft)
+__device__ __forceinline__ void fft_stage_ct(C2<T> *buf, const C2<T> *tw, int nfft)
 {
     constexpr int nb = N / R, twstep = N / (NS * R), LD = ring_ld(N);
     constexpr int nthr = RING_THREADS;
