@@ -89,7 +89,7 @@ def kernel_costs(prec_bytes, tangential=True):
 # kernel of each class on the headline (tangential) path, as rocprofv3 names it
 TANG_KERNELS = {"spectral_pre": "spectral_pre_kernel<{T}>", "synth_vector": "synth_kernel<{T}, true, 2, 4, 16, true>",
                 "ring_fft_products": "ring_kernel<{T}, 3, 384>", "analysis_vector": "ana_res_kernel<{T}, true, 1, 96>",
-                "analysis_scalar": "ana_res_kernel<{T}, false, 1, 96>", "cnab2_solve": "cnab2_chain_kernel<3, 4, {T}>"}
+                "analysis_scalar": "ana_res_kernel<{T}, false, 1, 96>", "cnab2_solve": "cnab2_chain_kernel<3, 4, {T}, true>"}
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "pmc_step_lmax127_nr65.json")
 COPY_PROBE_BYTES = 268435456
 

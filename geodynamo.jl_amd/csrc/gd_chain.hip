@@ -21,20 +21,31 @@
 // curl and radial-component factors): when x[i] is known, row i + BW has its
 // whole band window x[i..i+2BW], so the state is not read again.
 namespace {
-template <int BW, int D, typename TS> __global__ void __launch_bounds__(64) cnab2_chain_kernel(ChainArgs c)
+template <int BW, int D, typename TS, bool DIRECT> __global__ void __launch_bounds__(64) cnab2_chain_kernel(ChainArgs c)
 {
     constexpr int W = 2 * BW + 1;
     const int n = c.n, nlm = c.nlm;
     const int idx = blockIdx.x * 64 + threadIdx.x;
     const int f = blockIdx.y, ri = blockIdx.z; // field, 0 re / 1 im
     const int k = f < 2 ? 0 : (f < 4 ? 1 : 2);
-    // level-uniform tables in LDS: nu*Laplacian band, D1 and D2 bands, 1/r, 1/r^2
+    // level-uniform tables in LDS: the explicit operator by row (ER doubles per
+    // row i: the coefficient of u row i-BW+q in slot q, 0 where that row does
+    // not exist, and 1/r_i^2 in slot W), D1 and D2 bands, 1/r, 1/r^2
+    constexpr int ER = (W + 2) & ~1;
     extern __shared__ double smem[];
-    double *Lap = smem, *sD1 = Lap + W * n, *sD2 = sD1 + W * n, *rinv_ = sD2 + W * n, *rinv2_ = rinv_ + n;
+    double *Ex = smem, *sD1 = Ex + ER * n, *sD2 = sD1 + W * n, *rinv_ = sD2 + W * n, *rinv2_ = rinv_ + n;
+    const double *Lapk = c.nuLap + (size_t)k * W * n;
     for (int t = threadIdx.x; t < W * n; t += 64) {
-        Lap[t] = c.nuLap[(size_t)k * W * n + t];
         sD1[t] = c.D1[t];
         sD2[t] = c.D2[t];
+    }
+    for (int t = threadIdx.x; t < ER * n; t += 64) {
+        const int i = t / ER, q = t - i * ER, j = i - BW + q;
+        double v = 0.0;
+        if (q == BW) v = Lapk[BW + W * i];
+        else if (q < W && j >= 0 && j < n) v = Lapk[(2 * BW - q) + W * j];
+        else if (q == W) v = c.rtab[i + n];
+        Ex[t] = v;
     }
     for (int t = threadIdx.x; t < n; t += 64) {
         rinv_[t] = c.rtab[t + 2 * n];
@@ -93,13 +104,10 @@ template <int BW, int D, typename TS> __global__ void __launch_bounds__(64) cnab
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             const int i = i0 + d, ic = cl(i);
+            const double *er = Ex + ER * ic;
             double lr = 0.0;
 #pragma unroll
-            for (int q = 0; q < W; ++q) {
-                const int j = i + q - BW;
-                const double cf = (q == BW) ? Lap[BW + W * ic] - nulf * rinv2_[ic] : Lap[(2 * BW - q) + W * cl(j)];
-                lr += ((j >= 0 && j < n) ? cf : 0.0) * w[q];
-            }
+            for (int q = 0; q < W; ++q) lr += ((q == BW) ? er[BW] - nulf * er[W] : er[q]) * w[q];
             double v = inv_dt * w[BW] + 1.5 * sn[d] - 0.5 * sp[d];
             v += lw * lr;
             v = (bcr && (i == 0 || i == n - 1)) ? 0.0 : v; // Dirichlet value
@@ -146,7 +154,11 @@ template <int BW, int D, typename TS> __global__ void __launch_bounds__(64) cnab
     for (int q = 0; q < W; ++q) xw[q] = 0.0;
 #pragma unroll
     for (int d = 0; d < D; ++d) bload(d, n - 1 - d);
-    const int iend = syn ? -BW : 0; // the synthesis rows trail the solve by BW
+    // the tangential step needs no radial derivatives: its columns (u itself,
+    // T/r) are written as x is produced; otherwise they trail the solve by BW
+    // DIRECT (compile-time: the launcher sets it exactly when synV is set and
+    // syn_tang is 1) keeps the branch out of the software-pipelined loop
+    const int iend = (syn && !DIRECT) ? -BW : 0;
     for (int i0 = n - 1; i0 >= iend; i0 -= D) {
 #pragma unroll
         for (int d = 0; d < D; ++d) {
@@ -160,6 +172,9 @@ template <int BW, int D, typename TS> __global__ void __launch_bounds__(64) cnab
             for (int q = W - 1; q > 0; --q) xw[q] = xw[q - 1];
             xw[0] = v;
             if (i >= 0) o[(size_t)nlm * i] = v;
+            if constexpr (DIRECT) {
+                if (wA && i >= 0) pA[(size_t)i * 18 * nlm] = (TS)(f == 4 ? v * rinv_[i] : v);
+            } else {
             const int r = i + BW; // xw[q] = x[r - BW + q]
             if (syn && wA && r >= 0 && r < n) {
                 const double rinv = rinv_[r], rinv2 = rinv2_[r];
@@ -200,6 +215,7 @@ template <int BW, int D, typename TS> __global__ void __launch_bounds__(64) cnab
                 if (wB) pB[ov] = (TS)bv;
                 if (wS) pS[os] = (TS)sv;
             }
+            }
             bload(d, i - D);
         }
     }
@@ -208,7 +224,8 @@ template <int BW, int D, typename TS> __global__ void __launch_bounds__(64) cnab
 
 int gd_cnab2_chain_supported(int n, int bw)
 {
-    return bw >= 1 && bw <= 7 && n >= 2 && sizeof(double) * (size_t)(3 * (2 * bw + 1) + 2) * n <= 64 * 1024;
+    return bw >= 1 && bw <= 7 && n >= 2 &&
+           sizeof(double) * (size_t)((((2 * bw + 1) + 2) & ~1) + 2 * (2 * bw + 1) + 2) * n <= 64 * 1024;
 }
 
 int gd_launch_cnab2_chain(hipStream_t st, const ChainArgs &c, bool syn_fp64)
@@ -218,12 +235,15 @@ int gd_launch_cnab2_chain(hipStream_t st, const ChainArgs &c, bool syn_fp64)
     GD_CHECK(!c.synV || (c.synS && c.D1 && c.D2 && c.rtab), GD_ERR_ARG, "cnab2_chain: incomplete synthesis outputs");
     GD_CHECK(gd_cnab2_chain_supported(c.n, c.bw), GD_ERR_ARG, "cnab2_chain: n=%d bw=%d unsupported", c.n, c.bw);
     dim3 grid((c.nlm + 63) / 64, 5, 2);
-    const size_t lds = sizeof(double) * (size_t)(3 * (2 * c.bw + 1) + 2) * c.n;
+    const size_t lds = sizeof(double) * (size_t)((((2 * c.bw + 1) + 2) & ~1) + 2 * (2 * c.bw + 1) + 2) * c.n;
     GD_CHECK(lds <= 64 * 1024, GD_ERR_ARG, "cnab2_chain: n=%d bw=%d needs %zu B of LDS", c.n, c.bw, lds);
+    const bool direct = c.synV && c.syn_tang;
 #define GD_CHAIN(B)                                                                                              \
     if (c.bw == B) {                                                                                             \
-        if (syn_fp64) hipLaunchKernelGGL((cnab2_chain_kernel<B, 4, double>), grid, dim3(64), lds, st, c);        \
-        else hipLaunchKernelGGL((cnab2_chain_kernel<B, 4, float>), grid, dim3(64), lds, st, c);                  \
+        if (syn_fp64 && direct) hipLaunchKernelGGL((cnab2_chain_kernel<B, 4, double, true>), grid, dim3(64), lds, st, c); \
+        else if (syn_fp64) hipLaunchKernelGGL((cnab2_chain_kernel<B, 4, double, false>), grid, dim3(64), lds, st, c); \
+        else if (direct) hipLaunchKernelGGL((cnab2_chain_kernel<B, 4, float, true>), grid, dim3(64), lds, st, c); \
+        else hipLaunchKernelGGL((cnab2_chain_kernel<B, 4, float, false>), grid, dim3(64), lds, st, c);           \
         return gd_check_launch("cnab2_chain_kernel");                                                            \
     }
     GD_CHAIN(1) GD_CHAIN(2) GD_CHAIN(3) GD_CHAIN(4) GD_CHAIN(5) GD_CHAIN(6) GD_CHAIN(7)
