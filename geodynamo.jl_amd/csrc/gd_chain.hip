@@ -57,6 +57,7 @@ template <int BW, int D, typename TS, bool DIRECT> __global__ void __launch_boun
     const double lf = (double)l * (double)(l + 1);
     const double nulf = c.nus[k] * lf;
     const double *LU = c.LUt + (size_t)k * W * n * c.nl + l;
+    const double *rdg = c.rdiag + (size_t)k * n * c.nl + l;
     const size_t fs = (size_t)nlm * n, off = (size_t)(2 * f + ri) * fs + idx;
     const double *u = c.st_in + off;
     // a field whose nonlinear terms are known to be zero reads one zero row (L2-resident)
@@ -141,7 +142,7 @@ template <int BW, int D, typename TS, bool DIRECT> __global__ void __launch_boun
     auto bload = [&](int s, int i) {
         const int ic = cl(i);
         by[s] = o[(size_t)nlm * ic];
-        bd[s] = lu(ic, ic);
+        bd[s] = rdg[(size_t)ic * c.nl];
 #pragma unroll
         for (int q = 0; q < BW; ++q) {
             const int j = i + 1 + q;
@@ -166,7 +167,7 @@ template <int BW, int D, typename TS, bool DIRECT> __global__ void __launch_boun
             double v = by[d];
 #pragma unroll
             for (int q = 0; q < BW; ++q) v -= bu[d][q] * xw[q];
-            v /= bd[d];
+            v *= bd[d]; // reciprocal of the U diagonal
             v = i >= 0 ? v : 0.0;
 #pragma unroll
             for (int q = W - 1; q > 0; --q) xw[q] = xw[q - 1];
@@ -222,6 +223,21 @@ template <int BW, int D, typename TS, bool DIRECT> __global__ void __launch_boun
 }
 } // namespace
 
+__global__ void recip_diag_kernel(const double *__restrict__ LUt, int nk, int n, int bw, int nl, double *__restrict__ rd)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nk * n * nl) return;
+    const int l = e % nl, i = (e / nl) % n, k = e / (nl * n), W = 2 * bw + 1;
+    rd[e] = 1.0 / LUt[((size_t)k * W * n + (size_t)i * W + bw) * nl + l];
+}
+
+int gd_launch_recip_diag(hipStream_t st, const double *LUt, int nk, int n, int bw, int nl, double *rdiag)
+{
+    const int total = nk * n * nl;
+    hipLaunchKernelGGL(recip_diag_kernel, dim3((total + 255) / 256), dim3(256), 0, st, LUt, nk, n, bw, nl, rdiag);
+    return gd_check_launch("recip_diag_kernel");
+}
+
 int gd_cnab2_chain_supported(int n, int bw)
 {
     return bw >= 1 && bw <= 7 && n >= 2 &&
@@ -231,6 +247,7 @@ int gd_cnab2_chain_supported(int n, int bw)
 int gd_launch_cnab2_chain(hipStream_t st, const ChainArgs &c, bool syn_fp64)
 {
     GD_CHECK(c.st_in != c.st_out, GD_ERR_ARG, "cnab2_chain: in and out state must differ");
+    GD_CHECK(c.rdiag, GD_ERR_ARG, "cnab2_chain: reciprocal diagonal table missing");
     GD_CHECK(!c.nl_zero_mask || c.zero_row, GD_ERR_ARG, "cnab2_chain: nl_zero_mask needs zero_row");
     GD_CHECK(!c.synV || (c.synS && c.D1 && c.D2 && c.rtab), GD_ERR_ARG, "cnab2_chain: incomplete synthesis outputs");
     GD_CHECK(gd_cnab2_chain_supported(c.n, c.bw), GD_ERR_ARG, "cnab2_chain: n=%d bw=%d unsupported", c.n, c.bw);

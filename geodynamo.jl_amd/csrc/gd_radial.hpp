@@ -32,11 +32,14 @@ struct ChainArgs {
     int nl_zero_mask;          // bit f: nl and nl_prev of field f are identically zero
     const double *zero_row;    // nlm zeros, read in their place
     const double *D1, *D2, *rtab;
+    const double *rdiag;       // [nu][row][l] = 1 / U_l[row][row] (gd_launch_recip_diag)
     int radials;
     int syn_tang; // only the u and T/r synthesis columns are needed (the tangential step)
     void *synV, *synS;
 };
 int gd_cnab2_chain_supported(int n, int bw);
+// rdiag[(k*n + i)*nl + l] = 1 / LU_t[k][(i*W + bw)][l] for nk diffusivities
+int gd_launch_recip_diag(hipStream_t st, const double *LUt, int nk, int n, int bw, int nl, double *rdiag);
 int gd_launch_cnab2_chain(hipStream_t st, const ChainArgs &c, bool syn_fp64);
 int gd_launch_energies(hipStream_t st, const double *state, const int *l_of_idx, int nlm, int n, double *part,
                        int nblk, double *out3);

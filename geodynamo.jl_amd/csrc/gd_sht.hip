@@ -524,6 +524,7 @@ __global__ void __launch_bounds__(64 * ANA_WAVES) ana_res_kernel(AnaArgs a)
     extern __shared__ __align__(16) char smem_raw[];
     T *sA = reinterpret_cast<T *>(smem_raw);    // [t][node][TMA]
     T *stg = sA + NT * KN * TMA;                 // [wave][16][TMP] when STAGE
+    __shared__ double sinv[TM];                  // 1 / (l(l+1)) of the tile's rows (0 at l = 0)
     const int gx = (a.maxgroup + TM - 1) / TM, nmz = a.mmax + 1;
     const int L = xcd_logical(blockIdx.x, gx * 2 * nmz);
     const int bx = L % gx, bz = L / gx;
@@ -537,6 +538,10 @@ __global__ void __launch_bounds__(64 * ANA_WAVES) ana_res_kernel(AnaArgs a)
     const T *gb = static_cast<const T *>(a.gb);
     const T *tab0 = static_cast<const T *>(a.tab0), *tab1 = static_cast<const T *>(a.tab1);
 
+    if (tid < TM) {
+        const int l = row0 + tid < Kp ? a.l_of_idx[kbase + row0 + tid] : 0;
+        sinv[tid] = l > 0 ? 1.0 / ((double)l * (double)(l + 1)) : 0.0;
+    }
     // table tile: all loads in flight together, then into LDS
     {
         T ra[NA];
@@ -649,9 +654,8 @@ __global__ void __launch_bounds__(64 * ANA_WAVES) ana_res_kernel(AnaArgs a)
                         }
                         double val = (double)o;
                         const int idx = kbase + grow;
-                        const int l = a.l_of_idx[idx];
                         const int sc = a.scale[outk];
-                        if (sc == 0) val = (l > 0) ? val / ((double)l * (double)(l + 1)) : 0.0;
+                        if (sc == 0) val *= sinv[row];
                         else if (sc == 2) val = -val;
                         if (m == 0 && reim) val = 0.0; // imag(m=0) := 0 (shtnskit_field_functions.jl:432-435)
                         a.out[outk][(size_t)reim * fs + (size_t)r * a.nlm + idx] = val;
@@ -682,9 +686,8 @@ __global__ void __launch_bounds__(64 * ANA_WAVES) ana_res_kernel(AnaArgs a)
                     }
                     double val = (double)wst[col * TMP + row];
                     const int idx = kbase + grow;
-                    const int l = a.l_of_idx[idx];
                     const int sc = a.scale[outk];
-                    if (sc == 0) val = (l > 0) ? val / ((double)l * (double)(l + 1)) : 0.0;
+                    if (sc == 0) val *= sinv[row];
                     else if (sc == 2) val = -val;
                     if (m == 0 && reim) val = 0.0; // imag(m=0) := 0 (shtnskit_field_functions.jl:432-435)
                     a.out[outk][(size_t)reim * fs + (size_t)r2 * a.nlm + idx] = val;

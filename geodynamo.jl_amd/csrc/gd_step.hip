@@ -40,6 +40,7 @@ struct gd_step {
     DevBuf part, eout, scratch;
     DevBuf status;
     DevBuf zrow;                   // nlm zeros
+    DevBuf rdiag;                  // [nu][row][l] reciprocal U diagonals
     int step = 0;
     bool chain = false;  // the column-register CNAB2 kernel applies (it also refreshes synV/synS)
     bool syn_ok = false; // synV/synS hold the synthesis columns of the current state
@@ -309,6 +310,7 @@ static int enqueue_step(gd_step *s, int parity, bool boot, hipEvent_t *ev = null
         // the magnetic nonlinear terms vanish identically in the tangential step
         c.nl_zero_mask = tangential(s) ? 0x0c : 0;
         c.zero_row = s->zrow.as<double>();
+        c.rdiag = s->rdiag.as<double>();
         c.D1 = s->D1.as<double>();
         c.D2 = s->D2.as<double>();
         c.rtab = s->rtab.as<double>();
@@ -344,7 +346,7 @@ static int step_release(gd_step *s)
     destroy_graphs(s);
     DevBuf *bufs[] = {&s->state[0], &s->state[1], &s->nlbuf[0], &s->nlbuf[1], &s->Lt, &s->LUt, &s->nuLap, &s->nus,
                       &s->D1, &s->D2, &s->rtab, &s->synV,
-                      &s->synS, &s->fb, &s->gb, &s->part, &s->eout, &s->scratch, &s->status, &s->zrow};
+                      &s->synS, &s->fb, &s->gb, &s->part, &s->eout, &s->scratch, &s->status, &s->zrow, &s->rdiag};
     for (DevBuf *b : bufs) b->release();
     gd_sht_plan_release(&s->sht);
     return GD_OK;
@@ -429,6 +431,8 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
     GD_TRY(s->part.ensure(sizeof(double) * 3 * 512));
     GD_TRY(s->eout.ensure(sizeof(double) * 4));
     GD_TRY(s->zrow.ensure(sizeof(double) * s->nlm));
+    GD_TRY(s->rdiag.ensure(sizeof(double) * 3 * nr * s->nl));
+    GD_TRY(gd_launch_recip_diag(ctx->stream, s->LUt.as<double>(), 3, nr, bw, s->nl, s->rdiag.as<double>()));
     GD_HIP(hipMemsetAsync(s->zrow.p, 0, sizeof(double) * s->nlm, ctx->stream));
     GD_HIP(hipMemsetAsync(s->state[0].p, 0, sizeof(double) * s->fs * 10, ctx->stream));
     GD_HIP(hipMemsetAsync(s->state[1].p, 0, sizeof(double) * s->fs * 10, ctx->stream));
