@@ -17,6 +17,8 @@ from .shtnskit_field_functions import (batch_shtnskit_transforms, batch_spectral
                                        shtnskit_vector_analysis, shtnskit_vector_synthesis)
 from .shtnskit_transforms import SHTnsKitConfig, create_shtnskit_config  # noqa: F401
 from .step import CNAB2Step  # noqa: F401
-from .timestep import (SHTnsImplicitMatrices, apply_explicit_operator, build_rhs_cnab2,  # noqa: F401
-                       compute_timestep_error, create_shtns_timestepping_matrices,
+from .timestep import (ERK2Cache, ERK2FieldBuffers, ETDCache, SHTnsImplicitMatrices,  # noqa: F401
+                       apply_explicit_operator, build_rhs_cnab2, compute_timestep_error, create_erk2_cache,
+                       create_etd_cache, create_shtns_timestepping_matrices, eab2_update, erk2_apply_stage,
+                       erk2_finalize_field, erk2_prepare_field, erk2_store_stage_nonlinear, exp_functions,
                        solve_implicit_step)

@@ -14,6 +14,7 @@ GD_OK, GD_ERR_ARG, GD_ERR_HIP, GD_ERR_NUMERIC, GD_ERR_MISSING_L, GD_ERR_NOMEM = 
 GD_PREC_FP64, GD_PREC_FP32, GD_PREC_BF16X3 = 0, 1, 2
 GD_HOST = 1
 GD_FD_LITERAL, GD_FD_TRANSPOSE = 0, 1
+GD_EXP_ETD, GD_EXP_ERK2 = 0, 1
 
 # every symbol include/gdhip.h declares
 EXPORTS = [
@@ -27,6 +28,10 @@ EXPORTS = [
     "gd_step_nonlinear", "gd_step_run", "gd_step_run_graph", "gd_step_count", "gd_step_energies",
     "gd_step_profile",
     "gd_timestep_error",
+    "gd_exp_functions", "gd_etd_cache_create", "gd_etd_cache_get", "gd_etd_cache_destroy", "gd_eab2_update",
+    "gd_erk2_cache_create", "gd_erk2_cache_from_matrices", "gd_erk2_cache_get", "gd_erk2_cache_destroy",
+    "gd_erk2_buffers_create", "gd_erk2_buffers_destroy", "gd_erk2_prepare_field", "gd_erk2_apply_stage",
+    "gd_erk2_store_stage_nonlinear", "gd_erk2_finalize_field",
 ]
 
 
@@ -103,6 +108,21 @@ def lib():
             "gd_step_energies": (i, [vp, vp]),
             "gd_step_profile": (i, [vp, i, vp]),
             "gd_timestep_error": (i, [vp, l, vp, vp, vp, vp, vp, i]),
+            "gd_exp_functions": (i, [i, i, i, vp, d, d, i, i, vp]),
+            "gd_etd_cache_create": (i, [vp, i, i, i, vp, d, d, i, pp]),
+            "gd_etd_cache_get": (i, [vp, vp, vp]),
+            "gd_etd_cache_destroy": (i, [vp]),
+            "gd_eab2_update": (i, [vp, i, i, i] + [vp] * 6 + [d, i]),
+            "gd_erk2_cache_create": (i, [vp, i, i, i, vp, d, d, i, pp]),
+            "gd_erk2_cache_from_matrices": (i, [vp, i, vp, i, d] + [vp] * 5 + [pp]),
+            "gd_erk2_cache_get": (i, [vp] * 6),
+            "gd_erk2_cache_destroy": (i, [vp]),
+            "gd_erk2_buffers_create": (i, [vp, i, i, i, pp]),
+            "gd_erk2_buffers_destroy": (i, [vp]),
+            "gd_erk2_prepare_field": (i, [vp] * 5 + [d, i]),
+            "gd_erk2_apply_stage": (i, [vp, vp, vp, i]),
+            "gd_erk2_store_stage_nonlinear": (i, [vp, vp, vp, i]),
+            "gd_erk2_finalize_field": (i, [vp, vp, vp, d, i]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

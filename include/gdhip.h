@@ -195,6 +195,65 @@ int gd_step_profile(gd_step *s, int nsteps, double *out);
 int gd_timestep_error(gd_ctx *ctx, long n, const double *a_re, const double *a_im,
                       const double *b_re, const double *b_im, double *out, int flags);
 
+/* ---- exponential integrators: EAB2 and staged ERK2 (FP64) -----------------
+ * A_l = nu (d2/dr2 + (2/r) d/dr - l(l+1)/r^2) for l = 0..lmax, dense nr x nr
+ * (no boundary rows), from the band Laplacian of gd_radial_laplacian.  Matrix
+ * functions are computed on the host (exp by [13/13] Pade scaling and
+ * squaring) and kept on the device; every matrix is Julia's Matrix layout,
+ * nr x nr column-major, one per l in l order.  Field arguments are l-major
+ * spectral re/im planes (nlm, nr) of a field with the given (lmax, mmax). */
+typedef struct gd_etd_cache gd_etd_cache;
+typedef struct gd_erk2_cache gd_erk2_cache;
+typedef struct gd_erk2_buffers gd_erk2_buffers;
+enum { GD_EXP_ETD = 0, GD_EXP_ERK2 = 1 };
+
+/* Host only (no device, no context): the matrices the caches below hold.
+ * GD_EXP_ETD: E = exp(dt A_l), phi1 = A_l^-1 (E - I)/dt      (timestep.jl:37-65)
+ * GD_EXP_ERK2: exp(dt/2 A_l), exp(dt A_l), phi1(dt/2 A_l), phi1(dt A_l),
+ *   phi2(dt A_l) with the reference's series fallbacks      (timestep.jl:896-1187)
+ * out: 2 (ETD) or 5 (ERK2) blocks of (lmax+1) matrices.  GD_ERR_NUMERIC when
+ * an exponential fails or (ETD) A_l is exactly singular, where the reference's
+ * exp/lu throw. */
+int gd_exp_functions(int lmax, int nr, int bw, const double *r, double nu, double dt, int fd_mode,
+                     int which, double *out);
+
+/* replaces create_etd_cache (timestep.jl:37-65) */
+int gd_etd_cache_create(gd_ctx *ctx, int lmax, int nr, int bw, const double *r, double nu, double dt,
+                        int fd_mode, gd_etd_cache **out);
+int gd_etd_cache_get(gd_etd_cache *c, double *E, double *phi1 /* host, (lmax+1) matrices each */);
+int gd_etd_cache_destroy(gd_etd_cache *c);
+/* replaces eab2_update! (timestep.jl:425-487): u = E_l u + dt phi1_l (3/2 nl - 1/2 nl_prev)
+ * in place per (lm, re/im); GD_ERR_MISSING_L if the field has an l the cache lacks. */
+int gd_eab2_update(gd_etd_cache *c, int lmax, int mmax, int nr, double *u_re, double *u_im,
+                   const double *nl_re, const double *nl_im, const double *np_re, const double *np_im,
+                   double dt, int flags);
+
+/* replaces create_erk2_cache (timestep.jl:896-968; dense path, use_krylov = false) */
+int gd_erk2_cache_create(gd_ctx *ctx, int lmax, int nr, int bw, const double *r, double nu, double dt,
+                         int fd_mode, gd_erk2_cache **out);
+/* the ERK2Cache constructor (timestep.jl:830-851): nl caller-given l values and
+ * their five nl-matrix sets (host, column-major nr x nr each) */
+int gd_erk2_cache_from_matrices(gd_ctx *ctx, int nl, const int *l_values, int nr, double dt,
+                                const double *E_half, const double *E_full, const double *phi1_half,
+                                const double *phi1_full, const double *phi2_full, gd_erk2_cache **out);
+int gd_erk2_cache_get(gd_erk2_cache *c, double *E_half, double *E_full, double *phi1_half,
+                      double *phi1_full, double *phi2_full /* host, nslots matrices each */);
+int gd_erk2_cache_destroy(gd_erk2_cache *c);
+/* replaces ERK2FieldBuffers (timestep.jl:1283-1313): per-field device planes */
+int gd_erk2_buffers_create(gd_erk2_cache *c, int lmax, int mmax, int nr, gd_erk2_buffers **out);
+int gd_erk2_buffers_destroy(gd_erk2_buffers *b);
+/* replaces erk2_prepare_field! (timestep.jl:1316-1438): n_current = nl,
+ * linear = E_full u, k1 = phi1_full nl, stage = E_half u + dt/2 phi1_half nl */
+int gd_erk2_prepare_field(gd_erk2_buffers *b, const double *u_re, const double *u_im,
+                          const double *nl_re, const double *nl_im, double dt, int flags);
+/* replaces erk2_apply_stage! (timestep.jl:1440-1444): u = stage */
+int gd_erk2_apply_stage(gd_erk2_buffers *b, double *u_re, double *u_im, int flags);
+/* replaces erk2_store_stage_nonlinear! (timestep.jl:1446-1452): stage_nl = nl */
+int gd_erk2_store_stage_nonlinear(gd_erk2_buffers *b, const double *nl_re, const double *nl_im, int flags);
+/* replaces erk2_finalize_field! (timestep.jl:1449-1555):
+ * u = linear + dt (k1 + phi2_full (stage_nl - n_current)) */
+int gd_erk2_finalize_field(gd_erk2_buffers *b, double *u_re, double *u_im, double dt, int flags);
+
 #ifdef __cplusplus
 }
 #endif
