@@ -222,6 +222,86 @@ def cpu_baseline(st, n_steps, fp64_one_step):
             "sample": "%d CNAB2 steps of the FP64 oracle at lmax=127 nr=65 (%.1f s)" % (n_steps, dt)}, err
 
 
+def exp_integrators(reps):
+    """EAB2 (timestep.jl:425-487) and the staged ERK2 calls (timestep.jl:1316-1555)
+    at this shell, nu = 1, dt = 1e-5, fields device resident.  reps consecutive
+    calls are captured in one graph and replayed (the per-call host time of the
+    ctypes boundary would otherwise hide the device time); the replay is checked
+    bit for bit against the same calls made eagerly."""
+    import torch
+
+    import gdhip
+
+    lmax, nr, dt = CFG["lmax"], CFG["nr"], CFG["dt"]
+    cfg = gdhip.create_shtnskit_config(lmax, lmax, nlat=CFG["nlat"], nlon=CFG["nlon"])
+    dom = gdhip.create_radial_domain(nr, CFG["rratio"])
+    t0 = time.perf_counter()
+    etd = gdhip.create_etd_cache(cfg, dom, 1.0, dt)
+    t_etd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    erk = gdhip.create_erk2_cache(cfg, dom, 1.0, dt)
+    t_erk = time.perf_counter() - t0
+    gen = torch.Generator(device="cuda").manual_seed(5)
+
+    def field():
+        f = gdhip.SHTnsSpectralField(cfg, nr)
+        f.data_real = torch.randn(nr, cfg.nlm, dtype=torch.float64, device="cuda", generator=gen)
+        f.data_imag = torch.randn(nr, cfg.nlm, dtype=torch.float64, device="cuda", generator=gen)
+        return f
+
+    u, nl, npv = field(), field(), field()
+    buf = gdhip.ERK2FieldBuffers(u, nl, erk)
+    u0 = (u.data_real.clone(), u.data_imag.clone())
+    calls = {
+        "eab2_update": lambda: gdhip.eab2_update(u, nl, npv, etd, cfg, dt),
+        "erk2_prepare_field": lambda: gdhip.erk2_prepare_field(buf, u, nl, erk, cfg, dt),
+        "erk2_finalize_field": lambda: gdhip.erk2_finalize_field(buf, u, erk, cfg, dt),
+    }
+    plane = cfg.nlm * nr * 8
+    mats = (lmax + 1) * nr * nr * 8
+    # algorithmic HBM bytes per call: planes read + written once, each matrix set once
+    nbytes = {"eab2_update": 8 * plane + 2 * mats, "erk2_prepare_field": 12 * plane + 4 * mats,
+              "erk2_finalize_field": 10 * plane + 1 * mats}
+    out = {"workload": "lmax=mmax=127 nr=65 nu=1 dt=1e-5, one field (re, im planes) in HBM",
+           "cache_build_s": {"etd": round(t_etd, 3), "erk2": round(t_erk, 3)}}
+    s = torch.cuda.Stream()
+    for name, fn in calls.items():
+        def reset():
+            u.data_real.copy_(u0[0])
+            u.data_imag.copy_(u0[1])
+        reset()
+        fn()  # first call sizes the library's work buffers outside the capture
+        torch.cuda.synchronize()
+        reset()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        eager = (u.data_real.clone(), u.data_imag.clone())
+        reset()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            for _ in range(reps):
+                fn()
+        reset()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        same = bool(torch.equal(u.data_real, eager[0]) and torch.equal(u.data_imag, eager[1]))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        nrep = 5
+        e0.record()
+        for _ in range(nrep):
+            g.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / (nrep * reps)
+        out[name] = {"ms_per_call": round(ms, 5), "algorithmic_MB": round(nbytes[name] / 1e6, 2),
+                     "GB_per_s": round(nbytes[name] / (ms * 1e-3) / 1e9, 1),
+                     "graph_replay_equals_eager": same}
+    return out
+
+
 def copy_probe(nbytes):
     """One plain copy kernel (copy_kernel) of nbytes, outside the timed region."""
     import ctypes
@@ -251,6 +331,9 @@ def main():
                     help="also run one plain device copy of this many bytes (HBM counter calibration)")
     ap.add_argument("--no-full", action="store_true",
                     help="skip the full-products comparison (profiling runs of the headline path)")
+    ap.add_argument("--no-exp", action="store_true", help="skip the EAB2 / ERK2 measurements")
+    ap.add_argument("--exp-reps", type=int, default=20)
+    ap.add_argument("--only-exp", action="store_true", help="print only the EAB2 / ERK2 measurements")
     ap.add_argument("--derive-traffic", metavar="DIR",
                     help="only turn rocprofv3 FETCH_SIZE/WRITE_SIZE runs under DIR into the traffic JSON")
     ap.add_argument("--summarize-trace", metavar="DIR",
@@ -268,6 +351,14 @@ def main():
         sys.exit(2)
 
     import numpy as np
+
+    if not args.no_exp:
+        # torch's bundled HIP runtime must be the process's HIP runtime before
+        # libgdhip loads, or the two do not see the same device
+        import torch  # noqa: F401
+    if args.only_exp:
+        print(json.dumps(exp_integrators(args.exp_reps)), flush=True)
+        return
 
     import gdhip
 
@@ -343,6 +434,11 @@ def main():
                                 [float("%.3e" % e) for e in err_full],
                                 "fp64_steps_per_s": round(args.steps / tf64, 2),
                                 "kernel_ms_mixed": {k: round(v, 5) for k, v in prof_full.items()}}
+    if not args.no_exp:
+        try:
+            out["exp_integrators"] = exp_integrators(args.exp_reps)
+        except Exception as exc:  # a secondary measurement never hides the headline line
+            out["exp_integrators"] = {"error": repr(exc)[:300]}
     if not args.no_cpu_baseline:
         cb, err64 = cpu_baseline(st, args.cpu_steps, one_step64)
         out["cpu_baseline"] = cb

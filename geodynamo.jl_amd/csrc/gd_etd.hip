@@ -316,88 +316,152 @@ extern "C" int gd_exp_functions(int lmax, int nr, int bw, const double *r, doubl
 
 // ---------------------------------------------------------------- device: per-l dense products
 namespace {
-constexpr int ETD_THREADS = 256;
-constexpr int RG = 8; // rows per work item
+constexpr int TC = 32; // lm columns per block (two 16-column MFMA tiles)
+constexpr int XB = 8;  // x-tile loads per batch
+constexpr int MAX_PASS = 4;
 
-// LDS bytes of the staged matrix: nr rows padded to a multiple of RG, row stride nr + 1
-inline size_t dense_lds(int nr) { return sizeof(double) * (size_t)((nr + RG - 1) / RG * RG) * (nr + 1); }
+// Device matrices: per slot nr columns of ldm = nr rounded up to even doubles
+// (row nr, when present, is zero).
+inline int ldm_of(int nr) { return nr + (nr & 1); }
 
-// out = b0 y0 + b1 y1 + sum over passes p of alpha_p M_p,l (ca_p x_p + cb_p xb_p)
-// on l-major planes (element (lm, r) at lm + nlm r).  A null M_p skips pass p,
-// a null y or xb skips that term.  Block (l, plane).
-struct DenseArgs {
-    const double *M[2]; // [slot] column-major nr x nr
-    double alpha[2], ca[2], cb[2];
-    const double *x[2][2]; // [pass][plane]
-    const double *xb[2][2];
+// One pass: acc += M_l (alpha x) on one re/im plane; with out set,
+// out = b0 y0 + b1 y1 + acc is stored after the pass and acc restarts; xcopy,
+// if set, receives a copy of x.  A pass's x must not be an earlier pass's out.
+// Consecutive passes with the same M stage it once.  Planes are l-major:
+// element (lm, r) at lm + nlm r.
+struct DensePass {
+    const double *M; // [slot][column][ldm]
+    double alpha;
+    const double *x[2];
+    double *xcopy[2];
+    double *out[2];
     const double *y0[2], *y1[2];
     double b0, b1;
-    double *out[2];
-    const int *slot_of_l; // l -> matrix slot (-1: none)
-    const int *col0;      // l-major start of each l, lmax + 2 entries
-    int nr, nlm;
-    int *status;
+};
+struct DenseProg {
+    DensePass p[MAX_PASS];
+    int npass;
+    const int *slot_of_l; // l -> matrix slot (the host has checked every l of the field)
+    const int4 *tiles;    // (l, first lm column, column count, 0)
+    int nr, nlm, ldm;
 };
 
-// Work item (row group g, column): RG rows of one column, so that every x
-// load feeds RG multiply-adds.  S holds the matrix padded to a multiple of RG
-// rows with zeros (no row guard in the inner loop).
-__global__ void __launch_bounds__(ETD_THREADS) dense_lmajor_kernel(DenseArgs a)
+// Launch shape: wave w of a block computes the 16-row tile w of the block's
+// rows against both 16-column tiles.  nr <= 68: one block holds every row
+// (W = ceil(nr/16) waves; in-place outputs are safe); 69 <= nr <= 127: one
+// wave per 16-row chunk, one block per chunk.
+inline int waves_of(int nr) { return nr <= 68 ? (nr + 15) / 16 : 1; }
+inline int chunks_of(int nr) { return nr <= 68 ? 1 : (nr + 15) / 16; }
+__host__ __device__ inline int kpad_of(int nr) { return (nr + 3) & ~3; }
+// LDS: x tile (kpad x 32) then the block's matrix rows (kpad columns of 16 W)
+inline size_t dense_lds(int nr) { return sizeof(double) * (size_t)kpad_of(nr) * (TC + 16 * waves_of(nr)); }
+
+// Block (tile of <= 32 columns of one l, plane blockIdx.y, row chunk
+// blockIdx.z).  Per pass the x tile is loaded (coalesced, two rows per wave
+// load; x columns past the tile are clamped copies whose results are not
+// stored) and, when it changes, the block's matrix rows are copied global ->
+// LDS by the lanes directly (global_load_lds, 16 bytes per lane, no
+// registers), then each wave runs FP64
+// MFMA 16x16x4 over k: A = M rows (16 x 4), B = alpha x (4 x 16).  Columns
+// and rows k >= nr of the tiles are zero.  Stores follow a block barrier, so
+// with one chunk an output may alias the pass's input.
+template <int W> __global__ void __launch_bounds__(64 * W) dense_mfma_kernel(DenseProg a)
 {
-    extern __shared__ double S[]; // npad x (nr + 1), S[r (nr+1) + j] = M(r, j)
-    const int l = blockIdx.x, pl = blockIdx.y, n = a.nr, ld = n + 1;
-    const int slot = a.slot_of_l[l];
-    if (slot < 0) {
-        // the reference errors on an l without a cache entry (timestep.jl:1366-1368)
-        if (threadIdx.x == 0) atomicOr(a.status, 4);
-        return;
+    constexpr int RB = 16 * W; // rows per block
+    extern __shared__ double sm[];
+    const int n = a.nr, ldm = a.ldm, kp = kpad_of(n);
+    double *Sx = sm;           // [k][TC]
+    double *Sm = sm + kp * TC; // [k][RB]: Sm[k RB + rr] = M(r0 + rr, k)
+    const int4 t = a.tiles[blockIdx.x];
+    const int pl = blockIdx.y, r0 = blockIdx.z * RB, tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6, ncol = t.z;
+    const size_t nlm = (size_t)a.nlm, col0 = (size_t)t.y;
+    const size_t moff = (size_t)a.slot_of_l[t.x] * n * ldm;
+    // the k padding (rows n..kp-1 of x, columns n..kp-1 of M) stays zero
+    for (int e = tid; e < (kp - n) * (TC + RB); e += 64 * W) {
+        if (e < (kp - n) * TC)
+            Sx[n * TC + e] = 0.0;
+        else
+            Sm[n * RB + e - (kp - n) * TC] = 0.0;
     }
-    const int ng = (n + RG - 1) / RG, npad = ng * RG;
-    const int c0 = a.col0[l], nc = a.col0[l + 1] - c0, total = nc * ng;
-    double *out = a.out[pl];
-    const size_t nlm = (size_t)a.nlm;
-    bool first = true;
-    for (int p = 0; p < 2; ++p) {
-        if (!a.M[p]) continue;
-        const double *Mg = a.M[p] + (size_t)slot * n * n;
-        __syncthreads(); // the previous pass has finished reading S
-        for (int e = threadIdx.x; e < npad * n; e += ETD_THREADS) {
-            const int j = e / npad, r = e - j * npad;
-            S[r * ld + j] = r < n ? Mg[r + (size_t)n * j] : 0.0;
-        }
-        __syncthreads();
-        const double *x = a.x[p][pl], *xb = a.xb[p][pl];
-        const double ca = a.ca[p], cb = a.cb[p], alpha = a.alpha[p];
-        for (int e = threadIdx.x; e < total; e += ETD_THREADS) {
-            const int g = e / nc, mc = e - g * nc;
-            const size_t col = (size_t)c0 + mc;
-            const double *Sg = S + g * RG * ld;
-            double acc[RG];
+    const double *Mslot0 = nullptr;
+    doublex4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int q = 0; q < RG; ++q) acc[q] = 0.0;
-            for (int j = 0; j < n; ++j) {
-                double v = ca * x[col + nlm * j];
-                if (xb) v += cb * xb[col + nlm * j];
+    for (int pi = 0; pi < MAX_PASS; ++pi) {
+        if (pi >= a.npass) break;
+        const DensePass &P = a.p[pi];
+        __syncthreads(); // the previous pass is done with the LDS tiles
+        {
+            // x tile: a wave load covers rows 2i and 2i + 1 (lane -> row 2i + lane / 32,
+            // column lane % 32, columns past the tile clamped); XB loads are issued
+            // before their LDS stores
+            const double *xs = P.x[pl] + col0 + min(lane & (TC - 1), ncol - 1);
+            const int xr = lane >> 5;
+            for (int i0 = wave; 2 * i0 < n; i0 += W * XB) {
+                double v[XB];
 #pragma unroll
-                for (int q = 0; q < RG; ++q) acc[q] += Sg[q * ld + j] * v;
-            }
+                for (int b = 0; b < XB; ++b) v[b] = xs[nlm * min(2 * (i0 + b * W) + xr, n - 1)];
 #pragma unroll
-            for (int q = 0; q < RG; ++q) {
-                const int r = g * RG + q;
-                if (r >= n) break;
-                const size_t o = col + nlm * r;
-                double v;
-                if (first) {
-                    v = 0.0;
-                    if (a.y0[pl]) v += a.b0 * a.y0[pl][o];
-                    if (a.y1[pl]) v += a.b1 * a.y1[pl][o];
-                } else {
-                    v = out[o]; // written by this same thread in the previous pass
+                for (int b = 0; b < XB; ++b) {
+                    const int k = 2 * (i0 + b * W) + xr;
+                    if (k < n) Sx[k * TC + (lane & (TC - 1))] = v[b];
                 }
-                out[o] = v + alpha * acc[q];
             }
         }
-        first = false;
+        if (P.M != Mslot0) {
+            // matrix rows r0 .. r0 + RB - 1 of columns 0 .. n - 1 in 16-byte row pairs:
+            // pair q of the flattened [k][RB] block (columns start 16-byte aligned,
+            // ldm is even); row pairs past the matrix clamped (not stored)
+            const double *Ms = P.M + moff;
+            const int nq = n * RB / 2;
+            for (int i = wave; i * 64 < nq; i += W) {
+                const int q = i * 64 + lane;
+                if (q < nq) {
+                    const int k = q / (RB / 2), w2 = q - k * (RB / 2);
+                    const int row = min(r0 + 2 * w2, ldm - 2);
+                    __builtin_amdgcn_global_load_lds(Ms + (size_t)k * ldm + row, Sm + i * 128, 16, 0, 0);
+                }
+            }
+            Mslot0 = P.M;
+        }
+        __builtin_amdgcn_s_waitcnt(0); // this wave's direct LDS copies have landed
+        __syncthreads();
+        if (P.xcopy[pl] && blockIdx.z == 0) {
+            for (int e = tid; e < n * TC; e += 64 * W)
+                if ((e & (TC - 1)) < ncol) P.xcopy[pl][col0 + (e & (TC - 1)) + nlm * (e >> 5)] = Sx[e];
+        }
+        // MFMA 16x16x4: lane holds A[lane % 16][lane / 16], B[lane / 16][lane % 16]
+        const double al = P.alpha;
+        const double *Sa = Sm + wave * 16 + (lane & 15) + (lane >> 4) * RB;
+        const double *Sb = Sx + (lane & 15) + (lane >> 4) * TC;
+#pragma unroll 4
+        for (int k0 = 0; k0 < kp; k0 += 4) {
+            const double av = Sa[k0 * RB];
+            const double b0 = al * Sb[k0 * TC], b1 = al * Sb[k0 * TC + 16];
+            acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b1, acc1, 0, 0, 0);
+        }
+        if (P.out[pl]) {
+            __syncthreads(); // every wave's reads of this pass are done (in-place outputs)
+            // accumulator element v of lane: row (lane / 16) + 4 v, column lane % 16
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = r0 + wave * 16 + (lane >> 4) + 4 * v;
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct) {
+                    const int cc = ct * 16 + (lane & 15);
+                    if (r < n && cc < ncol) {
+                        const size_t o = col0 + cc + nlm * r;
+                        double val = ct ? acc1[v] : acc0[v];
+                        if (P.y0[pl]) val += P.b0 * P.y0[pl][o];
+                        if (P.y1[pl]) val += P.b1 * P.y1[pl][o];
+                        P.out[pl][o] = val;
+                    }
+                }
+            }
+            acc0 = doublex4{0.0, 0.0, 0.0, 0.0};
+            acc1 = doublex4{0.0, 0.0, 0.0, 0.0};
+        }
     }
 }
 } // namespace
@@ -406,19 +470,20 @@ __global__ void __launch_bounds__(ETD_THREADS) dense_lmajor_kernel(DenseArgs a)
 struct gd_exp_cache {
     gd_ctx *ctx = nullptr;
     int kind = 0; // GD_EXP_ETD or GD_EXP_ERK2
-    int nslots = 0, nr = 0, lmax = -1, nmat = 0;
+    int nslots = 0, nr = 0, ldm = 0, lmax = -1, nmat = 0;
     double dt = 0.0;
     std::vector<int> l_values;
-    DevBuf mats;      // nmat blocks of nslots column-major nr x nr matrices
-    DevBuf slot_of_l; // lmax + 1 ints
-    DevBuf status;
-    DevBuf col0; // l-major column starts for (c_lmax, c_mmax)
-    int c_lmax = -1, c_mmax = -1, c_nlm = 0;
-    DevBuf work, hin; // result planes; host staging
-    const double *mat(int k) const { return mats.as<double>() + (size_t)k * nslots * nr * nr; }
+    DevBuf mats;                // nmat blocks of nslots matrices, columns padded to ldm
+    std::vector<int> slot_host; // lmax + 1 ints: l -> slot, -1 if the cache has no matrix for l
+    DevBuf slot_of_l;           // its device copy
+    DevBuf tiles;               // column tiles of a (t_lmax, t_mmax) field
+    int t_lmax = -1, t_mmax = -1, ntiles = 0;
+    DevBuf hin, hout; // host staging
+    size_t per() const { return (size_t)nslots * nr * ldm; }
+    const double *mat(int k) const { return mats.as<double>() + (size_t)k * per(); }
     void release()
     {
-        for (DevBuf *b : {&mats, &slot_of_l, &status, &col0, &work, &hin}) b->release();
+        for (DevBuf *b : {&mats, &slot_of_l, &tiles, &hin, &hout}) b->release();
     }
 };
 struct gd_etd_cache : gd_exp_cache {};  // ETDCache: E, phi1 (timestep.jl:24-29)
@@ -432,6 +497,7 @@ int cache_upload(gd_ctx *ctx, gd_exp_cache *c, int kind, const std::vector<int> 
     c->kind = kind;
     c->nslots = (int)lvals.size();
     c->nr = nr;
+    c->ldm = ldm_of(nr);
     c->dt = dt;
     c->l_values = lvals;
     c->nmat = (int)host_mats.size();
@@ -443,18 +509,20 @@ int cache_upload(gd_ctx *ctx, gd_exp_cache *c, int kind, const std::vector<int> 
     std::vector<int> slot(c->lmax + 1, -1);
     for (int s = 0; s < c->nslots; ++s)
         if (slot[lvals[s]] < 0) slot[lvals[s]] = s; // first match, as findfirst
-    const size_t per = (size_t)c->nslots * nr * nr;
-    GD_HIP(hipSetDevice(ctx->device));
-    GD_TRY(c->mats.ensure(sizeof(double) * per * c->nmat));
+    // columns padded from nr to ldm doubles
+    std::vector<double> padded(c->per() * c->nmat, 0.0);
     for (int k = 0; k < c->nmat; ++k)
-        GD_HIP(hipMemcpyAsync(c->mats.as<double>() + k * per, host_mats[k], sizeof(double) * per,
-                              hipMemcpyHostToDevice, ctx->stream));
+        for (size_t sj = 0; sj < (size_t)c->nslots * nr; ++sj)
+            std::copy(host_mats[k] + sj * nr, host_mats[k] + (sj + 1) * nr, padded.begin() + k * c->per() + sj * c->ldm);
+    GD_HIP(hipSetDevice(ctx->device));
+    GD_TRY(c->mats.ensure(sizeof(double) * padded.size()));
+    GD_HIP(hipMemcpyAsync(c->mats.p, padded.data(), sizeof(double) * padded.size(), hipMemcpyHostToDevice,
+                          ctx->stream));
     GD_TRY(c->slot_of_l.ensure(sizeof(int) * slot.size()));
     GD_HIP(hipMemcpyAsync(c->slot_of_l.p, slot.data(), sizeof(int) * slot.size(), hipMemcpyHostToDevice,
                           ctx->stream));
-    GD_TRY(c->status.ensure(sizeof(int)));
-    GD_HIP(hipMemsetAsync(c->status.p, 0, sizeof(int), ctx->stream));
     GD_HIP(hipStreamSynchronize(ctx->stream));
+    c->slot_host = slot;
     return GD_OK;
 }
 
@@ -473,13 +541,17 @@ int cache_create(gd_ctx *ctx, int kind, const std::vector<int> &lv, int nr, doub
     return GD_OK;
 }
 
+// host copy of matrix set k, nr x nr column-major per slot
 int cache_get(const gd_exp_cache *c, int k, double *host)
 {
     if (!host) return GD_OK;
-    const size_t per = (size_t)c->nslots * c->nr * c->nr;
+    std::vector<double> padded(c->per());
     GD_HIP(hipSetDevice(c->ctx->device));
-    GD_HIP(hipMemcpyAsync(host, c->mat(k), sizeof(double) * per, hipMemcpyDeviceToHost, c->ctx->stream));
+    GD_HIP(hipMemcpyAsync(padded.data(), c->mat(k), sizeof(double) * c->per(), hipMemcpyDeviceToHost,
+                          c->ctx->stream));
     GD_HIP(hipStreamSynchronize(c->ctx->stream));
+    for (size_t sj = 0; sj < (size_t)c->nslots * c->nr; ++sj)
+        std::copy(padded.begin() + sj * c->ldm, padded.begin() + sj * c->ldm + c->nr, host + sj * c->nr);
     return GD_OK;
 }
 
@@ -492,56 +564,61 @@ int cache_destroy(gd_exp_cache *c)
     return GD_OK;
 }
 
-// l-major column starts of a (lmax, mmax) field: l occupies min(l, mmax) + 1
-// columns (shtnskit_transforms.jl:177-188)
-int col_starts(gd_exp_cache *c, int lmax, int mmax)
+// column tiles of a (lmax, mmax) field: l occupies the min(l, mmax) + 1
+// l-major columns after those of l - 1 (shtnskit_transforms.jl:177-188),
+// cut into tiles of at most 32
+int field_tiles(gd_exp_cache *c, int lmax, int mmax)
 {
-    if (c->c_lmax == lmax && c->c_mmax == mmax) return GD_OK;
-    std::vector<int> c0(lmax + 2);
-    c0[0] = 0;
-    for (int l = 0; l <= lmax; ++l) c0[l + 1] = c0[l] + std::min(l, mmax) + 1;
-    GD_TRY(c->col0.ensure(sizeof(int) * c0.size()));
-    GD_HIP(hipMemcpyAsync(c->col0.p, c0.data(), sizeof(int) * c0.size(), hipMemcpyHostToDevice, c->ctx->stream));
-    GD_HIP(hipStreamSynchronize(c->ctx->stream)); // c0 is a stack array
-    c->c_lmax = lmax;
-    c->c_mmax = mmax;
-    c->c_nlm = c0[lmax + 1];
+    if (c->t_lmax == lmax && c->t_mmax == mmax) return GD_OK;
+    std::vector<int4> t;
+    int c0 = 0;
+    for (int l = 0; l <= lmax; ++l) {
+        const int nc = std::min(l, mmax) + 1;
+        for (int s = 0; s < nc; s += TC) t.push_back(make_int4(l, c0 + s, std::min(TC, nc - s), 0));
+        c0 += nc;
+    }
+    GD_TRY(c->tiles.ensure(sizeof(int4) * t.size()));
+    GD_HIP(hipMemcpyAsync(c->tiles.p, t.data(), sizeof(int4) * t.size(), hipMemcpyHostToDevice, c->ctx->stream));
+    GD_HIP(hipStreamSynchronize(c->ctx->stream)); // t is a host temporary
+    c->t_lmax = lmax;
+    c->t_mmax = mmax;
+    c->ntiles = (int)t.size();
     return GD_OK;
 }
 
-int check_field(const gd_exp_cache *c, int lmax, int mmax, int nr, const char *what)
+// shape checks; with check_l also that every l of the field has a matrix
+int check_field(const gd_exp_cache *c, int lmax, int mmax, int nr, const char *what, bool check_l = true)
 {
     GD_CHECK(lmax >= 0 && mmax >= 0 && mmax <= lmax, GD_ERR_ARG, "%s: bad lmax=%d mmax=%d", what, lmax, mmax);
     GD_CHECK(nr == c->nr, GD_ERR_ARG, "%s: field nr=%d, cache nr=%d", what, nr, c->nr);
-    GD_CHECK(lmax <= c->lmax, GD_ERR_MISSING_L, "%s: missing exponential cache entry for l=%d", what, c->lmax + 1);
-    GD_CHECK(dense_lds(c->nr) <= 64 * 1024, GD_ERR_ARG, "%s: nr=%d too large for an LDS-staged matrix", what, c->nr);
+    // the reference errors on an l without a cache entry (timestep.jl:446-447, 1366-1368)
+    for (int l = 0; check_l && l <= lmax; ++l)
+        GD_CHECK(l <= c->lmax && c->slot_host[l] >= 0, GD_ERR_MISSING_L, "%s: missing exponential cache entry for l=%d",
+                 what, l);
+    GD_CHECK(c->nr <= 127, GD_ERR_ARG, "%s: nr=%d above the supported 127", what, c->nr);
     return GD_OK;
 }
 
-int launch_dense(gd_exp_cache *c, DenseArgs a, int lmax)
+int launch_prog(gd_exp_cache *c, DenseProg a, int lmax, int mmax)
 {
+    GD_TRY(field_tiles(c, lmax, mmax));
     a.slot_of_l = c->slot_of_l.as<int>();
-    a.status = c->status.as<int>();
-    a.col0 = c->col0.as<int>();
+    a.tiles = c->tiles.as<int4>();
     a.nr = c->nr;
-    a.nlm = c->c_nlm;
-    hipLaunchKernelGGL(dense_lmajor_kernel, dim3(lmax + 1, 2), dim3(ETD_THREADS), dense_lds(c->nr), c->ctx->stream, a);
-    return gd_check_launch("dense_lmajor_kernel");
-}
-
-// synchronises; GD_ERR_MISSING_L if a block met an l without a cache entry
-int read_status(gd_exp_cache *c)
-{
-    int st = 0;
-    GD_HIP(hipMemcpyAsync(&st, c->status.p, sizeof(int), hipMemcpyDeviceToHost, c->ctx->stream));
-    GD_HIP(hipStreamSynchronize(c->ctx->stream));
-    if (st) {
-        GD_HIP(hipMemsetAsync(c->status.p, 0, sizeof(int), c->ctx->stream));
-        GD_HIP(hipStreamSynchronize(c->ctx->stream));
-        gd_set_error("missing exponential cache entry for an l of the field");
-        return GD_ERR_MISSING_L;
+    a.nlm = gd_nlm_of(lmax, mmax);
+    a.ldm = c->ldm;
+    const int W = waves_of(c->nr);
+    const dim3 grid(c->ntiles, 2, chunks_of(c->nr)), block(64 * W);
+    const size_t lds = dense_lds(c->nr);
+    hipStream_t st = c->ctx->stream;
+    switch (W) {
+    case 1: hipLaunchKernelGGL(dense_mfma_kernel<1>, grid, block, lds, st, a); break;
+    case 2: hipLaunchKernelGGL(dense_mfma_kernel<2>, grid, block, lds, st, a); break;
+    case 3: hipLaunchKernelGGL(dense_mfma_kernel<3>, grid, block, lds, st, a); break;
+    case 4: hipLaunchKernelGGL(dense_mfma_kernel<4>, grid, block, lds, st, a); break;
+    default: hipLaunchKernelGGL(dense_mfma_kernel<5>, grid, block, lds, st, a); break;
     }
-    return GD_OK;
+    return gd_check_launch("dense_mfma_kernel");
 }
 
 // device views of caller planes; GD_HOST inputs are staged into c->hin
@@ -560,11 +637,44 @@ int stage_in(gd_exp_cache *c, size_t plane, int flags, int n, const double *cons
     return GD_OK;
 }
 
-int copy_out(gd_exp_cache *c, size_t plane, int flags, const double *src, double *dst)
+// output planes: the caller's (device) or c->hout (GD_HOST, copied back by finish_out)
+int out_planes(gd_exp_cache *c, size_t plane, int flags, double *re, double *im, double **dev)
 {
-    GD_HIP(hipMemcpyAsync(dst, src, sizeof(double) * plane,
-                          (flags & GD_HOST) ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, c->ctx->stream));
+    if (!(flags & GD_HOST)) {
+        dev[0] = re;
+        dev[1] = im;
+        return GD_OK;
+    }
+    GD_TRY(c->hout.ensure(sizeof(double) * plane * 2));
+    dev[0] = c->hout.as<double>();
+    dev[1] = c->hout.as<double>() + plane;
     return GD_OK;
+}
+
+int finish_out(gd_exp_cache *c, size_t plane, int flags, double *re, double *im)
+{
+    if (!(flags & GD_HOST)) return GD_OK;
+    GD_HIP(hipMemcpyAsync(re, c->hout.p, sizeof(double) * plane, hipMemcpyDeviceToHost, c->ctx->stream));
+    GD_HIP(hipMemcpyAsync(im, c->hout.as<double>() + plane, sizeof(double) * plane, hipMemcpyDeviceToHost,
+                          c->ctx->stream));
+    GD_HIP(hipStreamSynchronize(c->ctx->stream));
+    return GD_OK;
+}
+
+DensePass pass(const double *M, double alpha, const double *x0, const double *x1)
+{
+    DensePass p{};
+    p.M = M;
+    p.alpha = alpha;
+    p.x[0] = x0;
+    p.x[1] = x1;
+    return p;
+}
+
+void set_out(DensePass &p, double *o0, double *o1)
+{
+    p.out[0] = o0;
+    p.out[1] = o1;
 }
 } // namespace
 
@@ -602,34 +712,29 @@ extern "C" int gd_eab2_update(gd_etd_cache *c, int lmax, int mmax, int nr, doubl
 {
     GD_CHECK(c && u_re && u_im && nl_re && nl_im && np_re && np_im, GD_ERR_ARG, "gd_eab2_update: null argument");
     GD_TRY(check_field(c, lmax, mmax, nr, "gd_eab2_update"));
-    gd_ctx *ctx = c->ctx;
-    GD_HIP(hipSetDevice(ctx->device));
-    GD_TRY(col_starts(c, lmax, mmax));
-    const size_t plane = (size_t)c->c_nlm * nr;
+    GD_HIP(hipSetDevice(c->ctx->device));
+    const size_t plane = (size_t)gd_nlm_of(lmax, mmax) * nr;
     const double *src[6] = {u_re, u_im, nl_re, nl_im, np_re, np_im}, *d[6];
+    double *o[2];
     GD_TRY(stage_in(c, plane, flags, 6, src, d));
-    GD_TRY(c->work.ensure(sizeof(double) * plane * 2));
-    // u = E u + dt phi1 (3/2 nl - 1/2 nl_prev)
-    DenseArgs a{};
-    a.M[0] = c->mat(0);
-    a.M[1] = c->mat(1);
-    a.alpha[0] = 1.0;
-    a.ca[0] = 1.0;
-    a.alpha[1] = dt;
-    a.ca[1] = 1.5;
-    a.cb[1] = -0.5;
-    for (int p = 0; p < 2; ++p) {
-        a.x[0][p] = d[p];
-        a.x[1][p] = d[2 + p];
-        a.xb[1][p] = d[4 + p];
-        a.out[p] = c->work.as<double>() + p * plane;
+    // in place when one block holds every row of its columns; otherwise through c->hout
+    const bool via_hout = (flags & GD_HOST) || chunks_of(nr) > 1;
+    GD_TRY(out_planes(c, plane, via_hout ? GD_HOST : 0, u_re, u_im, o));
+    // u = E u + dt phi1 (3/2 nl - 1/2 nl_prev): the phi1 product as two passes
+    // over the same staged matrix
+    DenseProg a{};
+    a.npass = 3;
+    a.p[0] = pass(c->mat(0), 1.0, d[0], d[1]);
+    a.p[1] = pass(c->mat(1), 1.5 * dt, d[2], d[3]);
+    a.p[2] = pass(c->mat(1), -0.5 * dt, d[4], d[5]);
+    set_out(a.p[2], o[0], o[1]);
+    GD_TRY(launch_prog(c, a, lmax, mmax));
+    if (via_hout && !(flags & GD_HOST)) {
+        GD_HIP(hipMemcpyAsync(u_re, o[0], sizeof(double) * plane, hipMemcpyDeviceToDevice, c->ctx->stream));
+        GD_HIP(hipMemcpyAsync(u_im, o[1], sizeof(double) * plane, hipMemcpyDeviceToDevice, c->ctx->stream));
+        return GD_OK;
     }
-    GD_TRY(launch_dense(c, a, lmax));
-    GD_TRY(read_status(c));
-    GD_TRY(copy_out(c, plane, flags, c->work.as<double>(), u_re));
-    GD_TRY(copy_out(c, plane, flags, c->work.as<double>() + plane, u_im));
-    if (flags & GD_HOST) GD_HIP(hipStreamSynchronize(ctx->stream));
-    return GD_OK;
+    return finish_out(c, plane, flags, u_re, u_im);
 }
 
 // ---------------------------------------------------------------- ERK2
@@ -688,7 +793,8 @@ extern "C" int gd_erk2_buffers_create(gd_erk2_cache *c, int lmax, int mmax, int 
 {
     GD_CHECK(c && out, GD_ERR_ARG, "gd_erk2_buffers_create: null argument");
     GD_CHECK(c->nslots > 0, GD_ERR_ARG, "ERK2 cache has no precomputed matrices");
-    GD_TRY(check_field(c, lmax, mmax, nr, "gd_erk2_buffers_create"));
+    // ERK2FieldBuffers only sizes the planes; a missing l errors in prepare/finalize as in the reference
+    GD_TRY(check_field(c, lmax, mmax, nr, "gd_erk2_buffers_create", false));
     GD_HIP(hipSetDevice(c->ctx->device));
     gd_erk2_buffers *b = new gd_erk2_buffers();
     b->cache = c;
@@ -726,40 +832,25 @@ extern "C" int gd_erk2_prepare_field(gd_erk2_buffers *b, const double *u_re, con
 {
     GD_CHECK(b && u_re && u_im && nl_re && nl_im, GD_ERR_ARG, "gd_erk2_prepare_field: null argument");
     gd_erk2_cache *c = b->cache;
+    GD_TRY(check_field(c, b->lmax, b->mmax, b->nr, "erk2_prepare_field"));
     GD_HIP(hipSetDevice(c->ctx->device));
-    GD_TRY(col_starts(c, b->lmax, b->mmax));
     const double *src[4] = {u_re, u_im, nl_re, nl_im}, *d[4];
     GD_TRY(stage_in(c, b->plane, flags, 4, src, d));
-    for (int p = 0; p < 2; ++p) // n_current = nl
-        GD_HIP(hipMemcpyAsync(b->pl(6 + p), d[2 + p], sizeof(double) * b->plane, hipMemcpyDeviceToDevice,
-                              c->ctx->stream));
     // matrices: 0 E_half, 1 E_full, 2 phi1_half, 3 phi1_full, 4 phi2_full
-    DenseArgs a{};
-    a.alpha[0] = 1.0;
-    a.ca[0] = 1.0;
-    a.M[0] = c->mat(1); // linear = E_full u
-    for (int p = 0; p < 2; ++p) {
-        a.x[0][p] = d[p];
-        a.out[p] = b->pl(0 + p);
-    }
-    GD_TRY(launch_dense(c, a, b->lmax));
-    a.M[0] = c->mat(3); // k1 = phi1_full nl
-    for (int p = 0; p < 2; ++p) {
-        a.x[0][p] = b->pl(6 + p);
-        a.out[p] = b->pl(2 + p);
-    }
-    GD_TRY(launch_dense(c, a, b->lmax));
-    a.M[0] = c->mat(0); // stage = E_half u + dt/2 phi1_half nl
-    a.M[1] = c->mat(2);
-    a.alpha[1] = dt / 2.0;
-    a.ca[1] = 1.0;
-    for (int p = 0; p < 2; ++p) {
-        a.x[0][p] = d[p];
-        a.x[1][p] = b->pl(6 + p);
-        a.out[p] = b->pl(4 + p);
-    }
-    GD_TRY(launch_dense(c, a, b->lmax));
-    return read_status(c);
+    DenseProg a{};
+    a.npass = 4;
+    a.p[0] = pass(c->mat(1), 1.0, d[0], d[1]); // linear = E_full u
+    set_out(a.p[0], b->pl(0), b->pl(1));
+    a.p[1] = pass(c->mat(3), 1.0, d[2], d[3]); // k1 = phi1_full nl; n_current = nl
+    a.p[1].xcopy[0] = b->pl(6);
+    a.p[1].xcopy[1] = b->pl(7);
+    set_out(a.p[1], b->pl(2), b->pl(3));
+    a.p[2] = pass(c->mat(0), 1.0, d[0], d[1]); // stage = E_half u + dt/2 phi1_half nl
+    a.p[3] = pass(c->mat(2), dt / 2.0, d[2], d[3]);
+    set_out(a.p[3], b->pl(4), b->pl(5));
+    GD_TRY(launch_prog(c, a, b->lmax, b->mmax));
+    if (flags & GD_HOST) GD_HIP(hipStreamSynchronize(c->ctx->stream)); // staged host inputs are read
+    return GD_OK;
 }
 
 extern "C" int gd_erk2_apply_stage(gd_erk2_buffers *b, double *u_re, double *u_im, int flags)
@@ -767,8 +858,9 @@ extern "C" int gd_erk2_apply_stage(gd_erk2_buffers *b, double *u_re, double *u_i
     GD_CHECK(b && u_re && u_im, GD_ERR_ARG, "gd_erk2_apply_stage: null argument");
     gd_exp_cache *c = b->cache;
     GD_HIP(hipSetDevice(c->ctx->device));
-    GD_TRY(copy_out(c, b->plane, flags, b->pl(4), u_re));
-    GD_TRY(copy_out(c, b->plane, flags, b->pl(5), u_im));
+    const hipMemcpyKind kind = (flags & GD_HOST) ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    GD_HIP(hipMemcpyAsync(u_re, b->pl(4), sizeof(double) * b->plane, kind, c->ctx->stream));
+    GD_HIP(hipMemcpyAsync(u_im, b->pl(5), sizeof(double) * b->plane, kind, c->ctx->stream));
     if (flags & GD_HOST) GD_HIP(hipStreamSynchronize(c->ctx->stream));
     return GD_OK;
 }
@@ -789,28 +881,22 @@ extern "C" int gd_erk2_finalize_field(gd_erk2_buffers *b, double *u_re, double *
 {
     GD_CHECK(b && u_re && u_im, GD_ERR_ARG, "gd_erk2_finalize_field: null argument");
     gd_erk2_cache *c = b->cache;
+    GD_TRY(check_field(c, b->lmax, b->mmax, b->nr, "erk2_finalize_field"));
     GD_HIP(hipSetDevice(c->ctx->device));
-    GD_TRY(col_starts(c, b->lmax, b->mmax));
-    GD_TRY(c->work.ensure(sizeof(double) * b->plane * 2));
-    // u = linear + dt k1 + dt phi2_full (stage_nl - n_current)
-    DenseArgs a{};
-    a.M[0] = c->mat(4);
-    a.alpha[0] = dt;
-    a.ca[0] = 1.0;
-    a.cb[0] = -1.0;
-    a.b0 = 1.0;
-    a.b1 = dt;
-    for (int p = 0; p < 2; ++p) {
-        a.x[0][p] = b->pl(8 + p);
-        a.xb[0][p] = b->pl(6 + p);
-        a.y0[p] = b->pl(0 + p);
-        a.y1[p] = b->pl(2 + p);
-        a.out[p] = c->work.as<double>() + p * b->plane;
-    }
-    GD_TRY(launch_dense(c, a, b->lmax));
-    GD_TRY(read_status(c));
-    GD_TRY(copy_out(c, b->plane, flags, c->work.as<double>(), u_re));
-    GD_TRY(copy_out(c, b->plane, flags, c->work.as<double>() + b->plane, u_im));
-    if (flags & GD_HOST) GD_HIP(hipStreamSynchronize(c->ctx->stream));
-    return GD_OK;
+    double *o[2];
+    GD_TRY(out_planes(c, b->plane, flags, u_re, u_im, o));
+    // u = linear + dt k1 + dt phi2_full (stage_nl - n_current): two passes over phi2
+    DenseProg a{};
+    a.npass = 2;
+    a.p[0] = pass(c->mat(4), dt, b->pl(8), b->pl(9));
+    a.p[1] = pass(c->mat(4), -dt, b->pl(6), b->pl(7));
+    set_out(a.p[1], o[0], o[1]);
+    a.p[1].b0 = 1.0;
+    a.p[1].b1 = dt;
+    a.p[1].y0[0] = b->pl(0);
+    a.p[1].y0[1] = b->pl(1);
+    a.p[1].y1[0] = b->pl(2);
+    a.p[1].y1[1] = b->pl(3);
+    GD_TRY(launch_prog(c, a, b->lmax, b->mmax));
+    return finish_out(c, b->plane, flags, u_re, u_im);
 }

@@ -100,8 +100,11 @@ def test_erk2_reference_known_answers(device):
     assert abs(got - want) < 1e-15
 
 
+# nr picks the launch shape: 9 (1 wave), 17 (2), 33 (3), 57 (4), 65 (5), 97 (rows split over blocks)
 @pytest.mark.parametrize("lmax,mmax,nr,dt,device", [(15, 15, 33, 1e-4, True), (15, 7, 17, 1e-3, False),
-                                                    (127, 127, 65, 1e-5, True)])
+                                                    (127, 127, 65, 1e-5, True), (31, 31, 9, 1e-3, True),
+                                                    (31, 20, 57, 1e-4, True), (23, 23, 97, 1e-5, True),
+                                                    (23, 11, 97, 1e-5, False)])
 def test_eab2_update_matches_oracle(lmax, mmax, nr, dt, device):
     import gdhip
 
@@ -121,7 +124,8 @@ def test_eab2_update_matches_oracle(lmax, mmax, nr, dt, device):
 
 
 @pytest.mark.parametrize("lmax,mmax,nr,dt,nu,device", [(31, 31, 33, 1e-4, 5.0, True), (15, 5, 17, 1e-3, 1.0, False),
-                                                        (127, 127, 65, 1e-5, 1.0, True)])
+                                                        (127, 127, 65, 1e-5, 1.0, True), (47, 47, 9, 1e-3, 1.0, True),
+                                                        (23, 23, 97, 1e-5, 1.0, True)])
 def test_erk2_staged_matches_oracle(lmax, mmax, nr, dt, nu, device):
     import gdhip
 
@@ -160,8 +164,16 @@ def test_missing_l_raises():
     with pytest.raises(gdhip.MissingLError):
         gdhip.erk2_prepare_field(buf, u, u, cache, cfg, 1e-3)
     big = types.SimpleNamespace(lmax=3, mmax=3, nlm=10, device=0, l_values=None)
+    ub = _field(big, nr, np.ones((nr, 10), complex), False)
+    bufb = gdhip.ERK2FieldBuffers(ub, ub, cache)  # sizes only, as the reference's constructor
     with pytest.raises(gdhip.MissingLError):
-        gdhip.ERK2FieldBuffers(_field(big, nr, np.ones((nr, 10), complex), False), None, cache)
+        gdhip.erk2_prepare_field(bufb, ub, ub, cache, big, 1e-3)
+    with pytest.raises(gdhip.MissingLError):
+        gdhip.erk2_finalize_field(bufb, ub, cache, big, 1e-3)
+    etd_cfg = types.SimpleNamespace(lmax=1, mmax=1, nlm=3, device=0)
+    etd = gdhip.create_etd_cache(etd_cfg, gdhip.create_radial_domain(nr, 0.35), 1.0, 1e-3)
+    with pytest.raises(gdhip.MissingLError):
+        gdhip.eab2_update(u, u, u, etd, cfg, 1e-3)
     # a cache with every l present still works after the error
     cache2 = gdhip.ERK2Cache(1e-3, [0, 1, 2], [eye] * 3, [2 * eye] * 3, [eye] * 3, [eye] * 3, [eye] * 3)
     buf2 = gdhip.ERK2FieldBuffers(u, u, cache2)
