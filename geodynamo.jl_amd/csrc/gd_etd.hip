@@ -591,7 +591,7 @@ int check_field(const gd_exp_cache *c, int lmax, int mmax, int nr, const char *w
 {
     GD_CHECK(lmax >= 0 && mmax >= 0 && mmax <= lmax, GD_ERR_ARG, "%s: bad lmax=%d mmax=%d", what, lmax, mmax);
     GD_CHECK(nr == c->nr, GD_ERR_ARG, "%s: field nr=%d, cache nr=%d", what, nr, c->nr);
-    // the reference errors on an l without a cache entry (timestep.jl:446-447, 1366-1368)
+    // the reference errors on an l without a cache entry (timestep.jl:447-449, 1357, 1475)
     for (int l = 0; check_l && l <= lmax; ++l)
         GD_CHECK(l <= c->lmax && c->slot_host[l] >= 0, GD_ERR_MISSING_L, "%s: missing exponential cache entry for l=%d",
                  what, l);

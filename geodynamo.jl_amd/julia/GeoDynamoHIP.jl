@@ -165,6 +165,105 @@ function solve_implicit_step!(sol_re, sol_im, rhs_re, rhs_im, M::HIPImplicitMatr
     return sol_re, sol_im
 end
 
+# ---------------------------------------------------------------- exponential integrators
+mutable struct HIPETDCache
+    ptr::Ptr{Cvoid}
+    lmax::Int; nr::Int; dt::Float64
+end
+
+"create_etd_cache (timestep.jl:37); r is domain.r (N, 7)"
+function create_etd_cache(lmax::Integer, r::Matrix{Float64}, diffusivity::Float64, dt::Float64; bw::Integer=3)
+    nr = size(r, 1)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve r check(ccall((:gd_etd_cache_create, libgdhip), Cint,
+        (Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Float64}, Cdouble, Cdouble, Cint, Ref{Ptr{Cvoid}}),
+        context(), lmax, nr, bw, r, diffusivity, dt, GD_FD_TRANSPOSE, h))
+    c = HIPETDCache(h[], lmax, nr, dt)
+    finalizer(q -> ccall((:gd_etd_cache_destroy, libgdhip), Cint, (Ptr{Cvoid},), q.ptr), c)
+    return c
+end
+
+"eab2_update! (timestep.jl:429): u = E u + dt phi1 (3/2 nl - 1/2 nl_prev), in place"
+function eab2_update!(u_re, u_im, nl_re, nl_im, np_re, np_im, etd::HIPETDCache, dt::Float64;
+                      lmax::Integer, mmax::Integer=lmax)
+    nr = size(u_re, 3)
+    GC.@preserve u_re u_im nl_re nl_im np_re np_im check(ccall((:gd_eab2_update, libgdhip), Cint,
+        (Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+         Ptr{Float64}, Cdouble, Cint),
+        etd.ptr, lmax, mmax, nr, u_re, u_im, nl_re, nl_im, np_re, np_im, dt, GD_HOST))
+    return u_re, u_im
+end
+
+mutable struct HIPERK2Cache
+    ptr::Ptr{Cvoid}
+    nr::Int; dt::Float64
+end
+
+"create_erk2_cache (timestep.jl:896), dense matrix functions"
+function create_erk2_cache(lmax::Integer, r::Matrix{Float64}, diffusivity::Float64, dt::Float64; bw::Integer=3)
+    nr = size(r, 1)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve r check(ccall((:gd_erk2_cache_create, libgdhip), Cint,
+        (Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Float64}, Cdouble, Cdouble, Cint, Ref{Ptr{Cvoid}}),
+        context(), lmax, nr, bw, r, diffusivity, dt, GD_FD_TRANSPOSE, h))
+    c = HIPERK2Cache(h[], nr, dt)
+    finalizer(q -> ccall((:gd_erk2_cache_destroy, libgdhip), Cint, (Ptr{Cvoid},), q.ptr), c)
+    return c
+end
+
+"The ERK2Cache constructor (timestep.jl:830): per-l matrices given by the caller"
+function create_erk2_cache(dt::Float64, l_values::Vector{Int}, E_half::Vector{Matrix{Float64}},
+                           E_full::Vector{Matrix{Float64}}, phi1_half::Vector{Matrix{Float64}},
+                           phi1_full::Vector{Matrix{Float64}}, phi2_full::Vector{Matrix{Float64}})
+    nr = size(E_full[1], 1)
+    flat(v) = reduce(vcat, vec.(v))   # column-major matrices one after another
+    mats = map(flat, (E_half, E_full, phi1_half, phi1_full, phi2_full))
+    lv = Cint.(l_values)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve lv mats check(ccall((:gd_erk2_cache_from_matrices, libgdhip), Cint,
+        (Ptr{Cvoid}, Cint, Ptr{Cint}, Cint, Cdouble, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+         Ptr{Float64}, Ref{Ptr{Cvoid}}),
+        context(), length(lv), lv, nr, dt, mats..., h))
+    c = HIPERK2Cache(h[], nr, dt)
+    finalizer(q -> ccall((:gd_erk2_cache_destroy, libgdhip), Cint, (Ptr{Cvoid},), q.ptr), c)
+    return c
+end
+
+mutable struct HIPERK2Buffers
+    ptr::Ptr{Cvoid}
+    cache::HIPERK2Cache   # keeps the cache alive while the buffers exist
+end
+
+"ERK2FieldBuffers (timestep.jl:1298)"
+function ERK2FieldBuffers(cache::HIPERK2Cache; lmax::Integer, mmax::Integer=lmax, nr::Integer=cache.nr)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:gd_erk2_buffers_create, libgdhip), Cint, (Ptr{Cvoid}, Cint, Cint, Cint, Ref{Ptr{Cvoid}}),
+                cache.ptr, lmax, mmax, nr, h))
+    b = HIPERK2Buffers(h[], cache)
+    finalizer(q -> ccall((:gd_erk2_buffers_destroy, libgdhip), Cint, (Ptr{Cvoid},), q.ptr), b)
+    return b
+end
+
+"erk2_prepare_field! (timestep.jl:1316)"
+erk2_prepare_field!(b::HIPERK2Buffers, u_re, u_im, nl_re, nl_im, dt::Float64) =
+    GC.@preserve u_re u_im nl_re nl_im check(ccall((:gd_erk2_prepare_field, libgdhip), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cdouble, Cint),
+        b.ptr, u_re, u_im, nl_re, nl_im, dt, GD_HOST))
+
+"erk2_apply_stage! (timestep.jl:1440)"
+erk2_apply_stage!(b::HIPERK2Buffers, u_re, u_im) = GC.@preserve u_re u_im check(ccall(
+    (:gd_erk2_apply_stage, libgdhip), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint), b.ptr, u_re, u_im, GD_HOST))
+
+"erk2_store_stage_nonlinear! (timestep.jl:1446)"
+erk2_store_stage_nonlinear!(b::HIPERK2Buffers, nl_re, nl_im) = GC.@preserve nl_re nl_im check(ccall(
+    (:gd_erk2_store_stage_nonlinear, libgdhip), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint),
+    b.ptr, nl_re, nl_im, GD_HOST))
+
+"erk2_finalize_field! (timestep.jl:1449)"
+erk2_finalize_field!(b::HIPERK2Buffers, u_re, u_im, dt::Float64) = GC.@preserve u_re u_im check(ccall(
+    (:gd_erk2_finalize_field, libgdhip), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cdouble, Cint),
+    b.ptr, u_re, u_im, dt, GD_HOST))
+
 # ---------------------------------------------------------------- device-resident step
 struct GDStepDesc
     E::Cdouble; Pm::Cdouble; Pr::Cdouble; Ra::Cdouble; rratio::Cdouble; dt::Cdouble; theta::Cdouble

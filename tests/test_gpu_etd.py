@@ -151,7 +151,7 @@ def test_erk2_staged_matches_oracle(lmax, mmax, nr, dt, nu, device):
 
 
 def test_missing_l_raises():
-    """The reference errors on an l without a cache entry (timestep.jl:1366-1368, 446-447)."""
+    """The reference errors on an l without a cache entry (timestep.jl:1357, 1475, 447-449)."""
     import gdhip
 
     print("case: missing l", flush=True)
