@@ -26,7 +26,7 @@ template <int BW, int D, typename TS, bool DIRECT> __global__ void __launch_boun
     constexpr int W = 2 * BW + 1;
     const int n = c.n, nlm = c.nlm;
     const int idx = blockIdx.x * 64 + threadIdx.x;
-    const int f = blockIdx.y, ri = blockIdx.z; // field, 0 re / 1 im
+    const int f = c.f0 + blockIdx.y, ri = blockIdx.z; // field, 0 re / 1 im
     const int k = f < 2 ? 0 : (f < 4 ? 1 : 2);
     // level-uniform tables in LDS: the explicit operator by row (ER doubles per
     // row i: the coefficient of u row i-BW+q in slot q, 0 where that row does
@@ -251,7 +251,9 @@ int gd_launch_cnab2_chain(hipStream_t st, const ChainArgs &c, bool syn_fp64)
     GD_CHECK(!c.nl_zero_mask || c.zero_row, GD_ERR_ARG, "cnab2_chain: nl_zero_mask needs zero_row");
     GD_CHECK(!c.synV || (c.synS && c.D1 && c.D2 && c.rtab), GD_ERR_ARG, "cnab2_chain: incomplete synthesis outputs");
     GD_CHECK(gd_cnab2_chain_supported(c.n, c.bw), GD_ERR_ARG, "cnab2_chain: n=%d bw=%d unsupported", c.n, c.bw);
-    dim3 grid((c.nlm + 63) / 64, 5, 2);
+    GD_CHECK(c.f0 >= 0 && c.nf >= 0 && c.f0 + (c.nf ? c.nf : 5) <= 5, GD_ERR_ARG, "cnab2_chain: fields %d+%d",
+             c.f0, c.nf);
+    dim3 grid((c.nlm + 63) / 64, c.nf ? c.nf : 5, 2);
     const size_t lds = sizeof(double) * (size_t)((((2 * c.bw + 1) + 2) & ~1) + 2 * (2 * c.bw + 1) + 2) * c.n;
     GD_CHECK(lds <= 64 * 1024, GD_ERR_ARG, "cnab2_chain: n=%d bw=%d needs %zu B of LDS", c.n, c.bw, lds);
     const bool direct = c.synV && c.syn_tang;

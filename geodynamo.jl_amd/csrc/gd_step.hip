@@ -22,6 +22,7 @@
 //      (timestep.jl:610-730); the nl history rolls by buffer swap
 //      (simulation.jl:1312-1329).
 #include <cmath>
+#include <cstdlib>
 
 #include "gd_plan.hpp"
 #include "gd_radial.hpp"
@@ -46,6 +47,16 @@ struct gd_step {
     bool syn_ok = false; // synV/synS hold the synthesis columns of the current state
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    // second stream of the tangential step (default priority): the magnetic
+    // solves, then the scalar analysis and the temperature solve, run beside
+    // the synthesis / ring / vector analysis / velocity solves
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_ring = nullptr, ev_join = nullptr;
+    // 0: one stream; 1: magnetic and temperature solves and the scalar analysis on
+    // the side stream; 2: the scalar analysis only.  Measured at lmax 127, nr 65:
+    // 2 is fastest on the FP32 path (4848 vs 4638 steps/s), 0 on the FP64 path
+    // (2807 vs 2721); 1 is slower on both (three partial CNAB2 launches)
+    int overlap = 0;
 };
 
 // ---------------------------------------------------------------- spectral preparation
@@ -123,7 +134,9 @@ static double *cur_state(gd_step *s) { return s->state[s->step & 1].as<double>()
 // only u and grad T need transforms; full_products = 1 keeps every product.
 static bool tangential(const gd_step *s) { return !s->d.radials && !s->d.full_products; }
 
-template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEvent_t *ev)
+// side (tangential step only): the scalar analysis runs on s->side after the
+// ring kernel, the vector analysis on the main stream
+template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEvent_t *ev, bool side)
 {
     hipStream_t st = s->ctx->stream;
     GD_MARK(0);
@@ -188,6 +201,10 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEven
         }
         GD_TRY(gd_launch_ring<T>(st, RING_STEP_TANG, r));
         GD_MARK(4);
+        if (side) {
+            GD_HIP(hipEventRecord(s->ev_ring, st));
+            GD_HIP(hipStreamWaitEvent(s->side, s->ev_ring, 0));
+        }
         // force -> (pol, tor) of velocity nl (velocity.jl:331); the magnetic nl
         // is u x B = 0 here and its planes stay zero
         g.nslot_g = 12;
@@ -264,15 +281,15 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEven
     g.tab1 = nullptr;
     g.out[0] = nl_out + 8 * fs; // -u.grad T -> T
     g.scale[0] = 1;
-    GD_TRY(gd_launch_ana<T>(st, g, false, maxgroup));
+    GD_TRY(gd_launch_ana<T>(side ? s->side : st, g, false, maxgroup));
     GD_MARK(6);
     return GD_OK;
 }
 
-static int nonlinear(gd_step *s, double *nl_out, hipEvent_t *ev = nullptr)
+static int nonlinear(gd_step *s, double *nl_out, hipEvent_t *ev = nullptr, bool side = false)
 {
-    if (s->d.prec == GD_PREC_FP64) return nonlinear_T<double>(s, nl_out, ev);
-    return nonlinear_T<float>(s, nl_out, ev);
+    if (s->d.prec == GD_PREC_FP64) return nonlinear_T<double>(s, nl_out, ev, side);
+    return nonlinear_T<float>(s, nl_out, ev, side);
 }
 
 // fields whose implicit operator carries Dirichlet rows (bit f for field f)
@@ -283,14 +300,26 @@ static int bc_mask(const gd_step *s)
     return 0;
 }
 
-// one full step with the nl buffers chosen by parity; boot = AB1 bootstrap
+// one full step with the nl buffers chosen by parity; boot = AB1 bootstrap.
+// Tangential step with the chain kernel (and no per-class timing), s->overlap:
+//   2  the scalar analysis runs on the side stream beside the vector analysis
+//      (both read the ring output; they write disjoint nl planes), joined
+//      before the one CNAB2 launch;
+//   1  as 2, and the magnetic fields' solves (their nonlinear terms are
+//      identically zero, so they depend only on the previous state) run on the
+//      side stream from the start of the step, the temperature solve after the
+//      scalar analysis there, the velocity solves on the main stream.  Written
+//      buffers are disjoint between the streams (state fields, nl planes,
+//      synthesis columns of u and of T/r).
 static int enqueue_step(gd_step *s, int parity, bool boot, hipEvent_t *ev = nullptr)
 {
     double *cur = s->nlbuf[parity].as<double>();
     double *prev = boot ? cur : s->nlbuf[parity ^ 1].as<double>();
-    GD_TRY(nonlinear(s, cur, ev));
+    hipStream_t st = s->ctx->stream;
+    const bool side_ok = s->chain && tangential(s) && !ev && s->side;
+    const bool split = side_ok && s->overlap == 1, ana_side = side_ok && s->overlap >= 1;
+    ChainArgs c{};
     if (s->chain) {
-        ChainArgs c{};
         c.nuLap = s->nuLap.as<double>();
         c.rinv2 = s->rtab.as<double>() + s->d.nr;
         c.nus = s->nus.as<double>();
@@ -318,10 +347,38 @@ static int enqueue_step(gd_step *s, int parity, bool boot, hipEvent_t *ev = null
         c.synV = s->synV.p;
         c.synS = s->synS.p;
         c.syn_tang = tangential(s) ? 1 : 0;
-        GD_TRY(gd_launch_cnab2_chain(s->ctx->stream, c, s->d.prec == GD_PREC_FP64));
+    }
+    const bool fp64 = s->d.prec == GD_PREC_FP64;
+    if (split) {
+        GD_HIP(hipEventRecord(s->ev_fork, st));
+        GD_HIP(hipStreamWaitEvent(s->side, s->ev_fork, 0));
+        ChainArgs cb = c; // B_tor, B_pol
+        cb.f0 = 2;
+        cb.nf = 2;
+        GD_TRY(gd_launch_cnab2_chain(s->side, cb, fp64));
+    }
+    GD_TRY(nonlinear(s, cur, ev, ana_side));
+    if (s->chain) {
+        if (split) {
+            ChainArgs cu = c, ct = c; // u_tor, u_pol; T
+            cu.f0 = 0;
+            cu.nf = 2;
+            ct.f0 = 4;
+            ct.nf = 1;
+            GD_TRY(gd_launch_cnab2_chain(st, cu, fp64));
+            GD_TRY(gd_launch_cnab2_chain(s->side, ct, fp64));
+            GD_HIP(hipEventRecord(s->ev_join, s->side));
+            GD_HIP(hipStreamWaitEvent(st, s->ev_join, 0));
+        } else {
+            if (ana_side) { // the scalar analysis ran on the side stream
+                GD_HIP(hipEventRecord(s->ev_join, s->side));
+                GD_HIP(hipStreamWaitEvent(st, s->ev_join, 0));
+            }
+            GD_TRY(gd_launch_cnab2_chain(st, c, fp64));
+        }
         s->syn_ok = true; // synV/synS now describe state[parity ^ 1]
     } else {
-        GD_TRY(gd_launch_cnab2_step(s->ctx->stream, s->nuLap.as<double>(), s->rtab.as<double>() + s->d.nr,
+        GD_TRY(gd_launch_cnab2_step(st, s->nuLap.as<double>(), s->rtab.as<double>() + s->d.nr,
                                     s->nus.as<double>(), s->LUt.as<double>(), s->sht.d_l_of_idx.as<int>(), s->nl,
                                     s->d.nr, s->d.bw, s->nlm, s->d.dt, s->d.theta, bc_mask(s),
                                     s->state[parity].as<double>(), s->state[parity ^ 1].as<double>(), cur, prev));
@@ -348,6 +405,10 @@ static int step_release(gd_step *s)
                       &s->D1, &s->D2, &s->rtab, &s->synV,
                       &s->synS, &s->fb, &s->gb, &s->part, &s->eout, &s->scratch, &s->status, &s->zrow, &s->rdiag};
     for (DevBuf *b : bufs) b->release();
+    for (hipEvent_t *e : {&s->ev_fork, &s->ev_ring, &s->ev_join})
+        if (*e) (void)hipEventDestroy(*e);
+    if (s->side) (void)hipStreamDestroy(s->side);
+    s->side = nullptr;
     gd_sht_plan_release(&s->sht);
     return GD_OK;
 }
@@ -444,6 +505,13 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
     s->step = 0;
     s->chain = gd_cnab2_chain_supported(nr, bw) != 0;
     s->syn_ok = false;
+    GD_HIP(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
+    for (hipEvent_t *e : {&s->ev_fork, &s->ev_ring, &s->ev_join})
+        GD_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    // GD_STEP_OVERLAP = 0 / 1 / 2 overrides the stream layout (measurement aid)
+    s->overlap = d->prec == GD_PREC_FP32 ? 2 : 0;
+    const char *ov = getenv("GD_STEP_OVERLAP");
+    if (ov && ov[0] >= '0' && ov[0] <= '2') s->overlap = ov[0] - '0';
     return GD_OK;
 }
 
@@ -466,6 +534,7 @@ extern "C" int gd_step_destroy(gd_step *s)
     if (!s) return GD_OK;
     (void)hipSetDevice(s->ctx->device);
     (void)hipStreamSynchronize(s->ctx->stream);
+    if (s->side) (void)hipStreamSynchronize(s->side);
     step_release(s);
     delete s;
     return GD_OK;
