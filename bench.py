@@ -302,6 +302,77 @@ def exp_integrators(reps):
     return out
 
 
+C1 = dict(lmax=255, nlat=384, nlon=768, nr=97)
+
+
+def sht_c1(reps):
+    """Transform round trip at lmax = mmax = 255, 384 x 768, 97 levels: three
+    scalar fields and one (tor, pol) pair, seeded normal x (l+1)^-1.5,
+    synthesis then analysis of each, FP64 and FP32, device tensors.  Per
+    precision: relative L2 error of the recovered coefficients (worst field),
+    ms per round trip (events around reps round trips) and the Legendre work
+    rate: 14 scalar-equivalent contractions (3 + 3 scalar, 4 + 4 for the
+    pair) x 2 nlat nlm nr flops (the even/odd count) = 34.3 GFLOP per round
+    trip, over the whole round trip (FFT and layout passes included), against
+    the MFMA peak of the arithmetic type."""
+    import numpy as np
+    import torch
+
+    import gdhip
+
+    L, nlat, nlon, nr = C1["lmax"], C1["nlat"], C1["nlon"], C1["nr"]
+    out = {"workload": "SHT round trip lmax=mmax=255 nlat=384 nlon=768, 97 levels, 3 scalars + 1 (tor,pol) pair"}
+    st = gdhip.seeded_state(2025, L, nr, nfields=5, amp=1.0, power=1.5, zero_boundaries=False)
+    for prec, peak in (("fp64", PEAK["fp64"]), ("fp32", PEAK["fp32"])):
+        cfg = gdhip.create_shtnskit_config(L, nlat=nlat, nlon=nlon, precision=prec)
+        st[3:, :, cfg.l_values == 0] = 0.0
+        nlm = cfg.nlm
+
+        def z():
+            return torch.zeros((nr, nlm), dtype=torch.float64, device="cuda")
+
+        def ph():
+            return gdhip.SHTnsPhysicalField(cfg, nr, data=torch.empty((nr, nlon, nlat), dtype=torch.float64,
+                                                                      device="cuda"))
+
+        ins = [gdhip.SHTnsSpectralField(cfg, nr, torch.from_numpy(np.ascontiguousarray(st[f].real)).cuda(),
+                                        torch.from_numpy(np.ascontiguousarray(st[f].imag)).cuda()) for f in range(5)]
+        outs = [gdhip.SHTnsSpectralField(cfg, nr, z(), z()) for _ in range(5)]
+        phs = [ph() for _ in range(3)]
+        vec = gdhip.SHTnsVectorField(cfg, nr, comps=[ph(), ph(), ph()])
+
+        def round_trip():
+            for f in range(3):
+                gdhip.shtnskit_spectral_to_physical(ins[f], phs[f])
+                gdhip.shtnskit_physical_to_spectral(phs[f], outs[f])
+            gdhip.shtnskit_vector_synthesis(ins[3], ins[4], vec)
+            gdhip.shtnskit_vector_analysis(vec, outs[3], outs[4])
+
+        s = torch.cuda.Stream()  # the library adopts this stream: the events bracket its work
+        with torch.cuda.stream(s):
+            round_trip()
+        torch.cuda.synchronize()
+        err = 0.0
+        for f in range(5):
+            d = (outs[f].data_real.cpu().numpy() - st[f].real) ** 2 + (outs[f].data_imag.cpu().numpy() - st[f].imag) ** 2
+            err = max(err, float(np.sqrt(d.sum() / (np.abs(st[f]) ** 2).sum())))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(s):
+            e0.record(s)
+            for _ in range(reps):
+                round_trip()
+            e1.record(s)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        gf = 14 * 2.0 * nlat * nlm * nr / 1e9
+        out[prec] = {"round_trip_rel_l2_max": float("%.3e" % err), "ms_per_round_trip": round(ms, 3),
+                     "legendre_gflop": round(gf, 2), "legendre_tflops": round(gf / ms, 2),
+                     "fraction_of_mfma_peak": round(gf / ms / peak, 4)}
+        del ins, outs, phs, vec, cfg
+        torch.cuda.empty_cache()
+    return out
+
+
 def copy_probe(nbytes):
     """One plain copy kernel (copy_kernel) of nbytes, outside the timed region."""
     import ctypes
@@ -334,6 +405,8 @@ def main():
     ap.add_argument("--no-exp", action="store_true", help="skip the EAB2 / ERK2 measurements")
     ap.add_argument("--exp-reps", type=int, default=20)
     ap.add_argument("--only-exp", action="store_true", help="print only the EAB2 / ERK2 measurements")
+    ap.add_argument("--no-c1", action="store_true", help="skip the lmax=255 transform round trip")
+    ap.add_argument("--only-c1", action="store_true", help="print only the lmax=255 transform round trip")
     ap.add_argument("--derive-traffic", metavar="DIR",
                     help="only turn rocprofv3 FETCH_SIZE/WRITE_SIZE runs under DIR into the traffic JSON")
     ap.add_argument("--summarize-trace", metavar="DIR",
@@ -352,12 +425,15 @@ def main():
 
     import numpy as np
 
-    if not args.no_exp:
+    if not (args.no_exp and args.no_c1):
         # torch's bundled HIP runtime must be the process's HIP runtime before
         # libgdhip loads, or the two do not see the same device
         import torch  # noqa: F401
     if args.only_exp:
         print(json.dumps(exp_integrators(args.exp_reps)), flush=True)
+        return
+    if args.only_c1:
+        print(json.dumps(sht_c1(3)), flush=True)
         return
 
     import gdhip
@@ -439,6 +515,11 @@ def main():
             out["exp_integrators"] = exp_integrators(args.exp_reps)
         except Exception as exc:  # a secondary measurement never hides the headline line
             out["exp_integrators"] = {"error": repr(exc)[:300]}
+    if not args.no_c1:
+        try:
+            out["sht_c1"] = sht_c1(3)
+        except Exception as exc:  # a secondary measurement never hides the headline line
+            out["sht_c1"] = {"error": repr(exc)[:300]}
     if not args.no_cpu_baseline:
         cb, err64 = cpu_baseline(st, args.cpu_steps, one_step64)
         out["cpu_baseline"] = cb

@@ -74,7 +74,10 @@ extern "C" int gd_ctx_create(int device, gd_ctx **out)
     GD_HIP(hipSetDevice(device));
     gd_ctx *c = new gd_ctx();
     c->device = device;
-    hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+    // a blocking stream (default priority): ordered with the legacy default
+    // stream, so a caller whose arrays come from work on that stream (a framework
+    // on its default stream) needs no extra synchronisation
+    hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamDefault);
     if (e != hipSuccess) {
         delete c;
         gd_set_error("hipStreamCreate: %s", hipGetErrorString(e));

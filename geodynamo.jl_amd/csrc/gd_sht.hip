@@ -508,10 +508,14 @@ template <typename T, bool VEC, int NF> __global__ void __launch_bounds__(64 * N
 // MFMA that used it has issued, so the next tile's loads overlap this tile's
 // MFMAs.  STAGE: results go through a per-wave LDS tile and each output
 // column's consecutive idx are stored contiguously.
-constexpr int ANA_WAVES = 4;
+// waves per analysis block (8 was measured slower at lmax 127 for FP32, 41 ->
+// 48 us: a wave's next-tile operand prefetch then spans fewer tiles; for FP64
+// it would exceed the register budget of a 512-thread block)
+template <typename T> constexpr int ana_waves() { return 4; }
 template <typename T, bool VEC, int NF, int KN>
-__global__ void __launch_bounds__(64 * ANA_WAVES) ana_res_kernel(AnaArgs a)
+__global__ void __launch_bounds__(64 * ana_waves<T>()) ana_res_kernel(AnaArgs a)
 {
+    constexpr int ANA_WAVES = ana_waves<T>();
     constexpr int NT = VEC ? 2 : 1;
     constexpr int VC = VEC ? 4 * NF : 2 * NF;
     constexpr int LT = 16 / VC; // levels per wave tile (16 columns)
@@ -537,6 +541,8 @@ __global__ void __launch_bounds__(64 * ANA_WAVES) ana_res_kernel(AnaArgs a)
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const T *gb = static_cast<const T *>(a.gb);
     const T *tab0 = static_cast<const T *>(a.tab0), *tab1 = static_cast<const T *>(a.tab1);
+    const int k0 = a.k0, nk = a.nk; // this launch's node chunk
+    const bool accum = a.accum != 0;
 
     if (tid < TM) {
         const int l = row0 + tid < Kp ? a.l_of_idx[kbase + row0 + tid] : 0;
@@ -551,7 +557,7 @@ __global__ void __launch_bounds__(64 * ANA_WAVES) ana_res_kernel(AnaArgs a)
             const int t = e / (TM * KN), rem = e - t * (TM * KN);
             const int i = rem / KN, k = rem - i * KN;
             T v = T(0);
-            if (row0 + i < Kp && k < a.nlat2) v = (t == 0 ? tab0 : tab1)[(size_t)(kbase + row0 + i) * a.nlat2 + k];
+            if (row0 + i < Kp && k < nk) v = (t == 0 ? tab0 : tab1)[(size_t)(kbase + row0 + i) * a.nlat2 + k0 + k];
             ra[q] = v;
         }
 #pragma unroll
@@ -593,7 +599,7 @@ __global__ void __launch_bounds__(64 * ANA_WAVES) ana_res_kernel(AnaArgs a)
     auto gload = [&](int tile, int t, int ks) {
         const int r = tile * LT + lv, k = node(ks);
         T v = T(0);
-        if (tile < ntiles && r < a.nr && k < a.nlat2) v = gm[(size_t)k * nstride + (size_t)r * a.nslot_g + slot[t]];
+        if (tile < ntiles && r < a.nr && k < nk) v = gm[(size_t)(k0 + k) * nstride + (size_t)r * a.nslot_g + slot[t]];
         return v;
     };
     T b[NT][KS];
@@ -658,7 +664,8 @@ __global__ void __launch_bounds__(64 * ANA_WAVES) ana_res_kernel(AnaArgs a)
                         if (sc == 0) val *= sinv[row];
                         else if (sc == 2) val = -val;
                         if (m == 0 && reim) val = 0.0; // imag(m=0) := 0 (shtnskit_field_functions.jl:432-435)
-                        a.out[outk][(size_t)reim * fs + (size_t)r * a.nlm + idx] = val;
+                        double *dst = a.out[outk] + (size_t)reim * fs + (size_t)r * a.nlm + idx;
+                        *dst = accum ? *dst + val : val;
                     }
                 }
             }
@@ -690,7 +697,8 @@ __global__ void __launch_bounds__(64 * ANA_WAVES) ana_res_kernel(AnaArgs a)
                     if (sc == 0) val *= sinv[row];
                     else if (sc == 2) val = -val;
                     if (m == 0 && reim) val = 0.0; // imag(m=0) := 0 (shtnskit_field_functions.jl:432-435)
-                    a.out[outk][(size_t)reim * fs + (size_t)r2 * a.nlm + idx] = val;
+                    double *dst = a.out[outk] + (size_t)reim * fs + (size_t)r2 * a.nlm + idx;
+                    *dst = accum ? *dst + val : val;
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -702,7 +710,7 @@ __global__ void __launch_bounds__(64 * ANA_WAVES) ana_res_kernel(AnaArgs a)
 template <typename T, bool VEC, int NF, int KN> constexpr size_t ana_res_lds()
 {
     constexpr int NT = VEC ? 2 : 1;
-    return sizeof(T) * ((size_t)NT * KN * TMA + (sizeof(T) == 4 ? (size_t)ANA_WAVES * 16 * (TM + 1) : 0));
+    return sizeof(T) * ((size_t)NT * KN * TMA + (sizeof(T) == 4 ? (size_t)ana_waves<T>() * 16 * (TM + 1) : 0));
 }
 
 } // namespace
@@ -751,12 +759,23 @@ template <typename T> int gd_launch_synth_scal(hipStream_t st, const SynthArgs &
 
 // grid rows cover the largest parity group (maxgroup = ceil((lmax+1)/2))
 // table-resident analysis for nlat/2 <= KN; false when no instance applies
-template <typename T, bool VEC, int NF, int KN> static bool try_ana_res(hipStream_t st, const AnaArgs &a)
+// nodes in chunks of at most KN: the first launch stores, the others add (the
+// sum over nodes is linear; the per-row factors, signs and imag(m=0) := 0
+// apply to every partial alike)
+template <typename T, bool VEC, int NF, int KN> static bool try_ana_res(hipStream_t st, const AnaArgs &a0)
 {
     constexpr size_t lds = ana_res_lds<T, VEC, NF, KN>();
-    if (a.nlat2 > KN || lds > 64 * 1024) return false;
-    const int gx = (a.maxgroup + TM - 1) / TM;
-    hipLaunchKernelGGL((ana_res_kernel<T, VEC, NF, KN>), dim3(gx * 2 * (a.mmax + 1)), dim3(64 * ANA_WAVES), lds, st, a);
+    if (lds > 64 * 1024) return false;
+    const int gx = (a0.maxgroup + TM - 1) / TM;
+    const int nchunk = (a0.nlat2 + KN - 1) / KN;
+    for (int c = 0; c < nchunk; ++c) {
+        AnaArgs a = a0;
+        a.k0 = c * KN;
+        a.nk = min(KN, a0.nlat2 - a.k0);
+        a.accum = c > 0;
+        hipLaunchKernelGGL((ana_res_kernel<T, VEC, NF, KN>), dim3(gx * 2 * (a.mmax + 1)), dim3(64 * ana_waves<T>()),
+                           lds, st, a);
+    }
     return true;
 }
 
@@ -767,7 +786,7 @@ template <typename T, bool VEC, int NF> static bool ana_res(hipStream_t st, cons
     if (a.nlat2 <= 96) return try_ana_res<T, VEC, NF, 96>(st, a);
     if constexpr (sizeof(T) == 4) // the FP64 operand registers of a 128-node tile would spill
         if (a.nlat2 <= 128) return try_ana_res<T, VEC, NF, 128>(st, a);
-    return false;
+    return try_ana_res<T, VEC, NF, 96>(st, a); // node chunks of 96
 }
 
 template <typename T> static int launch_ana(hipStream_t st, const AnaArgs &a, bool vec, int maxgroup)
@@ -1036,7 +1055,19 @@ template <typename T, int MODE, int NC> __global__ void __launch_bounds__(256) r
     C2<T> *buf = tw + n;
     __shared__ int srad[12];
     __shared__ int sfn[16], sfs[16];
-    const int i = blockIdx.x, r = blockIdx.y;
+    // The boundary layouts (RING_SYNTH writes, RING_ANALYSIS reads) are
+    // colatitude-fastest: a 128-byte line holds 16 consecutive rings of one
+    // (longitude, level).  Consecutive blocks go to different XCDs (block b to
+    // XCD b mod 8, each with its own L2), so those modes renumber the blocks
+    // for each XCD to take a contiguous range of (level, ring) and a line's
+    // pieces meet in one L2.  The step modes read and write ring-contiguous
+    // records and keep the plain order.
+    int i = blockIdx.x, r = blockIdx.y;
+    if (MODE == RING_SYNTH || MODE == RING_ANALYSIS) {
+        const int L = xcd_logical(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+        i = L % gridDim.x;
+        r = L / gridDim.x;
+    }
     const int tid = threadIdx.x;
     constexpr int nthr = RING_THREADS;
     const int is = a.nlat - 1 - i; // southern mirror
@@ -1242,6 +1273,7 @@ template <typename T, int MODE> static void ring_dispatch(hipStream_t st, dim3 g
     case 192: ring_launch<T, MODE, 192>(st, grid, lds, a); break;
     case 288: ring_launch<T, MODE, 288>(st, grid, lds, a); break;
     case 384: ring_launch<T, MODE, 384>(st, grid, lds, a); break;
+    case 768: ring_launch<T, MODE, 768>(st, grid, lds, a); break;
     default: ring_launch<T, MODE, 0>(st, grid, lds, a); break;
     }
 }

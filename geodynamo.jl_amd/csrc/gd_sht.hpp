@@ -40,6 +40,9 @@ struct AnaArgs {
     // outputs, FP64 m-major [reim][level][idx]; vector field V: out[2V] gets S, out[2V+1] gets T
     double *out[4];
     int scale[4];         // 0: /l(l+1) (0 at l=0), 1: x1, 2: x(-1)
+    // table-resident analysis in node chunks (set by its launcher): nodes
+    // k0 .. k0 + nk - 1 (nk = 0: all), accum = 1 adds to out instead of storing
+    int k0, nk, accum;
 };
 
 struct RingArgs {
