@@ -302,6 +302,57 @@ def exp_integrators(reps):
     return out
 
 
+def batch_c3(nshell, steps, nstreams):
+    """Uneven batch: nshell independent shells, lmax drawn from {31, 47, 63, 95,
+    127}, nr from {33, 49, 65, 97}, grid 3(L+1)/2 x 3(L+1), radius ratio
+    U[0.2, 0.5] (numpy seed 3), FP64 steps, (l+1)^-2 states of amplitude
+    1e-3.  Aggregate shell-steps/s of the batch graph (every shell's step in
+    one graph, shells spread over nstreams streams) against the same shells
+    advanced one after another by their own step graphs."""
+    import numpy as np
+
+    import gdhip
+
+    rng = np.random.default_rng(3)
+    shells = []
+    for k in range(nshell):
+        L = int(rng.choice([31, 47, 63, 95, 127]))
+        nr = int(rng.choice([33, 49, 65, 97]))
+        shells.append((L, nr, 3 * (L + 1) // 2, 3 * (L + 1), float(rng.uniform(0.2, 0.5))))
+    groups = {"batch": [], "sequential": []}
+    for k, (L, nr, nlat, nlon, rr) in enumerate(shells):
+        st = gdhip.seeded_state(500 + k, L, nr, amp=1e-3)
+        for g in groups.values():
+            s = gdhip.CNAB2Step(L, nlat, nlon, nr, rratio=rr, Ra=500.0, dt=1e-5, precision="fp64")
+            s.set_state_complex(st)
+            g.append(s)
+    batch = gdhip.StepBatch(groups["batch"], nstreams=nstreams)
+    batch.run(3)  # bootstrap, graph capture of both parities
+    for s in groups["sequential"]:
+        s.run(3, graph=True)
+    batch.synchronize()
+    t0 = time.perf_counter()
+    batch.run(steps)
+    batch.synchronize()
+    tb = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for s in groups["sequential"]:
+            s.run(1, graph=True)
+    groups["sequential"][0].synchronize()
+    ts = time.perf_counter() - t0
+    same = all(np.array_equal(a.get_state_complex(), b.get_state_complex())
+               for a, b in zip(groups["batch"], groups["sequential"]))
+    batch.close()
+    return {"workload": "%d uneven shells, lmax {31,47,63,95,127}, nr {33,49,65,97}, 3(L+1)/2 x 3(L+1), "
+                        "FP64 steps" % nshell,
+            "shells_lmax_nr": [[L, nr] for L, nr, _, _, _ in shells],
+            "batch_shell_steps_per_s": round(nshell * steps / tb, 1),
+            "sequential_shell_steps_per_s": round(nshell * steps / ts, 1),
+            "batch_over_sequential": round(ts / tb, 3), "streams": nstreams, "steps": steps,
+            "states_equal_bitwise": same}
+
+
 C1 = dict(lmax=255, nlat=384, nlon=768, nr=97)
 
 
@@ -406,6 +457,8 @@ def main():
     ap.add_argument("--exp-reps", type=int, default=20)
     ap.add_argument("--only-exp", action="store_true", help="print only the EAB2 / ERK2 measurements")
     ap.add_argument("--no-c1", action="store_true", help="skip the lmax=255 transform round trip")
+    ap.add_argument("--no-c3", action="store_true", help="skip the uneven shell batch")
+    ap.add_argument("--only-c3", action="store_true", help="print only the uneven shell batch")
     ap.add_argument("--only-c1", action="store_true", help="print only the lmax=255 transform round trip")
     ap.add_argument("--derive-traffic", metavar="DIR",
                     help="only turn rocprofv3 FETCH_SIZE/WRITE_SIZE runs under DIR into the traffic JSON")
@@ -434,6 +487,9 @@ def main():
         return
     if args.only_c1:
         print(json.dumps(sht_c1(3)), flush=True)
+        return
+    if args.only_c3:
+        print(json.dumps(batch_c3(64, 20, 8)), flush=True)
         return
 
     import gdhip
@@ -520,6 +576,11 @@ def main():
             out["sht_c1"] = sht_c1(3)
         except Exception as exc:  # a secondary measurement never hides the headline line
             out["sht_c1"] = {"error": repr(exc)[:300]}
+    if not args.no_c3:
+        try:
+            out["batch_c3"] = batch_c3(64, 20, 8)
+        except Exception as exc:  # a secondary measurement never hides the headline line
+            out["batch_c3"] = {"error": repr(exc)[:300]}
     if not args.no_cpu_baseline:
         cb, err64 = cpu_baseline(st, args.cpu_steps, one_step64)
         out["cpu_baseline"] = cb

@@ -701,3 +701,140 @@ extern "C" int gd_step_profile(gd_step *s, int nsteps, double *out)
     if (rc == GD_ERR_HIP) gd_set_error("gd_step_profile: event failure");
     return rc;
 }
+
+// ---------------------------------------------------------------- batches of independent shells
+// Several steps (shells of any lmax, nr, grid) advanced together: one graph
+// per parity holds every shell's step, the shells spread round-robin over the
+// batch's streams (forked from and joined to the context stream), so small
+// shells run side by side instead of one after another.  No counterpart in the
+// reference (its shells are separate runs).
+struct gd_batch {
+    gd_ctx *ctx = nullptr;
+    std::vector<gd_step *> steps;
+    std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> joins;
+    hipEvent_t fork = nullptr;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+};
+
+static void batch_release(gd_batch *b)
+{
+    for (int k = 0; k < 2; ++k) {
+        if (b->gexec[k]) (void)hipGraphExecDestroy(b->gexec[k]);
+        if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
+    }
+    for (hipEvent_t e : b->joins)
+        if (e) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    for (hipStream_t s : b->streams)
+        if (s) (void)hipStreamDestroy(s);
+}
+
+extern "C" int gd_batch_create(gd_ctx *ctx, int n, gd_step *const *steps, int nstreams, gd_batch **out)
+{
+    GD_CHECK(ctx && steps && out && n >= 1 && nstreams >= 1 && nstreams <= 64, GD_ERR_ARG,
+             "gd_batch_create: bad arguments");
+    for (int i = 0; i < n; ++i)
+        GD_CHECK(steps[i] && steps[i]->ctx == ctx, GD_ERR_ARG, "gd_batch_create: step %d is null or on another context",
+                 i);
+    GD_HIP(hipSetDevice(ctx->device));
+    gd_batch *b = new gd_batch();
+    b->ctx = ctx;
+    b->steps.assign(steps, steps + n);
+    hipError_t e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+    for (int k = 0; k < nstreams && e == hipSuccess; ++k) {
+        hipStream_t s = nullptr;
+        hipEvent_t j = nullptr;
+        e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking); // default priority
+        if (e == hipSuccess) b->streams.push_back(s);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&j, hipEventDisableTiming);
+        if (e == hipSuccess) b->joins.push_back(j);
+    }
+    if (e != hipSuccess) {
+        gd_set_error("gd_batch_create: %s", hipGetErrorString(e));
+        batch_release(b);
+        delete b;
+        return GD_ERR_HIP;
+    }
+    *out = b;
+    return GD_OK;
+}
+
+extern "C" int gd_batch_destroy(gd_batch *b)
+{
+    if (!b) return GD_OK;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    batch_release(b);
+    delete b;
+    return GD_OK;
+}
+
+// every shell's step of the given parity, forked over the batch streams
+static int batch_enqueue(gd_batch *b, int parity)
+{
+    hipStream_t main = b->ctx->stream;
+    const int ns = (int)b->streams.size();
+    GD_HIP(hipEventRecord(b->fork, main));
+    for (int k = 0; k < ns; ++k) GD_HIP(hipStreamWaitEvent(b->streams[k], b->fork, 0));
+    int rc = GD_OK;
+    for (size_t i = 0; i < b->steps.size() && rc == GD_OK; ++i) {
+        gd_step *st = b->steps[i];
+        b->ctx->stream = b->streams[i % ns]; // the step's launches follow the context stream
+        // no second stream inside a batch stream: a capture that forked each
+        // FP32 shell's side stream from its batch stream ended in a host
+        // segmentation fault in this function; the cause was not found, so
+        // batches keep every shell's step on its batch stream
+        const int ov = st->overlap;
+        st->overlap = 0;
+        rc = enqueue_step(st, parity, false);
+        st->overlap = ov;
+    }
+    b->ctx->stream = main;
+    GD_TRY(rc);
+    for (int k = 0; k < ns; ++k) {
+        GD_HIP(hipEventRecord(b->joins[k], b->streams[k]));
+        GD_HIP(hipStreamWaitEvent(main, b->joins[k], 0));
+    }
+    return GD_OK;
+}
+
+extern "C" int gd_batch_run(gd_batch *b, int nsteps)
+{
+    GD_CHECK(b && nsteps >= 0, GD_ERR_ARG, "gd_batch_run: bad args");
+    GD_HIP(hipSetDevice(b->ctx->device));
+    const int s0 = b->steps[0]->step;
+    for (gd_step *s : b->steps) GD_CHECK(s->step == s0, GD_ERR_ARG, "gd_batch_run: shells at different steps");
+    for (int k = 0; k < nsteps; ++k) {
+        const int step = b->steps[0]->step;
+        if (step == 0) {
+            for (gd_step *s : b->steps) { // AB1 bootstraps, eager, one stream per shell as in the graphs
+                const int ov = s->overlap;
+                s->overlap = 0;
+                const int rc = enqueue_step(s, 0, true);
+                s->overlap = ov;
+                GD_TRY(rc);
+            }
+        } else {
+            const int par = step & 1;
+            if (!b->gexec[par]) {
+                hipStream_t main = b->ctx->stream;
+                GD_HIP(hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal));
+                const int rc = batch_enqueue(b, par);
+                hipGraph_t g = nullptr;
+                const hipError_t e = hipStreamEndCapture(main, &g);
+                GD_TRY(rc);
+                if (e != hipSuccess) {
+                    gd_set_error("gd_batch_run: hipStreamEndCapture: %s", hipGetErrorString(e));
+                    return GD_ERR_HIP;
+                }
+                b->graph[par] = g;
+                GD_HIP(hipGraphInstantiate(&b->gexec[par], g, nullptr, nullptr, 0));
+            }
+            GD_HIP(hipGraphLaunch(b->gexec[par], b->ctx->stream));
+        }
+        for (gd_step *s : b->steps) ++s->step;
+    }
+    return GD_OK;
+}

@@ -16,7 +16,7 @@ from .shtnskit_field_functions import (batch_shtnskit_transforms, batch_spectral
                                        shtnskit_physical_to_spectral, shtnskit_spectral_to_physical,
                                        shtnskit_vector_analysis, shtnskit_vector_synthesis)
 from .shtnskit_transforms import SHTnsKitConfig, create_shtnskit_config  # noqa: F401
-from .step import CNAB2Step  # noqa: F401
+from .step import CNAB2Step, StepBatch  # noqa: F401
 from .timestep import (ERK2Cache, ERK2FieldBuffers, ETDCache, SHTnsImplicitMatrices,  # noqa: F401
                        apply_explicit_operator, build_rhs_cnab2, compute_timestep_error, create_erk2_cache,
                        create_etd_cache, create_shtns_timestepping_matrices, eab2_update, erk2_apply_stage,

@@ -32,6 +32,7 @@ EXPORTS = [
     "gd_erk2_cache_create", "gd_erk2_cache_from_matrices", "gd_erk2_cache_get", "gd_erk2_cache_destroy",
     "gd_erk2_buffers_create", "gd_erk2_buffers_destroy", "gd_erk2_prepare_field", "gd_erk2_apply_stage",
     "gd_erk2_store_stage_nonlinear", "gd_erk2_finalize_field",
+    "gd_batch_create", "gd_batch_run", "gd_batch_destroy",
 ]
 
 
@@ -123,6 +124,9 @@ def lib():
             "gd_erk2_apply_stage": (i, [vp, vp, vp, i]),
             "gd_erk2_store_stage_nonlinear": (i, [vp, vp, vp, i]),
             "gd_erk2_finalize_field": (i, [vp, vp, vp, d, i]),
+            "gd_batch_create": (i, [vp, i, vp, i, pp]),
+            "gd_batch_run": (i, [vp, i]),
+            "gd_batch_destroy": (i, [vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

@@ -91,3 +91,38 @@ class CNAB2Step:
             self.close()
         except Exception:
             pass
+
+
+class StepBatch:
+    """Independent shells advanced together: one graph per step parity holds
+    every shell's step, spread over `nstreams` streams (gd_batch_*).  The
+    shells must be at the same step count; results equal each shell's own
+    run(graph=True)."""
+
+    def __init__(self, steps, nstreams=8):
+        if not steps:
+            raise ValueError("empty batch")
+        self.steps = list(steps)
+        arr = (ctypes.c_void_p * len(self.steps))(*[s._h.value for s in self.steps])
+        self._arr = arr
+        self._h = ctypes.c_void_p()
+        check(lib().gd_batch_create(context(self.steps[0].device), len(self.steps), arr, int(nstreams),
+                                    ctypes.byref(self._h)))
+
+    def run(self, nsteps=1):
+        check(lib().gd_batch_run(self._h, nsteps))
+
+    def synchronize(self):
+        check(lib().gd_ctx_synchronize(context(self.steps[0].device)))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().gd_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

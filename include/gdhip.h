@@ -191,6 +191,17 @@ int gd_step_energies(gd_step *s, double *out3);
  * analysis (vector), 5 Legendre analysis (scalar), 6 CNAB2 rhs + banded solve. */
 #define GD_NKCLASS 7
 int gd_step_profile(gd_step *s, int nsteps, double *out);
+/* Batches of independent shells (no counterpart in the reference, whose shells
+ * are separate runs): steps of any (lmax, nr, grid) on one context advanced
+ * together.  Each graph replay holds every shell's step, the shells spread
+ * round-robin over nstreams streams forked from and joined to the context
+ * stream.  All shells must be at the same step count; step 0 (the AB1
+ * bootstrap) runs eagerly.  A shell's results equal its own gd_step_run_graph. */
+typedef struct gd_batch gd_batch;
+int gd_batch_create(gd_ctx *ctx, int n, gd_step *const *steps, int nstreams, gd_batch **out);
+int gd_batch_run(gd_batch *b, int nsteps);
+int gd_batch_destroy(gd_batch *b);
+
 /* replaces compute_timestep_error (timestep.jl:733-753) on device arrays */
 int gd_timestep_error(gd_ctx *ctx, long n, const double *a_re, const double *a_im,
                       const double *b_re, const double *b_im, double *out, int flags);
