@@ -57,6 +57,7 @@ struct gd_step {
     // 2 is fastest on the FP32 path (4848 vs 4638 steps/s), 0 on the FP64 path
     // (2807 vs 2721); 1 is slower on both (three partial CNAB2 launches)
     int overlap = 0;
+    int yreg = 1; // CNAB2 kernel: forward-substitution result in registers where supported
 };
 
 // ---------------------------------------------------------------- spectral preparation
@@ -136,19 +137,35 @@ static bool tangential(const gd_step *s) { return !s->d.radials && !s->d.full_pr
 
 // side (tangential step only): the scalar analysis runs on s->side after the
 // ring kernel, the vector analysis on the main stream
+// the Legendre-synthesis columns of the current state
+template <typename T> static int spectral_pre(gd_step *s)
+{
+    gd_sht_plan *p = &s->sht;
+    const int nr = s->d.nr, nlm = s->nlm;
+    hipLaunchKernelGGL((spectral_pre_kernel<T>), dim3((nlm + 255) / 256, nr), dim3(256), 0, s->ctx->stream,
+                       cur_state(s), p->d_l_of_idx.as<int>(), s->D1.as<double>(), s->D2.as<double>(),
+                       s->rtab.as<double>(), nlm, nr, s->d.bw, s->d.radials, s->synV.as<T>(), s->synS.as<T>());
+    GD_TRY(gd_check_launch("spectral_pre_kernel"));
+    s->syn_ok = true;
+    return GD_OK;
+}
+
+// Graphs of the chain path are captured once the synthesis columns are valid
+// and do not contain the spectral preparation; after a restore the columns
+// are stale, so they are rebuilt eagerly before the next replay.
+static int ensure_syn(gd_step *s)
+{
+    if (s->syn_ok || !s->chain) return GD_OK;
+    return s->d.prec == GD_PREC_FP64 ? spectral_pre<double>(s) : spectral_pre<float>(s);
+}
+
 template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEvent_t *ev, bool side)
 {
     hipStream_t st = s->ctx->stream;
     GD_MARK(0);
     gd_sht_plan *p = &s->sht;
     const int nr = s->d.nr, nlm = s->nlm;
-    if (!s->syn_ok) {
-        hipLaunchKernelGGL((spectral_pre_kernel<T>), dim3((nlm + 255) / 256, nr), dim3(256), 0, st,
-                           cur_state(s), p->d_l_of_idx.as<int>(), s->D1.as<double>(), s->D2.as<double>(),
-                           s->rtab.as<double>(), nlm, nr, s->d.bw, s->d.radials, s->synV.as<T>(), s->synS.as<T>());
-        GD_TRY(gd_check_launch("spectral_pre_kernel"));
-        s->syn_ok = true;
-    }
+    if (!s->syn_ok) GD_TRY(spectral_pre<T>(s));
     GD_MARK(1);
 
     SynthArgs a{};
@@ -347,6 +364,7 @@ static int enqueue_step(gd_step *s, int parity, bool boot, hipEvent_t *ev = null
         c.synV = s->synV.p;
         c.synS = s->synS.p;
         c.syn_tang = tangential(s) ? 1 : 0;
+        c.yreg = s->yreg;
     }
     const bool fp64 = s->d.prec == GD_PREC_FP64;
     if (split) {
@@ -512,6 +530,10 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
     s->overlap = d->prec == GD_PREC_FP32 ? 2 : 0;
     const char *ov = getenv("GD_STEP_OVERLAP");
     if (ov && ov[0] >= '0' && ov[0] <= '2') s->overlap = ov[0] - '0';
+    // GD_CHAIN_YREG = 0 keeps the forward-substitution result of the CNAB2
+    // kernel in HBM also where a register-resident kernel exists (same values)
+    const char *yr = getenv("GD_CHAIN_YREG");
+    s->yreg = !(yr && yr[0] == '0');
     return GD_OK;
 }
 
@@ -603,6 +625,46 @@ extern "C" int gd_step_get_state(gd_step *s, double *re, double *im, int flags)
     return get_planes(s, cur_state(s), re, im, flags);
 }
 
+// the AB2 history: the nonlinear terms of the previous step, which the next
+// step combines with its own (simulation.jl:1312-1329 keeps them as
+// prev_nonlinear)
+static double *history(gd_step *s) { return s->nlbuf[(s->step & 1) ^ 1].as<double>(); }
+
+extern "C" int gd_step_get_history(gd_step *s, double *nl_re, double *nl_im, int flags)
+{
+    GD_CHECK(s && nl_re && nl_im, GD_ERR_ARG, "gd_step_get_history: null argument");
+    GD_CHECK(s->step > 0, GD_ERR_ARG, "gd_step_get_history: no nonlinear history before the first step");
+    return get_planes(s, history(s), nl_re, nl_im, flags);
+}
+
+extern "C" int gd_step_restore(gd_step *s, const double *re, const double *im, const double *nl_re,
+                               const double *nl_im, int step, int flags)
+{
+    GD_CHECK(s && re && im, GD_ERR_ARG, "gd_step_restore: null argument");
+    GD_CHECK(step >= 0, GD_ERR_ARG, "gd_step_restore: step %d < 0", step);
+    GD_CHECK(step == 0 || (nl_re && nl_im), GD_ERR_ARG, "gd_step_restore: step %d > 0 needs the nonlinear history",
+             step);
+    GD_HIP(hipSetDevice(s->ctx->device));
+    const size_t n = s->fs * 5;
+    const int nplanes = step > 0 ? 4 : 2;
+    const double *src[4] = {re, im, nl_re, nl_im};
+    const double *dev[4] = {re, im, nl_re, nl_im};
+    if (flags & GD_HOST) {
+        GD_TRY(s->scratch.ensure(sizeof(double) * n * nplanes));
+        for (int k = 0; k < nplanes; ++k) {
+            dev[k] = s->scratch.as<double>() + k * n;
+            GD_HIP(hipMemcpyAsync(const_cast<double *>(dev[k]), src[k], sizeof(double) * n, hipMemcpyHostToDevice,
+                                  s->ctx->stream));
+        }
+    }
+    s->step = step;
+    s->syn_ok = false;
+    GD_TRY(convert_state(s, dev[0], dev[1], cur_state(s), true));
+    if (step > 0) GD_TRY(convert_state(s, dev[2], dev[3], history(s), true));
+    if (flags & GD_HOST) GD_HIP(hipStreamSynchronize(s->ctx->stream));
+    return GD_OK;
+}
+
 extern "C" int gd_step_nonlinear(gd_step *s, double *nl_re, double *nl_im, int flags)
 {
     GD_CHECK(s && nl_re && nl_im, GD_ERR_ARG, "gd_step_nonlinear: null argument");
@@ -650,6 +712,7 @@ extern "C" int gd_step_run_graph(gd_step *s, int nsteps)
             GD_TRY(enqueue_step(s, 0, true)); // AB1 bootstrap is not part of the graphs
         } else {
             const int par = s->step & 1;
+            GD_TRY(ensure_syn(s));
             if (!s->gexec[par]) GD_TRY(capture(s, par));
             GD_HIP(hipGraphLaunch(s->gexec[par], s->ctx->stream));
         }
@@ -659,6 +722,26 @@ extern "C" int gd_step_run_graph(gd_step *s, int nsteps)
 }
 
 extern "C" int gd_step_count(const gd_step *s) { return s ? s->step : -1; }
+
+extern "C" int gd_step_shape(const gd_step *s, int *lmax, int *mmax, int *nr)
+{
+    GD_CHECK(s && lmax && mmax && nr, GD_ERR_ARG, "gd_step_shape: null argument");
+    *lmax = s->d.lmax;
+    *mmax = s->d.mmax;
+    *nr = s->d.nr;
+    return GD_OK;
+}
+
+// the radial grid r_i of the step's shell (fields.jl:108-131), host out[nr]
+extern "C" int gd_step_radii(const gd_step *s, double *r)
+{
+    GD_CHECK(s && r, GD_ERR_ARG, "gd_step_radii: null argument");
+    const int nr = s->d.nr;
+    std::vector<double> t((size_t)nr * 7);
+    GD_TRY(gd_radial_grid(nr, s->d.rratio, t.data()));
+    for (int i = 0; i < nr; ++i) r[i] = t[(size_t)nr * 3 + i];
+    return GD_OK;
+}
 
 extern "C" int gd_step_energies(gd_step *s, double *out3)
 {
@@ -818,6 +901,7 @@ extern "C" int gd_batch_run(gd_batch *b, int nsteps)
             }
         } else {
             const int par = step & 1;
+            for (gd_step *s : b->steps) GD_TRY(ensure_syn(s));
             if (!b->gexec[par]) {
                 hipStream_t main = b->ctx->stream;
                 GD_HIP(hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal));

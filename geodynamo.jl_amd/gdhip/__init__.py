@@ -22,3 +22,6 @@ from .timestep import (ERK2Cache, ERK2FieldBuffers, ETDCache, SHTnsImplicitMatri
                        create_etd_cache, create_shtns_timestepping_matrices, eab2_update, erk2_apply_stage,
                        erk2_finalize_field, erk2_prepare_field, erk2_store_stage_nonlinear, exp_functions,
                        solve_implicit_step)
+from .outputs_writer import (RANK_TIME, TIME_RANK, OutputConfig, TimeTracker, find_restart_files,  # noqa: F401
+                             generate_filename, read_restart, read_restart_arrays, write_restart,
+                             write_restart_arrays)

@@ -33,6 +33,9 @@ EXPORTS = [
     "gd_erk2_buffers_create", "gd_erk2_buffers_destroy", "gd_erk2_prepare_field", "gd_erk2_apply_stage",
     "gd_erk2_store_stage_nonlinear", "gd_erk2_finalize_field",
     "gd_batch_create", "gd_batch_run", "gd_batch_destroy",
+    "gd_step_get_history", "gd_step_restore", "gd_step_shape", "gd_step_radii",
+    "gd_restart_write", "gd_restart_read", "gd_restart_write_arrays", "gd_restart_read_arrays",
+    "gd_restart_filename",
 ]
 
 
@@ -44,6 +47,13 @@ class GDError(RuntimeError):
 
 class MissingLError(GDError):
     pass
+
+
+class RestartMeta(ctypes.Structure):
+    _fields_ = [("time", ctypes.c_double), ("last_output_time", ctypes.c_double),
+                ("output_count", ctypes.c_int), ("restart_count", ctypes.c_int),
+                ("grid_file_written", ctypes.c_int), ("mpi_rank", ctypes.c_int),
+                ("mpi_total_ranks", ctypes.c_int), ("geometry", ctypes.c_char * 32)]
 
 
 class StepDesc(ctypes.Structure):
@@ -127,6 +137,17 @@ def lib():
             "gd_batch_create": (i, [vp, i, vp, i, pp]),
             "gd_batch_run": (i, [vp, i]),
             "gd_batch_destroy": (i, [vp]),
+            "gd_step_get_history": (i, [vp, vp, vp, i]),
+            "gd_step_restore": (i, [vp] * 5 + [i, i]),
+            "gd_step_shape": (i, [vp, vp, vp, vp]),
+            "gd_step_radii": (i, [vp, vp]),
+            "gd_restart_write": (i, [vp, ctypes.c_char_p, ctypes.POINTER(RestartMeta), i]),
+            "gd_restart_read": (i, [vp, ctypes.c_char_p, ctypes.POINTER(RestartMeta)]),
+            "gd_restart_write_arrays": (i, [ctypes.c_char_p, ctypes.POINTER(RestartMeta), i, i, i, vp, i]
+                                        + [vp] * 4 + [i]),
+            "gd_restart_read_arrays": (i, [ctypes.c_char_p, i, i, i, vp, vp] + [vp] * 4
+                                       + [vp, ctypes.POINTER(RestartMeta)]),
+            "gd_restart_filename": (i, [ctypes.c_char_p] * 4 + [d, i, i, ctypes.c_char_p, i]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

@@ -50,6 +50,33 @@ class CNAB2Step:
         re, im = self.get_state()
         return re + 1j * im
 
+    def get_history(self):
+        """AB2 history: the previous step's nonlinear terms (the reference's
+        prev_nonlinear, simulation.jl:1312-1329), complex (5, nr, nlm)."""
+        re, im = np.zeros(self.shape), np.zeros(self.shape)
+        ctx, flags, (a, b) = prepare(re, im, device=self.device)
+        check(lib().gd_step_get_history(self._h, a, b, flags))
+        return re + 1j * im
+
+    def restore(self, state, history=None, step=0):
+        """Sets state, AB2 history and step counter together (step 0: no history,
+        the next step is the AB1 bootstrap)."""
+        re, im = np.ascontiguousarray(state.real), np.ascontiguousarray(state.imag)
+        if step > 0:
+            if history is None:
+                raise ValueError("step > 0 needs the nonlinear history")
+            hre, him = np.ascontiguousarray(history.real), np.ascontiguousarray(history.imag)
+            ctx, flags, (a, b, c, d) = prepare(re, im, hre, him, device=self.device)
+        else:
+            ctx, flags, (a, b) = prepare(re, im, device=self.device)
+            c = d = None
+        check(lib().gd_step_restore(self._h, a, b, c, d, int(step), flags))
+
+    def radii(self):
+        r = np.zeros(self.nr)
+        check(lib().gd_step_radii(self._h, r.ctypes.data))
+        return r
+
     def nonlinear(self):
         re, im = np.zeros(self.shape), np.zeros(self.shape)
         ctx, flags, (a, b) = prepare(re, im, device=self.device)

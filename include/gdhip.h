@@ -202,6 +202,58 @@ int gd_batch_create(gd_ctx *ctx, int n, gd_step *const *steps, int nstreams, gd_
 int gd_batch_run(gd_batch *b, int nsteps);
 int gd_batch_destroy(gd_batch *b);
 
+/* ---- restart files ---------------------------------------------------------
+ * The step's state and its AB2 history.  gd_step_get_history returns the
+ * nonlinear terms of the previous step (the reference keeps them as
+ * prev_nonlinear, simulation.jl:1312-1329; GD_ERR_ARG before the first step).
+ * gd_step_restore sets state, history and step counter together (step 0: no
+ * history, the next step is the AB1 bootstrap); a run continued from a restore
+ * takes exactly the uninterrupted run's steps.  Planes as gd_step_set_state. */
+int gd_step_get_history(gd_step *s, double *nl_re, double *nl_im, int flags);
+int gd_step_restore(gd_step *s, const double *re, const double *im, const double *nl_re,
+                    const double *nl_im, int step, int flags);
+int gd_step_shape(const gd_step *s, int *lmax, int *mmax, int *nr);
+/* radial grid r_i of the step's shell (fields.jl:108-131), host out[nr] */
+int gd_step_radii(const gd_step *s, double *r);
+
+/* Restart metadata: the TimeTracker and run fields that write_restart!
+ * (outputs_writer.jl:1221-1273) stores beside the fields; the step number is
+ * the step's own counter. */
+typedef struct {
+    double time;             /* simulation time: "time", "current_time", "restart_time" */
+    double last_output_time; /* "last_output_time" */
+    int output_count, restart_count, grid_file_written;
+    int mpi_rank, mpi_total_ranks;
+    char geometry[32];       /* "geometry" ("shell" when empty) */
+} gd_restart_meta;
+
+/* replaces write_restart! (outputs_writer.jl:1221-1273): NetCDF classic
+ * (CDF-2) file with the reference's dimensions, variables and attributes
+ * (velocity_toroidal_real ... temperature_spectral_imag as (r, spectral_mode),
+ * l_values, m_values, r, time, step, the restart scalars), plus the AB2 history
+ * as <component>_prev_nonlinear_{real,imag} when the step count is > 0.
+ * precision: GD_PREC_FP64 or GD_PREC_FP32 (config.output_precision). */
+int gd_restart_write(gd_step *s, const char *path, const gd_restart_meta *meta, int precision);
+/* replaces read_restart! (outputs_writer.jl:1275-1370) for one file: checks
+ * spectral_mode and r against the step, maps modes by l_values/m_values,
+ * restores state and history (a file without history restores at step 0);
+ * meta (may be NULL) receives time, tracker scalars and geometry. */
+int gd_restart_read(gd_step *s, const char *path, gd_restart_meta *meta);
+/* host-only forms of the two on l-major planes 5 x (nlm, nr) (no GPU used);
+ * nl_re/nl_im may be NULL (no history written / not returned) */
+int gd_restart_write_arrays(const char *path, const gd_restart_meta *meta, int lmax, int mmax, int nr,
+                            const double *r, int step, const double *re, const double *im,
+                            const double *nl_re, const double *nl_im, int precision);
+int gd_restart_read_arrays(const char *path, int lmax, int mmax, int nr, double *r, int *step,
+                           double *re, double *im, double *nl_re, double *nl_im, int *has_history,
+                           gd_restart_meta *meta);
+/* replaces generate_filename (outputs_writer.jl:338-361):
+ * <dir>/<prefix>_<type>_geom_<geometry>_rank_XXXX_time_<t>.nc with t = %.6f, "." -> "p", or
+ * time before rank when time_first = 1 (TIME_RANK naming) */
+int gd_restart_filename(const char *output_dir, const char *prefix, const char *file_type,
+                        const char *geometry, double time, int rank, int time_first, char *buf,
+                        int buflen);
+
 /* replaces compute_timestep_error (timestep.jl:733-753) on device arrays */
 int gd_timestep_error(gd_ctx *ctx, long n, const double *a_re, const double *a_im,
                       const double *b_re, const double *b_im, double *out, int flags);

@@ -1,0 +1,99 @@
+"""Restart through the C ABI (write_restart! / read_restart!,
+outputs_writer.jl:1221-1370, with the AB2 history beside the fields): a run
+continued from a restart file takes exactly the uninterrupted run's steps,
+bit for bit, on both arithmetic paths, eager and graph-replayed, into a fresh
+step object and into one whose step graphs were already captured."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+CASE = dict(lmax=31, nlat=48, nlon=96, nr=17)
+
+
+def _step(prec):
+    import gdhip
+
+    return gdhip.CNAB2Step(CASE["lmax"], CASE["nlat"], CASE["nlon"], CASE["nr"], Ra=200.0, dt=1e-4, precision=prec)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("graph", [True, False])
+def test_continued_run_equals_uninterrupted(tmp_path, prec, graph):
+    import gdhip
+
+    print("case: restart %s graph=%s lmax=%d nr=%d" % (prec, graph, CASE["lmax"], CASE["nr"]), flush=True)
+    st0 = gdhip.seeded_state(11, CASE["lmax"], CASE["nr"], amp=1e-3)
+    a = _step(prec)
+    a.set_state_complex(st0)
+    a.run(7, graph=graph)
+    ref = a.get_state_complex()
+
+    b = _step(prec)
+    b.set_state_complex(st0)
+    b.run(4, graph=graph)
+    cfg = gdhip.OutputConfig(output_dir=str(tmp_path), filename_prefix="gd")
+    tr = gdhip.TimeTracker(last_output_time=0.0003, output_count=2)
+    path = gdhip.write_restart(b, tr, {"current_time": 4e-4}, cfg)
+
+    c = _step(prec)
+    tr2 = gdhip.TimeTracker()
+    md = gdhip.read_restart(c, tr2, str(tmp_path), 4e-4, cfg)
+    assert md["file"] == path and md["current_step"] == 4 and c.step_count == 4
+    assert tr2.output_count == 2 and tr2.last_output_time == 0.0003
+    assert np.array_equal(c.get_state_complex(), b.get_state_complex())
+    assert np.array_equal(c.get_history(), b.get_history())
+    c.run(3, graph=graph)
+    assert c.step_count == 7
+    assert np.array_equal(c.get_state_complex(), ref)
+
+    # b has captured its step graphs; it continues, then returns to step 4
+    # from the file and must retrace the same steps
+    b.run(2, graph=graph)
+    gdhip.read_restart(b, gdhip.TimeTracker(), str(tmp_path), 4e-4, cfg)
+    assert b.step_count == 4
+    b.run(3, graph=graph)
+    assert np.array_equal(b.get_state_complex(), ref)
+
+
+def test_history_is_the_previous_steps_nonlinear_terms():
+    import gdhip
+
+    print("case: history = nonlinear of the previous state", flush=True)
+    s = _step("fp64")
+    s.set_state_complex(gdhip.seeded_state(12, CASE["lmax"], CASE["nr"], amp=1e-3))
+    with pytest.raises(gdhip.GDError):
+        s.get_history()  # no history before the first step
+    s.run(2)
+    n2 = s.nonlinear()  # N(state at step 2), which step 2 computes again
+    s.run(1)
+    assert np.array_equal(s.get_history(), n2)
+
+
+def test_restart_without_history_bootstraps(tmp_path):
+    """A file that stores no history (as the reference's restart) restores at
+    step 0, so the continued run starts with an AB1 step like a fresh one."""
+    import gdhip
+
+    print("case: restart without history", flush=True)
+    st0 = gdhip.seeded_state(13, CASE["lmax"], CASE["nr"], amp=1e-3)
+    a = _step("fp64")
+    a.set_state_complex(st0)
+    a.run(3)
+    mid = a.get_state_complex()
+    a.run(2)
+    path = str(tmp_path / "nohist.nc")
+    gdhip.write_restart_arrays(path, CASE["lmax"], CASE["nr"], np.linspace(0.5, 1.5, CASE["nr"]), mid, None, step=0)
+    c = _step("fp64")
+    c.set_state_complex(gdhip.seeded_state(14, CASE["lmax"], CASE["nr"], amp=1e-3))
+    import ctypes
+
+    from gdhip._lib import RestartMeta, check, lib
+
+    check(lib().gd_restart_read(c._h, path.encode(), ctypes.byref(RestartMeta())))
+    assert c.step_count == 0
+    b = _step("fp64")
+    b.set_state_complex(mid)
+    b.run(2)
+    c.run(2)
+    assert np.array_equal(c.get_state_complex(), b.get_state_complex())
