@@ -17,6 +17,10 @@ fields with the state resident in HBM.
                  flops per launch over its HIP-event duration
   cpu_baseline   the FP64 oracle step (restatement of the reference step) on
                  the host cores, a bounded sample of steps
+  full_products, exp_integrators, sht_c1, batch_c3, restart_io
+                 secondary measurements (every product formed; EAB2 / ERK2
+                 updates; the lmax=255 transform round trip; a batch of
+                 uneven shells; the restart file write / read)
 
 Prints one JSON line.
 """
@@ -353,6 +357,49 @@ def batch_c3(nshell, steps, nstreams):
             "states_equal_bitwise": same}
 
 
+def restart_io(step, reps):
+    """Restart file of a step at its current state (after the timed runs): the
+    five fields and the AB2 history in FP64, NetCDF CDF-2, written to and read
+    back from a temporary directory on the local disk (median of reps).  Also
+    times the device -> host part alone (state and history planes through the
+    l-major permutation and the copy), and checks that the restored state and
+    history equal the written ones bit for bit."""
+    import shutil
+    import tempfile
+
+    import numpy as np
+
+    import gdhip
+
+    d = tempfile.mkdtemp(prefix="gd_restart_")
+    try:
+        cfg = gdhip.OutputConfig(output_dir=d, filename_prefix="bench")
+        tr = gdhip.TimeTracker()
+        st0, h0 = step.get_state_complex(), step.get_history()
+        tw, tr_, tg = [], [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            step.get_state()
+            step.get_history()
+            tg.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            path = gdhip.write_restart(step, tr, {"current_time": 1.0}, cfg)
+            tw.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            gdhip.read_restart(step, tr, d, 1.0, cfg)
+            step.synchronize()
+            tr_.append(time.perf_counter() - t0)
+        same = bool(np.array_equal(step.get_state_complex(), st0) and np.array_equal(step.get_history(), h0))
+        nbytes = os.path.getsize(path)
+        w, r, g = float(np.median(tw)), float(np.median(tr_)), float(np.median(tg))
+        return {"workload": "restart file of the lmax=127 nr=65 step: 5 fields + AB2 history, FP64, NetCDF CDF-2, "
+                            "local disk", "file_bytes": nbytes, "write_s": round(w, 4), "read_s": round(r, 4),
+                "write_GB_per_s": round(nbytes / w / 1e9, 3), "read_GB_per_s": round(nbytes / r / 1e9, 3),
+                "device_to_host_s": round(g, 4), "restored_bitwise": same, "reps": reps}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 C1 = dict(lmax=255, nlat=384, nlon=768, nr=97)
 
 
@@ -460,6 +507,7 @@ def main():
     ap.add_argument("--no-c3", action="store_true", help="skip the uneven shell batch")
     ap.add_argument("--only-c3", action="store_true", help="print only the uneven shell batch")
     ap.add_argument("--only-c1", action="store_true", help="print only the lmax=255 transform round trip")
+    ap.add_argument("--no-restart", action="store_true", help="skip the restart file write / read timing")
     ap.add_argument("--derive-traffic", metavar="DIR",
                     help="only turn rocprofv3 FETCH_SIZE/WRITE_SIZE runs under DIR into the traffic JSON")
     ap.add_argument("--summarize-trace", metavar="DIR",
@@ -581,6 +629,11 @@ def main():
             out["batch_c3"] = batch_c3(64, 20, 8)
         except Exception as exc:  # a secondary measurement never hides the headline line
             out["batch_c3"] = {"error": repr(exc)[:300]}
+    if not args.no_restart:
+        try:
+            out["restart_io"] = restart_io(fp64, 3)
+        except Exception as exc:  # a secondary measurement never hides the headline line
+            out["restart_io"] = {"error": repr(exc)[:300]}
     if not args.no_cpu_baseline:
         cb, err64 = cpu_baseline(st, args.cpu_steps, one_step64)
         out["cpu_baseline"] = cb

@@ -37,8 +37,6 @@ struct ChainArgs {
     int syn_tang; // only the u and T/r synthesis columns are needed (the tangential step)
     void *synV, *synS;
     int f0, nf;   // fields f0 .. f0 + nf - 1 (nf = 0: all five)
-    int yreg;     // 1: keep the forward-substitution result in registers where the
-                  // launcher has a kernel for this (n, bw); 0: through st_out
 };
 int gd_cnab2_chain_supported(int n, int bw);
 // rdiag[(k*n + i)*nl + l] = 1 / LU_t[k][(i*W + bw)][l] for nk diffusivities

@@ -57,7 +57,6 @@ struct gd_step {
     // 2 is fastest on the FP32 path (4848 vs 4638 steps/s), 0 on the FP64 path
     // (2807 vs 2721); 1 is slower on both (three partial CNAB2 launches)
     int overlap = 0;
-    int yreg = 1; // CNAB2 kernel: forward-substitution result in registers where supported
 };
 
 // ---------------------------------------------------------------- spectral preparation
@@ -364,7 +363,6 @@ static int enqueue_step(gd_step *s, int parity, bool boot, hipEvent_t *ev = null
         c.synV = s->synV.p;
         c.synS = s->synS.p;
         c.syn_tang = tangential(s) ? 1 : 0;
-        c.yreg = s->yreg;
     }
     const bool fp64 = s->d.prec == GD_PREC_FP64;
     if (split) {
@@ -530,10 +528,6 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
     s->overlap = d->prec == GD_PREC_FP32 ? 2 : 0;
     const char *ov = getenv("GD_STEP_OVERLAP");
     if (ov && ov[0] >= '0' && ov[0] <= '2') s->overlap = ov[0] - '0';
-    // GD_CHAIN_YREG = 0 keeps the forward-substitution result of the CNAB2
-    // kernel in HBM also where a register-resident kernel exists (same values)
-    const char *yr = getenv("GD_CHAIN_YREG");
-    s->yreg = !(yr && yr[0] == '0');
     return GD_OK;
 }
 

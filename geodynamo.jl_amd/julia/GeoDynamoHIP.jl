@@ -300,4 +300,49 @@ function energies(s::HIPStep)
     return out
 end
 
+# ---- restart files (outputs_writer.jl:1221-1370): NetCDF classic, the
+# reference's variables plus the AB2 history
+struct GDRestartMeta
+    time::Float64
+    last_output_time::Float64
+    output_count::Cint
+    restart_count::Cint
+    grid_file_written::Cint
+    mpi_rank::Cint
+    mpi_total_ranks::Cint
+    geometry::NTuple{32,UInt8}
+end
+
+function GDRestartMeta(time, last_output_time, output_count, restart_count, grid_file_written, rank, nranks,
+                       geometry::String="shell")
+    g = zeros(UInt8, 32)
+    copyto!(g, 1, codeunits(geometry), 1, min(31, ncodeunits(geometry)))
+    GDRestartMeta(time, last_output_time, output_count, restart_count, grid_file_written, rank, nranks, Tuple(g))
+end
+
+"""write_restart!(s, filename, tracker_fields...; precision=:fp64): the step's
+state and AB2 history in the file the reference's write_restart! would name."""
+function write_restart!(s::HIPStep, filename::String; time::Float64, last_output_time::Float64=0.0,
+                        output_count::Integer=0, restart_count::Integer=0, grid_file_written::Bool=false,
+                        rank::Integer=0, nranks::Integer=1, geometry::String="shell", precision::Symbol=:fp64)
+    meta = GDRestartMeta(time, last_output_time, output_count, restart_count, Cint(grid_file_written), rank, nranks,
+                         geometry)
+    check(ccall((:gd_restart_write, libgdhip), Cint, (Ptr{Cvoid}, Cstring, Ref{GDRestartMeta}, Cint),
+                s.ptr, filename, meta, PREC[precision]))
+end
+
+"""read_restart!(s, filename) -> metadata NamedTuple; restores state, AB2
+history and the step counter."""
+function read_restart!(s::HIPStep, filename::String)
+    meta = Ref{GDRestartMeta}()
+    check(ccall((:gd_restart_read, libgdhip), Cint, (Ptr{Cvoid}, Cstring, Ref{GDRestartMeta}),
+                s.ptr, filename, meta))
+    m = meta[]
+    g = collect(m.geometry)
+    geometry = String(g[1:something(findfirst(==(0x00), g), 33) - 1])
+    return (current_time=m.time, last_output_time=m.last_output_time, output_count=Int(m.output_count),
+            restart_count=Int(m.restart_count), grid_file_written=m.grid_file_written != 0, geometry=geometry,
+            current_step=Int(ccall((:gd_step_count, libgdhip), Cint, (Ptr{Cvoid},), s.ptr)))
+end
+
 end # module
