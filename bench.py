@@ -17,10 +17,11 @@ fields with the state resident in HBM.
                  flops per launch over its HIP-event duration
   cpu_baseline   the FP64 oracle step (restatement of the reference step) on
                  the host cores, a bounded sample of steps
-  full_products, exp_integrators, sht_c1, batch_c3, restart_io
+  full_products, exp_integrators, sht_c1, batch_c3, restart_io, scalar_bc
                  secondary measurements (every product formed; EAB2 / ERK2
                  updates; the lmax=255 transform round trip; a batch of
-                 uneven shells; the restart file write / read)
+                 uneven shells; the restart file write / read; the scalar
+                 boundary conditions with the tau flux correction)
 
 Prints one JSON line.
 """
@@ -400,6 +401,56 @@ def restart_io(step, reps):
         shutil.rmtree(d, ignore_errors=True)
 
 
+def scalar_bc(reps):
+    """Boundary conditions of one scalar field at lmax=127, nr=65 (the
+    temperature / composition functions with the tau flux correction): every
+    mode with a flux condition at both boundaries, random profiles with
+    non-zero imaginary parts, device tensors resident in HBM; reps
+    consecutive applications captured in one graph, replayed five times and
+    timed with events.
+    Algorithmic bytes: both planes read and written once (4 x 8 x nr x nlm)."""
+    import numpy as np
+    import torch
+
+    import gdhip
+
+    lmax, nr = CFG["lmax"], CFG["nr"]
+    cfg = gdhip.create_shtnskit_config(lmax, nlat=CFG["nlat"], nlon=CFG["nlon"])
+    dom = gdhip.create_radial_domain(nr, CFG["rratio"])
+    nlm = cfg.nlm
+    g = torch.Generator(device="cuda").manual_seed(7)
+    re = torch.randn(nr, nlm, dtype=torch.float64, device="cuda", generator=g)
+    im = torch.randn(nr, nlm, dtype=torch.float64, device="cuda", generator=g)
+    bc = gdhip.ScalarBoundaryConditions(cfg, dom, bw=CFG["bw"])
+    two = np.full(nlm, 2, dtype=np.int32)
+    bc.set(two, two, np.random.default_rng(8).standard_normal((2, nlm)))
+    bc.apply(re, im)  # warm-up, eager
+    torch.cuda.synchronize()
+    # reps applications captured in one graph, so the launch rate of the host
+    # loop does not enter the kernel time
+    s = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        for _ in range(reps):
+            bc.apply(re, im)
+    g.replay()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    nrep = 5
+    e0.record()
+    for _ in range(nrep):
+        g.replay()
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / (reps * nrep)
+    nbytes = 4 * 8 * nr * nlm
+    bc.close()
+    return {"workload": "scalar field lmax=127 nr=65, flux (tau) conditions at both boundaries on every mode, "
+                        "device planes", "ms_per_apply": round(ms, 5), "algorithmic_bytes": nbytes,
+            "GB_per_s": round(nbytes / (ms * 1e-3) / 1e9, 1),
+            "hbm_frac": round(nbytes / (ms * 1e-3) / 8e12, 4), "reps": reps * nrep}
+
+
 C1 = dict(lmax=255, nlat=384, nlon=768, nr=97)
 
 
@@ -508,6 +559,8 @@ def main():
     ap.add_argument("--only-c3", action="store_true", help="print only the uneven shell batch")
     ap.add_argument("--only-c1", action="store_true", help="print only the lmax=255 transform round trip")
     ap.add_argument("--no-restart", action="store_true", help="skip the restart file write / read timing")
+    ap.add_argument("--no-bc", action="store_true", help="skip the scalar boundary-condition timing")
+    ap.add_argument("--only-bc", action="store_true", help="print only the scalar boundary-condition timing")
     ap.add_argument("--derive-traffic", metavar="DIR",
                     help="only turn rocprofv3 FETCH_SIZE/WRITE_SIZE runs under DIR into the traffic JSON")
     ap.add_argument("--summarize-trace", metavar="DIR",
@@ -526,7 +579,7 @@ def main():
 
     import numpy as np
 
-    if not (args.no_exp and args.no_c1):
+    if not (args.no_exp and args.no_c1 and args.no_bc) or args.only_bc:
         # torch's bundled HIP runtime must be the process's HIP runtime before
         # libgdhip loads, or the two do not see the same device
         import torch  # noqa: F401
@@ -535,6 +588,9 @@ def main():
         return
     if args.only_c1:
         print(json.dumps(sht_c1(3)), flush=True)
+        return
+    if args.only_bc:
+        print(json.dumps(scalar_bc(50)), flush=True)
         return
     if args.only_c3:
         print(json.dumps(batch_c3(64, 20, 8)), flush=True)
@@ -629,6 +685,11 @@ def main():
             out["batch_c3"] = batch_c3(64, 20, 8)
         except Exception as exc:  # a secondary measurement never hides the headline line
             out["batch_c3"] = {"error": repr(exc)[:300]}
+    if not args.no_bc:
+        try:
+            out["scalar_bc"] = scalar_bc(50)
+        except Exception as exc:  # a secondary measurement never hides the headline line
+            out["scalar_bc"] = {"error": repr(exc)[:300]}
     if not args.no_restart:
         try:
             out["restart_io"] = restart_io(fp64, 3)

@@ -25,3 +25,5 @@ from .timestep import (ERK2Cache, ERK2FieldBuffers, ETDCache, SHTnsImplicitMatri
 from .outputs_writer import (RANK_TIME, TIME_RANK, OutputConfig, TimeTracker, find_restart_files,  # noqa: F401
                              generate_filename, read_restart, read_restart_arrays, write_restart,
                              write_restart_arrays)
+from .scalar_field_common import (ScalarBoundaryConditions, apply_composition_boundary_conditions_spectral,  # noqa: F401
+                                  apply_scalar_flux_bc_spectral, apply_temperature_boundary_conditions_spectral)

@@ -36,6 +36,7 @@ EXPORTS = [
     "gd_step_get_history", "gd_step_restore", "gd_step_shape", "gd_step_radii",
     "gd_restart_write", "gd_restart_read", "gd_restart_write_arrays", "gd_restart_read_arrays",
     "gd_restart_filename",
+    "gd_scalar_bc_create", "gd_scalar_bc_set", "gd_scalar_bc_apply", "gd_scalar_bc_tau", "gd_scalar_bc_destroy",
 ]
 
 
@@ -148,6 +149,11 @@ def lib():
             "gd_restart_read_arrays": (i, [ctypes.c_char_p, i, i, i, vp, vp] + [vp] * 4
                                        + [vp, ctypes.POINTER(RestartMeta)]),
             "gd_restart_filename": (i, [ctypes.c_char_p] * 4 + [d, i, i, ctypes.c_char_p, i]),
+            "gd_scalar_bc_create": (i, [vp, i, i, i, i, vp, i, pp]),
+            "gd_scalar_bc_set": (i, [vp, vp, vp, vp]),
+            "gd_scalar_bc_apply": (i, [vp, vp, vp, i, i]),
+            "gd_scalar_bc_tau": (i, [vp, vp, vp]),
+            "gd_scalar_bc_destroy": (i, [vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

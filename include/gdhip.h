@@ -202,6 +202,33 @@ int gd_batch_create(gd_ctx *ctx, int n, gd_step *const *steps, int nstreams, gd_
 int gd_batch_run(gd_batch *b, int nsteps);
 int gd_batch_destroy(gd_batch *b);
 
+/* ---- boundary conditions of a scalar field ----------------------------------
+ * replaces apply_temperature_boundary_conditions_spectral! (thermal.jl:262-313)
+ * and apply_composition_boundary_conditions_spectral! (compositional.jl:318-370)
+ * with apply_scalar_flux_bc_spectral!(...; method=:tau)
+ * (scalar_field_common.jl:1166-1199, :866-929).  Per (l, m): bc type 1 sets
+ * the boundary row to boundary_values (imag 0); type 2 adds the tau-function
+ * correction that removes the flux error D1 p - value of the real profile at
+ * that boundary (both boundaries: T_{N-1}, T_N with analytic end derivatives;
+ * one boundary: T_{N-1} - a T_N, its derivative through D1), to the imaginary
+ * profile too when one of its entries exceeds 1e-12 (the reference's rule).
+ * r: the nr x 7 radial table (gd_radial_grid); D1 from gd_derivative_matrix
+ * with bw and fd_mode.  bc_inner/bc_outer: nlm ints; values: Julia
+ * boundary_values[2, nlm] (inner, outer per mode), l-major modes. */
+typedef struct gd_scalar_bc gd_scalar_bc;
+int gd_scalar_bc_create(gd_ctx *ctx, int lmax, int mmax, int nr, int bw, const double *r, int fd_mode,
+                        gd_scalar_bc **out);
+int gd_scalar_bc_set(gd_scalar_bc *b, const int *bc_inner, const int *bc_outer, const double *values);
+/* re, im: (nlm, nr) planes as gd_build_rhs_cnab2; dirichlet = 1 applies the
+ * type-1 rows first (the temperature / composition functions), 0 only the flux
+ * correction (apply_scalar_flux_bc_spectral! alone) */
+int gd_scalar_bc_apply(gd_scalar_bc *b, double *re, double *im, int dirichlet, int flags);
+/* host copies: tab[4][nr] = T_{N-1}, T_N, inner and outer tau functions;
+ * scalars[6] = end derivatives dT_{N-1}(r_i), dT_{N-1}(r_o), dT_N(r_i), dT_N(r_o),
+ * D1 derivative of the inner function at r_i and of the outer one at r_o */
+int gd_scalar_bc_tau(const gd_scalar_bc *b, double *tab, double *scalars);
+int gd_scalar_bc_destroy(gd_scalar_bc *b);
+
 /* ---- restart files ---------------------------------------------------------
  * The step's state and its AB2 history.  gd_step_get_history returns the
  * nonlinear terms of the previous step (the reference keeps them as
