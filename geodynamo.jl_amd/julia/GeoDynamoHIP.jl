@@ -300,6 +300,41 @@ function energies(s::HIPStep)
     return out
 end
 
+# ---- scalar boundary conditions (thermal.jl:262-313, compositional.jl:318-370,
+# scalar_field_common.jl:866-929, 1166-1199; method :tau)
+mutable struct HIPScalarBC
+    ptr::Ptr{Cvoid}
+    nlm::Int
+    nr::Int
+end
+
+function create_scalar_bc(lmax::Integer, r::Matrix{Float64}; mmax::Integer=lmax, bw::Integer=3)
+    nr = size(r, 1)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve r check(ccall((:gd_scalar_bc_create, libgdhip), Cint,
+        (Ptr{Cvoid}, Cint, Cint, Cint, Cint, Ptr{Float64}, Cint, Ref{Ptr{Cvoid}}),
+        context(), lmax, mmax, nr, bw, r, GD_FD_TRANSPOSE, h))
+    nlm = sum(min(l, mmax) + 1 for l in 0:lmax)
+    b = HIPScalarBC(h[], nlm, nr)
+    finalizer(q -> ccall((:gd_scalar_bc_destroy, libgdhip), Cint, (Ptr{Cvoid},), q.ptr), b)
+    return b
+end
+
+function _scalar_bc!(b::HIPScalarBC, re, im, bc_type_inner::Vector{Int}, bc_type_outer::Vector{Int},
+                     boundary_values::Matrix{Float64}, dirichlet::Integer)
+    ti, to = Cint.(bc_type_inner), Cint.(bc_type_outer)
+    GC.@preserve ti to boundary_values check(ccall((:gd_scalar_bc_set, libgdhip), Cint,
+        (Ptr{Cvoid}, Ptr{Cint}, Ptr{Cint}, Ptr{Float64}), b.ptr, ti, to, boundary_values))
+    GC.@preserve re im check(ccall((:gd_scalar_bc_apply, libgdhip), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint, Cint), b.ptr, re, im, dirichlet, GD_HOST))
+end
+
+apply_temperature_boundary_conditions_spectral!(b::HIPScalarBC, re, im, ti, to, vals) =
+    _scalar_bc!(b, re, im, ti, to, vals, 1)
+apply_composition_boundary_conditions_spectral!(b::HIPScalarBC, re, im, ti, to, vals) =
+    _scalar_bc!(b, re, im, ti, to, vals, 1)
+apply_scalar_flux_bc_spectral!(b::HIPScalarBC, re, im, ti, to, vals) = _scalar_bc!(b, re, im, ti, to, vals, 0)
+
 # ---- restart files (outputs_writer.jl:1221-1370): NetCDF classic, the
 # reference's variables plus the AB2 history
 struct GDRestartMeta
