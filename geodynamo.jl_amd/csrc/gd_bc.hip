@@ -359,20 +359,44 @@ extern "C" int gd_scalar_bc_create(gd_ctx *ctx, int lmax, int mmax, int nr, int 
     return GD_OK;
 }
 
-extern "C" int gd_scalar_bc_set(gd_scalar_bc *b, const int *bc_inner, const int *bc_outer, const double *values)
+// column k of the planes holds l-major mode lm_of_pos[k] (identity when null)
+int gd_scalar_bc_set_perm(gd_scalar_bc *b, const int *bc_inner, const int *bc_outer, const double *values,
+                          const int *lm_of_pos)
 {
     GD_CHECK(b && bc_inner && bc_outer && values, GD_ERR_ARG, "gd_scalar_bc_set: null argument");
     GD_HIP(hipSetDevice(b->ctx->device));
     std::vector<int> t((size_t)2 * b->nlm);
+    std::vector<double> v((size_t)2 * b->nlm);
     for (int k = 0; k < b->nlm; ++k) {
-        t[2 * k] = bc_inner[k];
-        t[2 * k + 1] = bc_outer[k];
+        const int lm = lm_of_pos ? lm_of_pos[k] : k;
+        t[2 * k] = bc_inner[lm];
+        t[2 * k + 1] = bc_outer[lm];
+        v[2 * k] = values[2 * lm];
+        v[2 * k + 1] = values[2 * lm + 1];
     }
     // the host arrays are read before the call returns
     GD_HIP(hipMemcpy(b->types.p, t.data(), sizeof(int) * t.size(), hipMemcpyHostToDevice));
-    GD_HIP(hipMemcpy(b->vals.p, values, sizeof(double) * 2 * b->nlm, hipMemcpyHostToDevice));
+    GD_HIP(hipMemcpy(b->vals.p, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice));
     b->set = true;
     return GD_OK;
+}
+
+extern "C" int gd_scalar_bc_set(gd_scalar_bc *b, const int *bc_inner, const int *bc_outer, const double *values)
+{
+    return gd_scalar_bc_set_perm(b, bc_inner, bc_outer, values, nullptr);
+}
+
+// device planes (nr, nlm), in place on stream st; no synchronisation
+int gd_scalar_bc_launch(gd_scalar_bc *b, hipStream_t st, double *re, double *im, int dirichlet)
+{
+    GD_CHECK(b->set, GD_ERR_ARG, "gd_scalar_bc_apply: no boundary conditions set (gd_scalar_bc_set)");
+    BcArgs a = b->args;
+    a.dirichlet = dirichlet ? 1 : 0;
+    const dim3 grid((b->nlm + BM - 1) / BM), block(BM * BL);
+    if (b->nr <= BL * 5 && b->bw <= 7) hipLaunchKernelGGL(scalar_bc_reg_kernel<5>, grid, block, 0, st, a, re, im);
+    else if (b->nr <= BL * 8 && b->bw <= 7) hipLaunchKernelGGL(scalar_bc_reg_kernel<8>, grid, block, 0, st, a, re, im);
+    else hipLaunchKernelGGL(scalar_bc_kernel, grid, block, 0, st, a, re, im);
+    return gd_check_launch("scalar_bc_kernel");
 }
 
 extern "C" int gd_scalar_bc_apply(gd_scalar_bc *b, double *re, double *im, int dirichlet, int flags)
@@ -390,13 +414,7 @@ extern "C" int gd_scalar_bc_apply(gd_scalar_bc *b, double *re, double *im, int d
         GD_HIP(hipMemcpyAsync(dre, re, sizeof(double) * n, hipMemcpyHostToDevice, st));
         GD_HIP(hipMemcpyAsync(dim, im, sizeof(double) * n, hipMemcpyHostToDevice, st));
     }
-    BcArgs a = b->args;
-    a.dirichlet = dirichlet ? 1 : 0;
-    const dim3 grid((b->nlm + BM - 1) / BM), block(BM * BL);
-    if (b->nr <= BL * 5 && b->bw <= 7) hipLaunchKernelGGL(scalar_bc_reg_kernel<5>, grid, block, 0, st, a, dre, dim);
-    else if (b->nr <= BL * 8 && b->bw <= 7) hipLaunchKernelGGL(scalar_bc_reg_kernel<8>, grid, block, 0, st, a, dre, dim);
-    else hipLaunchKernelGGL(scalar_bc_kernel, grid, block, 0, st, a, dre, dim);
-    GD_TRY(gd_check_launch("scalar_bc_kernel"));
+    GD_TRY(gd_scalar_bc_launch(b, st, dre, dim, dirichlet));
     if (flags & GD_HOST) {
         GD_TRY(gd_from_device(b->ctx, re, dre, sizeof(double) * n));
         GD_TRY(gd_from_device(b->ctx, im, dim, sizeof(double) * n));

@@ -66,7 +66,11 @@ template <int BW, int D, typename TS, bool DIRECT> __global__ void __launch_boun
     const double *pp = nlz ? c.zero_row + idx : c.np_arr + off;
     const size_t nstride = nlz ? 0 : (size_t)nlm;
     double *o = c.st_out + off;
-    const bool tbc = (f == 4);
+    // bit 5 of bc_mask: a boundary-condition stage has already set the
+    // temperature rows in st_in; read them as stored, and Dirichlet rhs rows
+    // of the temperature take the stored boundary value
+    const bool tkeep = (f == 4) && ((c.bc_mask >> 5) & 1);
+    const bool tbc = (f == 4) && !tkeep;
     const bool bcr = (c.bc_mask >> f) & 1;
     const double inv_dt = 1.0 / c.dt, lw = 1.0 - c.theta;
     auto cl = [&](int j) { return j < 0 ? 0 : (j > n - 1 ? n - 1 : j); }; // a valid row
@@ -111,7 +115,7 @@ template <int BW, int D, typename TS, bool DIRECT> __global__ void __launch_boun
             for (int q = 0; q < W; ++q) lr += ((q == BW) ? er[BW] - nulf * er[W] : er[q]) * w[q];
             double v = inv_dt * w[BW] + 1.5 * sn[d] - 0.5 * sp[d];
             v += lw * lr;
-            v = (bcr && (i == 0 || i == n - 1)) ? 0.0 : v; // Dirichlet value
+            v = (bcr && (i == 0 || i == n - 1)) ? (tkeep ? w[BW] : 0.0) : v; // Dirichlet value
 #pragma unroll
             for (int q = 0; q < BW; ++q) v -= sl[d][q] * yw[q];
 #pragma unroll

@@ -105,3 +105,10 @@ struct FftPlan {
 
 // launch helper: returns GD_ERR_HIP on launch failure
 int gd_check_launch(const char *what);
+
+// scalar boundary conditions (gd_bc.hip) for callers inside the library:
+// per-mode types/values given in l-major order, stored for planes whose column
+// k holds l-major mode lm_of_pos[k]; launch in place on device planes
+int gd_scalar_bc_set_perm(gd_scalar_bc *b, const int *bc_inner, const int *bc_outer, const double *values,
+                          const int *lm_of_pos);
+int gd_scalar_bc_launch(gd_scalar_bc *b, hipStream_t st, double *re, double *im, int dirichlet);

@@ -237,8 +237,10 @@ int gd_launch_cnab2_lmajor(hipStream_t st, const double *L, const double *LU, in
 //                  every l and is read with uniform (scalar) loads
 //   LU(S_l):       from the l-fastest transposed table LU_t[(col*W + row)*nl + l]
 // nu index of field f: {0,0,1,1,2}.  Field 4 (temperature) has its boundary
-// rows set to 0 before the rhs (thermal.jl:262-313); fields in bc_mask have
-// Dirichlet rows (rhs rows 0 and n-1 := 0).
+// rows set to 0 before the rhs (thermal.jl:262-313) unless bc_mask bit 5 says
+// a boundary-condition stage already set them; fields in bc_mask have
+// Dirichlet rows (rhs rows 0 and n-1 := 0, the stored value for such a
+// temperature).
 namespace {
 template <int BW, int D> struct FwdBlk {
     double ur[D], ui[D], nr_[D], ni[D], pr[D], pi[D], fl[D][BW];
@@ -271,7 +273,10 @@ __global__ void __launch_bounds__(256) cnab2_step_kernel(const double *__restric
     double *ore = st_out + (size_t)(2 * f) * fs + idx, *oim = ore + fs;
     const double *nre = nl_arr + (size_t)(2 * f) * fs + idx, *nim = nre + fs;
     const double *pre = np_arr + (size_t)(2 * f) * fs + idx, *pim = pre + fs;
-    const bool tbc = (f == 4);
+    // bc_mask bit 5: temperature rows already set by a boundary-condition
+    // stage (read as stored; Dirichlet rhs rows take the stored value)
+    const bool tkeep = (f == 4) && ((bc_mask >> 5) & 1);
+    const bool tbc = (f == 4) && !tkeep;
     const bool bcr = (bc_mask >> f) & 1;
     const double inv_dt = 1.0 / dt, lw = 1.0 - theta;
     auto urow = [&](int j) { return j >= 0 && j < n && !(tbc && (j == 0 || j == n - 1)); };
@@ -327,7 +332,10 @@ __global__ void __launch_bounds__(256) cnab2_step_kernel(const double *__restric
                 vr += lw * lr;
                 vi += lw * li;
             }
-            if (bcr && (i == 0 || i == n - 1)) vr = vi = 0.0; // Dirichlet value
+            if (bcr && (i == 0 || i == n - 1)) { // Dirichlet value
+                vr = tkeep ? wr[BW] : 0.0;
+                vi = tkeep ? wi[BW] : 0.0;
+            }
 #pragma unroll
             for (int q = 0; q < BW; ++q) {
                 vr -= b.fl[d][q] * yr[q];

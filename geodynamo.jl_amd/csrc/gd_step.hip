@@ -57,6 +57,9 @@ struct gd_step {
     // 2 is fastest on the FP32 path (4848 vs 4638 steps/s), 0 on the FP64 path
     // (2807 vs 2721); 1 is slower on both (three partial CNAB2 launches)
     int overlap = 0;
+    // per-mode temperature conditions (gd_step_set_temperature_bc); null: the
+    // boundary rows of T are read as 0 (thermal.jl defaults: Dirichlet, value 0)
+    gd_scalar_bc *tbc = nullptr;
 };
 
 // ---------------------------------------------------------------- spectral preparation
@@ -311,9 +314,19 @@ static int nonlinear(gd_step *s, double *nl_out, hipEvent_t *ev = nullptr, bool 
 // fields whose implicit operator carries Dirichlet rows (bit f for field f)
 static int bc_mask(const gd_step *s)
 {
-    if (s->d.bc_rows == 1) return 0x1f;
-    if (s->d.bc_rows == 2) return 0x10;
-    return 0;
+    const int t = s->tbc ? 0x20 : 0; // temperature rows set by the condition stage
+    if (s->d.bc_rows == 1) return 0x1f | t;
+    if (s->d.bc_rows == 2) return 0x10 | t;
+    return t;
+}
+
+// the temperature conditions of thermal.jl:262-313 on the state the solve
+// reads, after the nonlinear terms (which see the state before them)
+static int temperature_bc(gd_step *s, int parity, hipStream_t st)
+{
+    if (!s->tbc) return GD_OK;
+    double *cur = s->state[parity].as<double>();
+    return gd_scalar_bc_launch(s->tbc, st, cur + 8 * s->fs, cur + 9 * s->fs, 1);
 }
 
 // one full step with the nl buffers chosen by parity; boot = AB1 bootstrap.
@@ -333,7 +346,7 @@ static int enqueue_step(gd_step *s, int parity, bool boot, hipEvent_t *ev = null
     double *prev = boot ? cur : s->nlbuf[parity ^ 1].as<double>();
     hipStream_t st = s->ctx->stream;
     const bool side_ok = s->chain && tangential(s) && !ev && s->side;
-    const bool split = side_ok && s->overlap == 1, ana_side = side_ok && s->overlap >= 1;
+    const bool split = side_ok && s->overlap == 1 && !s->tbc, ana_side = side_ok && s->overlap >= 1;
     ChainArgs c{};
     if (s->chain) {
         c.nuLap = s->nuLap.as<double>();
@@ -390,10 +403,12 @@ static int enqueue_step(gd_step *s, int parity, bool boot, hipEvent_t *ev = null
                 GD_HIP(hipEventRecord(s->ev_join, s->side));
                 GD_HIP(hipStreamWaitEvent(st, s->ev_join, 0));
             }
+            GD_TRY(temperature_bc(s, parity, st));
             GD_TRY(gd_launch_cnab2_chain(st, c, fp64));
         }
         s->syn_ok = true; // synV/synS now describe state[parity ^ 1]
     } else {
+        GD_TRY(temperature_bc(s, parity, st));
         GD_TRY(gd_launch_cnab2_step(st, s->nuLap.as<double>(), s->rtab.as<double>() + s->d.nr,
                                     s->nus.as<double>(), s->LUt.as<double>(), s->sht.d_l_of_idx.as<int>(), s->nl,
                                     s->d.nr, s->d.bw, s->nlm, s->d.dt, s->d.theta, bc_mask(s),
@@ -423,6 +438,8 @@ static int step_release(gd_step *s)
     for (DevBuf *b : bufs) b->release();
     for (hipEvent_t *e : {&s->ev_fork, &s->ev_ring, &s->ev_join})
         if (*e) (void)hipEventDestroy(*e);
+    if (s->tbc) (void)gd_scalar_bc_destroy(s->tbc);
+    s->tbc = nullptr;
     if (s->side) (void)hipStreamDestroy(s->side);
     s->side = nullptr;
     gd_sht_plan_release(&s->sht);
@@ -724,6 +741,33 @@ extern "C" int gd_step_shape(const gd_step *s, int *lmax, int *mmax, int *nr)
     *mmax = s->d.mmax;
     *nr = s->d.nr;
     return GD_OK;
+}
+
+// Per-mode temperature conditions for the step (the reference's
+// bc_type_inner / bc_type_outer / boundary_values of the temperature field,
+// applied by apply_temperature_boundary_conditions_spectral!, thermal.jl:262-313,
+// between the nonlinear terms and the implicit step); null arrays return to
+// the default (boundary rows read as 0).  Captured graphs are dropped.
+extern "C" int gd_step_set_temperature_bc(gd_step *s, const int *bc_inner, const int *bc_outer, const double *values)
+{
+    GD_CHECK(s, GD_ERR_ARG, "gd_step_set_temperature_bc: null step");
+    GD_CHECK((bc_inner && bc_outer && values) || (!bc_inner && !bc_outer && !values), GD_ERR_ARG,
+             "gd_step_set_temperature_bc: give all three arrays or none");
+    GD_HIP(hipSetDevice(s->ctx->device));
+    GD_HIP(hipStreamSynchronize(s->ctx->stream));
+    destroy_graphs(s);
+    if (!bc_inner) {
+        if (s->tbc) (void)gd_scalar_bc_destroy(s->tbc);
+        s->tbc = nullptr;
+        return GD_OK;
+    }
+    if (!s->tbc) {
+        std::vector<double> r((size_t)s->d.nr * 7);
+        GD_TRY(gd_radial_grid(s->d.nr, s->d.rratio, r.data()));
+        GD_TRY(gd_scalar_bc_create(s->ctx, s->d.lmax, s->d.mmax, s->d.nr, s->d.bw, r.data(), s->d.fd_mode, &s->tbc));
+    }
+    // the step's planes are in the device (m-major) mode order
+    return gd_scalar_bc_set_perm(s->tbc, bc_inner, bc_outer, values, s->sht.maps.lmaj_of_idx.data());
 }
 
 // the radial grid r_i of the step's shell (fields.jl:108-131), host out[nr]

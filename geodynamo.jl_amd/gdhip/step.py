@@ -72,6 +72,20 @@ class CNAB2Step:
             c = d = None
         check(lib().gd_step_restore(self._h, a, b, c, d, int(step), flags))
 
+    def set_temperature_bc(self, bc_type_inner=None, bc_type_outer=None, boundary_values=None):
+        """Per-mode temperature conditions (thermal.jl:262-313): types 1
+        (Dirichlet) / 2 (flux, tau method) per l-major mode and
+        boundary_values (2, nlm); no arguments: back to the default (rows 0)."""
+        if bc_type_inner is None:
+            check(lib().gd_step_set_temperature_bc(self._h, None, None, None))
+            return
+        ti = np.ascontiguousarray(bc_type_inner, dtype=np.int32)
+        to = np.ascontiguousarray(bc_type_outer, dtype=np.int32)
+        v = np.ascontiguousarray(np.asarray(boundary_values, dtype=np.float64).T)
+        if ti.shape != (self.nlm,) or to.shape != (self.nlm,) or v.shape != (self.nlm, 2):
+            raise ValueError("bc types need nlm entries and boundary_values shape (2, nlm)")
+        check(lib().gd_step_set_temperature_bc(self._h, ti.ctypes.data, to.ctypes.data, v.ctypes.data))
+
     def radii(self):
         r = np.zeros(self.nr)
         check(lib().gd_step_radii(self._h, r.ctypes.data))

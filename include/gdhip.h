@@ -240,6 +240,16 @@ int gd_step_get_history(gd_step *s, double *nl_re, double *nl_im, int flags);
 int gd_step_restore(gd_step *s, const double *re, const double *im, const double *nl_re,
                     const double *nl_im, int step, int flags);
 int gd_step_shape(const gd_step *s, int *lmax, int *mmax, int *nr);
+/* Per-mode temperature conditions of the step: the temperature field's
+ * bc_type_inner / bc_type_outer / boundary_values[2, nlm] (l-major, as
+ * gd_scalar_bc_set), applied as apply_temperature_boundary_conditions_spectral!
+ * does (thermal.jl:262-313, tau flux correction for type 2) to the state
+ * between the nonlinear terms and the implicit step.  With bc_rows 1 or 2 the
+ * temperature's Dirichlet rows of S_l take the boundary values the conditions
+ * leave in the state.  NULL arrays: back to the default (boundary rows read as
+ * 0, the reference's default Dirichlet 0).  Drops the step's captured graphs;
+ * a batch keeps the conditions its graphs were captured with. */
+int gd_step_set_temperature_bc(gd_step *s, const int *bc_inner, const int *bc_outer, const double *values);
 /* radial grid r_i of the step's shell (fields.jl:108-131), host out[nr] */
 int gd_step_radii(const gd_step *s, double *r);
 
