@@ -688,6 +688,17 @@ def main():
     if not args.no_bc:
         try:
             out["scalar_bc"] = scalar_bc(50)
+            # the mixed step with flux (tau) temperature conditions on every mode
+            # at both boundaries (value 0): the condition stage inside the step
+            bstep = gdhip.CNAB2Step(lmax, nlat, nlon, nr, precision="fp32", **kw)
+            two = np.full(bstep.nlm, 2)
+            bstep.set_temperature_bc(two, two, np.zeros((2, bstep.nlm)))
+            tb = timed(bstep, st, args.steps, args.warmup)
+            out["scalar_bc"]["step_with_temperature_flux_bc"] = {
+                "mixed_steps_per_s": round(args.steps / tb, 2),
+                "vs_default_mixed": round(tmx / tb, 3),
+                "state_finite": bool(np.all(np.isfinite(bstep.get_state_complex())))}
+            bstep.close()
         except Exception as exc:  # a secondary measurement never hides the headline line
             out["scalar_bc"] = {"error": repr(exc)[:300]}
     if not args.no_restart:
