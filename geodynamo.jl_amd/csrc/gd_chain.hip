@@ -27,7 +27,7 @@ template <int BW, int D, typename TS, bool DIRECT> __global__ void __launch_boun
     const int n = c.n, nlm = c.nlm;
     const int idx = blockIdx.x * 64 + threadIdx.x;
     const int f = c.f0 + blockIdx.y, ri = blockIdx.z; // field, 0 re / 1 im
-    const int k = f < 2 ? 0 : (f < 4 ? 1 : 2);
+    const int k = c.kdiff > 0 ? c.kdiff - 1 : (f < 2 ? 0 : (f < 4 ? 1 : 2));
     // level-uniform tables in LDS: the explicit operator by row (ER doubles per
     // row i: the coefficient of u row i-BW+q in slot q, 0 where that row does
     // not exist, and 1/r_i^2 in slot W), D1 and D2 bands, 1/r, 1/r^2

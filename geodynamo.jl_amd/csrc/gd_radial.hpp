@@ -37,6 +37,7 @@ struct ChainArgs {
     int syn_tang; // only the u and T/r synthesis columns are needed (the tangential step)
     void *synV, *synS;
     int f0, nf;   // fields f0 .. f0 + nf - 1 (nf = 0: all five)
+    int kdiff;    // 0: diffusivity slot by field (0 0 1 1 2); k + 1: slot k for every field
 };
 int gd_cnab2_chain_supported(int n, int bw);
 // rdiag[(k*n + i)*nl + l] = 1 / LU_t[k][(i*W + bw)][l] for nk diffusivities

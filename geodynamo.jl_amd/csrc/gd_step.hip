@@ -60,6 +60,16 @@ struct gd_step {
     // per-mode temperature conditions (gd_step_set_temperature_bc); null: the
     // boundary rows of T are read as 0 (thermal.jl defaults: Dirichlet, value 0)
     gd_scalar_bc *tbc = nullptr;
+    // optional passive composition (gd_step_enable_composition, tangential
+    // step): state and nonlinear history in the planes 8, 9 (field 4) of
+    // five-field buffers whose other planes stay zero, so the CNAB2 kernel
+    // treats it as a temperature-like field with diffusivity slot 3 (Pm/Sc);
+    // its own synthesis columns and ring output
+    bool comp = false;
+    double Sc = 1.0;
+    DevBuf cstate[2], cnl[2], csynV, csynS, cgb;
+    gd_scalar_bc *cbc = nullptr;
+    bool csyn_ok = false;
 };
 
 // ---------------------------------------------------------------- spectral preparation
@@ -139,16 +149,29 @@ static bool tangential(const gd_step *s) { return !s->d.radials && !s->d.full_pr
 
 // side (tangential step only): the scalar analysis runs on s->side after the
 // ring kernel, the vector analysis on the main stream
-// the Legendre-synthesis columns of the current state
-template <typename T> static int spectral_pre(gd_step *s)
+// the Legendre-synthesis columns of a state
+template <typename T> static int spectral_pre_into(gd_step *s, const double *state, DevBuf &synV, DevBuf &synS)
 {
     gd_sht_plan *p = &s->sht;
     const int nr = s->d.nr, nlm = s->nlm;
-    hipLaunchKernelGGL((spectral_pre_kernel<T>), dim3((nlm + 255) / 256, nr), dim3(256), 0, s->ctx->stream,
-                       cur_state(s), p->d_l_of_idx.as<int>(), s->D1.as<double>(), s->D2.as<double>(),
-                       s->rtab.as<double>(), nlm, nr, s->d.bw, s->d.radials, s->synV.as<T>(), s->synS.as<T>());
-    GD_TRY(gd_check_launch("spectral_pre_kernel"));
+    hipLaunchKernelGGL((spectral_pre_kernel<T>), dim3((nlm + 255) / 256, nr), dim3(256), 0, s->ctx->stream, state,
+                       p->d_l_of_idx.as<int>(), s->D1.as<double>(), s->D2.as<double>(), s->rtab.as<double>(), nlm, nr,
+                       s->d.bw, s->d.radials, synV.as<T>(), synS.as<T>());
+    return gd_check_launch("spectral_pre_kernel");
+}
+
+template <typename T> static int spectral_pre(gd_step *s)
+{
+    GD_TRY(spectral_pre_into<T>(s, cur_state(s), s->synV, s->synS));
     s->syn_ok = true;
+    return GD_OK;
+}
+
+// the composition's columns (C/r in columns 16, 17 of csynV)
+template <typename T> static int spectral_pre_comp(gd_step *s)
+{
+    GD_TRY(spectral_pre_into<T>(s, s->cstate[s->step & 1].as<double>(), s->csynV, s->csynS));
+    s->csyn_ok = true;
     return GD_OK;
 }
 
@@ -157,8 +180,11 @@ template <typename T> static int spectral_pre(gd_step *s)
 // are stale, so they are rebuilt eagerly before the next replay.
 static int ensure_syn(gd_step *s)
 {
-    if (s->syn_ok || !s->chain) return GD_OK;
-    return s->d.prec == GD_PREC_FP64 ? spectral_pre<double>(s) : spectral_pre<float>(s);
+    if (!s->chain) return GD_OK;
+    const bool f64 = s->d.prec == GD_PREC_FP64;
+    if (!s->syn_ok) GD_TRY(f64 ? spectral_pre<double>(s) : spectral_pre<float>(s));
+    if (s->comp && !s->csyn_ok) GD_TRY(f64 ? spectral_pre_comp<double>(s) : spectral_pre_comp<float>(s));
+    return GD_OK;
 }
 
 template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEvent_t *ev, bool side)
@@ -202,15 +228,28 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEven
     const int maxgroup = (p->lmax + 2) / 2;
 
     if (tangential(s)) {
-        // u (S = u_pol, T = u_tor) and grad T (S = T/r) only; FB records of 16 slots
-        a.nslot = 16;
+        // u (S = u_pol, T = u_tor) and grad T (S = T/r) only; FB records of 16
+        // slots, 24 with the composition's gradient (S = C/r) in slots 16..23
+        const int nslot = s->comp ? 24 : 16;
+        a.nslot = nslot;
         a.nf = 2;
         static const int vmap[8] = {0, 1, 2, 3, 16, 17, -1, -1};
         for (int k = 0; k < 20; ++k) a.colmap[k] = k < 8 ? vmap[k] : -1;
         GD_TRY(gd_launch_synth_vec<T>(st, a));
+        if (s->comp) {
+            if (!s->csyn_ok) GD_TRY(spectral_pre_comp<T>(s));
+            SynthArgs ac = a;
+            ac.in = s->csynV.p;
+            ac.nf = 1;
+            for (int k = 0; k < 20; ++k) ac.colmap[k] = -1;
+            ac.colmap[0] = 16;
+            ac.colmap[1] = 17;
+            ac.slot0 = 16;
+            GD_TRY(gd_launch_synth_vec<T>(st, ac));
+        }
         GD_MARK(2);
         GD_MARK(3);
-        r.nslot_f = 16;
+        r.nslot_f = nslot;
         r.nslot_g = 12;
         r.nfield = 4;
         static const int fn[4] = {0, 2, 8, 10}, fsou[4] = {4, 6, 12, 14};
@@ -219,6 +258,28 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEven
             r.fs_off[F] = fsou[F];
         }
         GD_TRY(gd_launch_ring<T>(st, RING_STEP_TANG, r));
+        if (s->comp) {
+            // the same ring pass with grad C in place of grad T: its third
+            // product is -u.grad C (compute_composition_nonlinear!,
+            // compositional.jl:219-260); the Coriolis products are not used
+            RingArgs rc = r;
+            rc.gb = s->cgb.p;
+            rc.fn_off[2] = 16;
+            rc.fn_off[3] = 18;
+            rc.fs_off[2] = 20;
+            rc.fs_off[3] = 22;
+            GD_TRY(gd_launch_ring<T>(st, RING_STEP_TANG, rc));
+            AnaArgs gc = g;
+            gc.gb = s->cgb.p;
+            gc.nslot_g = 12;
+            gc.gslot0 = 8;
+            gc.nf = 1;
+            gc.tab0 = gd_sht_tab(p, 0);
+            gc.tab1 = nullptr;
+            gc.out[0] = s->cnl[s->step & 1].as<double>() + 8 * fs;
+            gc.scale[0] = 1;
+            GD_TRY(gd_launch_ana<T>(st, gc, false, maxgroup));
+        }
         GD_MARK(4);
         if (side) {
             GD_HIP(hipEventRecord(s->ev_ring, st));
@@ -407,6 +468,27 @@ static int enqueue_step(gd_step *s, int parity, bool boot, hipEvent_t *ev = null
             GD_TRY(gd_launch_cnab2_chain(st, c, fp64));
         }
         s->syn_ok = true; // synV/synS now describe state[parity ^ 1]
+        if (s->comp) {
+            // the composition's conditions (compositional.jl:318-370, default
+            // no flux at both boundaries) on the state the solve reads, then
+            // its CNAB2 column solves with diffusivity slot 3
+            double *cc = s->cstate[parity].as<double>();
+            GD_TRY(gd_scalar_bc_launch(s->cbc, st, cc + 8 * s->fs, cc + 9 * s->fs, 1));
+            ChainArgs ck = c;
+            ck.st_in = cc;
+            ck.st_out = s->cstate[parity ^ 1].as<double>();
+            ck.nl_arr = s->cnl[parity].as<double>();
+            ck.np_arr = boot ? ck.nl_arr : s->cnl[parity ^ 1].as<double>();
+            ck.f0 = 4;
+            ck.nf = 1;
+            ck.kdiff = 4; // slot 3
+            ck.nl_zero_mask = 0;
+            ck.bc_mask = (s->d.bc_rows == 1 ? 0x10 : 0) | 0x20;
+            ck.synV = s->csynV.p;
+            ck.synS = s->csynS.p;
+            GD_TRY(gd_launch_cnab2_chain(st, ck, fp64));
+            s->csyn_ok = true;
+        }
     } else {
         GD_TRY(temperature_bc(s, parity, st));
         GD_TRY(gd_launch_cnab2_step(st, s->nuLap.as<double>(), s->rtab.as<double>() + s->d.nr,
@@ -440,6 +522,10 @@ static int step_release(gd_step *s)
         if (*e) (void)hipEventDestroy(*e);
     if (s->tbc) (void)gd_scalar_bc_destroy(s->tbc);
     s->tbc = nullptr;
+    if (s->cbc) (void)gd_scalar_bc_destroy(s->cbc);
+    s->cbc = nullptr;
+    for (DevBuf *b : {&s->cstate[0], &s->cstate[1], &s->cnl[0], &s->cnl[1], &s->csynV, &s->csynS, &s->cgb})
+        b->release();
     if (s->side) (void)hipStreamDestroy(s->side);
     s->side = nullptr;
     gd_sht_plan_release(&s->sht);
@@ -479,8 +565,10 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
     GD_TRY(upload_vec(ctx, s->D2, d2));
     // implicit operators per diffusivity (simulation.jl:229-231), FP64 on the GPU
     const size_t nb = (size_t)s->W * nr * s->nl;
-    GD_TRY(s->Lt.ensure(sizeof(double) * nb * 3));
-    GD_TRY(s->LUt.ensure(sizeof(double) * nb * 3));
+    // room for four diffusivities: velocity, magnetic, temperature and (once
+    // enabled) composition
+    GD_TRY(s->Lt.ensure(sizeof(double) * nb * 4));
+    GD_TRY(s->LUt.ensure(sizeof(double) * nb * 4));
     DevBuf dlap, tmpL, tmpS, tmpLU;
     int rc = upload_vec(ctx, dlap, lap);
     if (rc == GD_OK) rc = tmpL.ensure(sizeof(double) * nb);
@@ -490,7 +578,8 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
     if (rc == GD_OK && hipMemsetAsync(s->status.p, 0, sizeof(int), ctx->stream) != hipSuccess) rc = GD_ERR_HIP;
     const double nus[3] = {d->Pm, 1.0, d->Pm / d->Pr};
     {
-        std::vector<double> nl3(nus, nus + 3), nlap((size_t)3 * s->W * nr);
+        std::vector<double> nl3(nus, nus + 3), nlap((size_t)4 * s->W * nr, 0.0);
+        nl3.push_back(0.0);
         for (int k = 0; k < 3; ++k)
             for (size_t q = 0; q < (size_t)s->W * nr; ++q) nlap[k * s->W * nr + q] = nus[k] * lap[q];
         if (rc == GD_OK) rc = upload_vec(ctx, s->nus, nl3);
@@ -525,7 +614,7 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
     GD_TRY(s->part.ensure(sizeof(double) * 3 * 512));
     GD_TRY(s->eout.ensure(sizeof(double) * 4));
     GD_TRY(s->zrow.ensure(sizeof(double) * s->nlm));
-    GD_TRY(s->rdiag.ensure(sizeof(double) * 3 * nr * s->nl));
+    GD_TRY(s->rdiag.ensure(sizeof(double) * 4 * nr * s->nl));
     GD_TRY(gd_launch_recip_diag(ctx->stream, s->LUt.as<double>(), 3, nr, bw, s->nl, s->rdiag.as<double>()));
     GD_HIP(hipMemsetAsync(s->zrow.p, 0, sizeof(double) * s->nlm, ctx->stream));
     GD_HIP(hipMemsetAsync(s->state[0].p, 0, sizeof(double) * s->fs * 10, ctx->stream));
@@ -605,8 +694,14 @@ extern "C" int gd_step_set_state(gd_step *s, const double *re, const double *im,
         a = s->scratch.as<double>();
         b = a + n;
     }
+    if (s->comp && (s->step & 1)) { // the composition state moves with the step parity
+        const size_t pb = sizeof(double) * s->fs * 2;
+        GD_HIP(hipMemcpyAsync(s->cstate[0].as<double>() + 8 * s->fs, s->cstate[1].as<double>() + 8 * s->fs, pb,
+                              hipMemcpyDeviceToDevice, s->ctx->stream));
+    }
     s->step = 0;
     s->syn_ok = false;
+    s->csyn_ok = false;
     GD_TRY(convert_state(s, a, b, cur_state(s), true));
     if (flags & GD_HOST) GD_HIP(hipStreamSynchronize(s->ctx->stream));
     return GD_OK;
@@ -652,6 +747,7 @@ extern "C" int gd_step_restore(gd_step *s, const double *re, const double *im, c
                                const double *nl_im, int step, int flags)
 {
     GD_CHECK(s && re && im, GD_ERR_ARG, "gd_step_restore: null argument");
+    GD_CHECK(!s->comp, GD_ERR_ARG, "gd_step_restore: the composition field is not part of a restore");
     GD_CHECK(step >= 0, GD_ERR_ARG, "gd_step_restore: step %d < 0", step);
     GD_CHECK(step == 0 || (nl_re && nl_im), GD_ERR_ARG, "gd_step_restore: step %d > 0 needs the nonlinear history",
              step);
@@ -768,6 +864,139 @@ extern "C" int gd_step_set_temperature_bc(gd_step *s, const int *bc_inner, const
     }
     // the step's planes are in the device (m-major) mode order
     return gd_scalar_bc_set_perm(s->tbc, bc_inner, bc_outer, values, s->sht.maps.lmaj_of_idx.data());
+}
+
+// Composition (compositional.jl): a passive scalar advected by u,
+// -u.grad C through its own gradient synthesis, ring pass and scalar
+// analysis, CNAB2 with diffusivity Pm/Sc (simulation.jl:130-131), conditions
+// applied before its solve (default: no flux at both boundaries, value 0,
+// compositional.jl:110-115).  Tangential step with the column kernel only.
+extern "C" int gd_step_enable_composition(gd_step *s, double Sc)
+{
+    GD_CHECK(s && Sc > 0.0 && std::isfinite(Sc), GD_ERR_ARG, "gd_step_enable_composition: bad arguments");
+    GD_CHECK(tangential(s), GD_ERR_ARG,
+             "gd_step_enable_composition: built for the tangential step (radials = 0, full_products = 0)");
+    GD_CHECK(s->chain, GD_ERR_ARG, "gd_step_enable_composition: nr=%d bw=%d has no column CNAB2 kernel", s->d.nr,
+             s->d.bw);
+    GD_HIP(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->ctx->stream;
+    GD_HIP(hipStreamSynchronize(st));
+    destroy_graphs(s);
+    const gd_step_desc *d = &s->d;
+    const int nr = d->nr, bw = d->bw;
+    const size_t nb = (size_t)s->W * nr * s->nl;
+    // operator of slot 3
+    std::vector<double> r((size_t)nr * 7), lap((size_t)s->W * nr);
+    GD_TRY(gd_radial_grid(nr, d->rratio, r.data()));
+    GD_TRY(gd_radial_laplacian(nr, bw, r.data(), d->fd_mode, lap.data()));
+    const double nu = d->Pm / Sc;
+    std::vector<double> nlap(lap.size());
+    for (size_t q = 0; q < lap.size(); ++q) nlap[q] = nu * lap[q];
+    GD_HIP(hipMemcpy(s->nus.as<double>() + 3, &nu, sizeof(double), hipMemcpyHostToDevice));
+    GD_HIP(hipMemcpy(s->nuLap.as<double>() + (size_t)3 * s->W * nr, nlap.data(), sizeof(double) * nlap.size(),
+                     hipMemcpyHostToDevice));
+    DevBuf dlap, tmpL, tmpS, tmpLU;
+    int rc = upload_vec(s->ctx, dlap, lap);
+    if (rc == GD_OK) rc = tmpL.ensure(sizeof(double) * nb);
+    if (rc == GD_OK) rc = tmpS.ensure(sizeof(double) * nb);
+    if (rc == GD_OK) rc = tmpLU.ensure(sizeof(double) * nb);
+    if (rc == GD_OK && hipMemsetAsync(s->status.p, 0, sizeof(int), st) != hipSuccess) rc = GD_ERR_HIP;
+    if (rc == GD_OK)
+        rc = gd_cnab2_build(st, dlap.as<double>(), s->rtab.as<double>() + nr, nu, d->dt, d->theta, s->nl, nr, bw,
+                            d->bc_rows == 1, tmpL.as<double>(), tmpS.as<double>(), tmpLU.as<double>(),
+                            s->Lt.as<double>() + 3 * nb, s->LUt.as<double>() + 3 * nb, s->status.as<int>());
+    if (rc == GD_OK) rc = gd_launch_recip_diag(st, s->LUt.as<double>(), 4, nr, bw, s->nl, s->rdiag.as<double>());
+    int stv = 0;
+    if (rc == GD_OK && hipMemcpyAsync(&stv, s->status.p, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
+        rc = GD_ERR_HIP;
+    if (rc == GD_OK && hipStreamSynchronize(st) != hipSuccess) rc = GD_ERR_HIP;
+    dlap.release();
+    tmpL.release();
+    tmpS.release();
+    tmpLU.release();
+    GD_TRY(rc);
+    GD_CHECK(stv == 0, GD_ERR_NUMERIC, "gd_step_enable_composition: zero or non-finite pivot in LU(S_l)");
+    // buffers (zeroed: the planes other than 8, 9 must stay zero)
+    const size_t esz = d->prec == GD_PREC_FP64 ? sizeof(double) : sizeof(float);
+    const int nm = d->mmax + 1;
+    for (DevBuf *b : {&s->cstate[0], &s->cstate[1], &s->cnl[0], &s->cnl[1]}) {
+        GD_TRY(b->ensure(sizeof(double) * s->fs * 10));
+        GD_HIP(hipMemsetAsync(b->p, 0, sizeof(double) * s->fs * 10, st));
+    }
+    GD_TRY(s->csynV.ensure(esz * s->fs * 18));
+    GD_TRY(s->csynS.ensure(esz * s->fs * 10));
+    GD_TRY(s->cgb.ensure(esz * (size_t)nm * s->sht.nlat2 * nr * 12));
+    GD_HIP(hipMemsetAsync(s->csynV.p, 0, esz * s->fs * 18, st));
+    GD_HIP(hipMemsetAsync(s->csynS.p, 0, esz * s->fs * 10, st));
+    if (!s->cbc) GD_TRY(gd_scalar_bc_create(s->ctx, d->lmax, d->mmax, nr, bw, r.data(), d->fd_mode, &s->cbc));
+    std::vector<int> two((size_t)s->nlm, 2);
+    std::vector<double> zero((size_t)2 * s->nlm, 0.0);
+    GD_TRY(gd_scalar_bc_set_perm(s->cbc, two.data(), two.data(), zero.data(), s->sht.maps.lmaj_of_idx.data()));
+    GD_HIP(hipStreamSynchronize(st));
+    s->Sc = Sc;
+    s->comp = true;
+    s->csyn_ok = false;
+    return GD_OK;
+}
+
+// composition planes (nlm, nr) re/im in the reference's l-major layout
+extern "C" int gd_step_set_composition(gd_step *s, const double *re, const double *im, int flags)
+{
+    GD_CHECK(s && re && im, GD_ERR_ARG, "gd_step_set_composition: null argument");
+    GD_CHECK(s->comp, GD_ERR_ARG, "gd_step_set_composition: composition not enabled");
+    GD_HIP(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->ctx->stream;
+    const double *a = re, *b = im;
+    if (flags & GD_HOST) {
+        GD_TRY(s->scratch.ensure(sizeof(double) * s->fs * 2));
+        GD_HIP(hipMemcpyAsync(s->scratch.p, re, sizeof(double) * s->fs, hipMemcpyHostToDevice, st));
+        GD_HIP(hipMemcpyAsync(s->scratch.as<double>() + s->fs, im, sizeof(double) * s->fs, hipMemcpyHostToDevice,
+                              st));
+        a = s->scratch.as<double>();
+        b = a + s->fs;
+    }
+    double *dst = s->cstate[s->step & 1].as<double>();
+    const int *map = s->sht.d_lmaj_of_idx.as<int>();
+    GD_TRY(gd_launch_perm(st, a, map, s->nlm, s->d.nr, dst + 8 * s->fs));
+    GD_TRY(gd_launch_perm(st, b, map, s->nlm, s->d.nr, dst + 9 * s->fs));
+    s->csyn_ok = false;
+    if (flags & GD_HOST) GD_HIP(hipStreamSynchronize(st));
+    return GD_OK;
+}
+
+extern "C" int gd_step_get_composition(gd_step *s, double *re, double *im, int flags)
+{
+    GD_CHECK(s && re && im, GD_ERR_ARG, "gd_step_get_composition: null argument");
+    GD_CHECK(s->comp, GD_ERR_ARG, "gd_step_get_composition: composition not enabled");
+    GD_HIP(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->ctx->stream;
+    double *a = re, *b = im;
+    if (flags & GD_HOST) {
+        GD_TRY(s->scratch.ensure(sizeof(double) * s->fs * 2));
+        a = s->scratch.as<double>();
+        b = a + s->fs;
+    }
+    const double *src = s->cstate[s->step & 1].as<double>();
+    const int *map = s->sht.d_idx_of_lmaj.as<int>();
+    GD_TRY(gd_launch_perm(st, src + 8 * s->fs, map, s->nlm, s->d.nr, a));
+    GD_TRY(gd_launch_perm(st, src + 9 * s->fs, map, s->nlm, s->d.nr, b));
+    if (flags & GD_HOST) {
+        GD_TRY(gd_from_device(s->ctx, re, a, sizeof(double) * s->fs));
+        GD_TRY(gd_from_device(s->ctx, im, b, sizeof(double) * s->fs));
+    }
+    return GD_OK;
+}
+
+// the composition field's bc_type_inner / bc_type_outer / boundary_values
+// (compositional.jl:48-50, 318-370), l-major; captured graphs are dropped
+extern "C" int gd_step_set_composition_bc(gd_step *s, const int *bc_inner, const int *bc_outer, const double *values)
+{
+    GD_CHECK(s && bc_inner && bc_outer && values, GD_ERR_ARG, "gd_step_set_composition_bc: null argument");
+    GD_CHECK(s->comp, GD_ERR_ARG, "gd_step_set_composition_bc: composition not enabled");
+    GD_HIP(hipSetDevice(s->ctx->device));
+    GD_HIP(hipStreamSynchronize(s->ctx->stream));
+    destroy_graphs(s);
+    return gd_scalar_bc_set_perm(s->cbc, bc_inner, bc_outer, values, s->sht.maps.lmaj_of_idx.data());
 }
 
 // the radial grid r_i of the step's shell (fields.jl:108-131), host out[nr]

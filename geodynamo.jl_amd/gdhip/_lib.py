@@ -34,6 +34,7 @@ EXPORTS = [
     "gd_erk2_store_stage_nonlinear", "gd_erk2_finalize_field",
     "gd_batch_create", "gd_batch_run", "gd_batch_destroy",
     "gd_step_get_history", "gd_step_restore", "gd_step_shape", "gd_step_radii", "gd_step_set_temperature_bc",
+    "gd_step_enable_composition", "gd_step_set_composition", "gd_step_get_composition", "gd_step_set_composition_bc",
     "gd_restart_write", "gd_restart_read", "gd_restart_write_arrays", "gd_restart_read_arrays",
     "gd_restart_filename",
     "gd_scalar_bc_create", "gd_scalar_bc_set", "gd_scalar_bc_apply", "gd_scalar_bc_tau", "gd_scalar_bc_destroy",
@@ -143,6 +144,10 @@ def lib():
             "gd_step_shape": (i, [vp, vp, vp, vp]),
             "gd_step_radii": (i, [vp, vp]),
             "gd_step_set_temperature_bc": (i, [vp, vp, vp, vp]),
+            "gd_step_enable_composition": (i, [vp, d]),
+            "gd_step_set_composition": (i, [vp, vp, vp, i]),
+            "gd_step_get_composition": (i, [vp, vp, vp, i]),
+            "gd_step_set_composition_bc": (i, [vp, vp, vp, vp]),
             "gd_restart_write": (i, [vp, ctypes.c_char_p, ctypes.POINTER(RestartMeta), i]),
             "gd_restart_read": (i, [vp, ctypes.c_char_p, ctypes.POINTER(RestartMeta)]),
             "gd_restart_write_arrays": (i, [ctypes.c_char_p, ctypes.POINTER(RestartMeta), i, i, i, vp, i]

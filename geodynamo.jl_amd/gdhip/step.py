@@ -86,6 +86,31 @@ class CNAB2Step:
             raise ValueError("bc types need nlm entries and boundary_values shape (2, nlm)")
         check(lib().gd_step_set_temperature_bc(self._h, ti.ctypes.data, to.ctypes.data, v.ctypes.data))
 
+    def enable_composition(self, Sc=1.0):
+        """Composition as a passive scalar of the step (compositional.jl), Pm/Sc
+        diffusivity, default no-flux conditions; tangential step only."""
+        check(lib().gd_step_enable_composition(self._h, float(Sc)))
+
+    def set_composition(self, C):
+        """C: complex (nr, nlm), l-major; call after set_state."""
+        re, im = np.ascontiguousarray(C.real), np.ascontiguousarray(C.imag)
+        ctx, flags, (a, b) = prepare(re, im, device=self.device)
+        check(lib().gd_step_set_composition(self._h, a, b, flags))
+
+    def get_composition(self):
+        re, im = np.zeros((self.nr, self.nlm)), np.zeros((self.nr, self.nlm))
+        ctx, flags, (a, b) = prepare(re, im, device=self.device)
+        check(lib().gd_step_get_composition(self._h, a, b, flags))
+        return re + 1j * im
+
+    def set_composition_bc(self, bc_type_inner, bc_type_outer, boundary_values):
+        ti = np.ascontiguousarray(bc_type_inner, dtype=np.int32)
+        to = np.ascontiguousarray(bc_type_outer, dtype=np.int32)
+        v = np.ascontiguousarray(np.asarray(boundary_values, dtype=np.float64).T)
+        if ti.shape != (self.nlm,) or to.shape != (self.nlm,) or v.shape != (self.nlm, 2):
+            raise ValueError("bc types need nlm entries and boundary_values shape (2, nlm)")
+        check(lib().gd_step_set_composition_bc(self._h, ti.ctypes.data, to.ctypes.data, v.ctypes.data))
+
     def radii(self):
         r = np.zeros(self.nr)
         check(lib().gd_step_radii(self._h, r.ctypes.data))

@@ -250,6 +250,17 @@ int gd_step_shape(const gd_step *s, int *lmax, int *mmax, int *nr);
  * 0, the reference's default Dirichlet 0).  Drops the step's captured graphs;
  * a batch keeps the conditions its graphs were captured with. */
 int gd_step_set_temperature_bc(gd_step *s, const int *bc_inner, const int *bc_outer, const double *values);
+/* Composition (compositional.jl): a passive scalar C of the tangential step
+ * (radials = 0, full_products = 0), advected by u (-u.grad C, as the
+ * temperature), CNAB2 with diffusivity Pm/Sc (simulation.jl:130-131), its
+ * conditions applied before its solve (default: no flux at both boundaries,
+ * value 0, compositional.jl:110-115; set with gd_step_set_composition_bc).
+ * Planes (nlm, nr) re/im, l-major; set after gd_step_set_state.  Restores and
+ * restart files do not carry it (gd_step_restore refuses). */
+int gd_step_enable_composition(gd_step *s, double Sc);
+int gd_step_set_composition(gd_step *s, const double *re, const double *im, int flags);
+int gd_step_get_composition(gd_step *s, double *re, double *im, int flags);
+int gd_step_set_composition_bc(gd_step *s, const int *bc_inner, const int *bc_outer, const double *values);
 /* radial grid r_i of the step's shell (fields.jl:108-131), host out[nr] */
 int gd_step_radii(const gd_step *s, double *r);
 
