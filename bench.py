@@ -699,6 +699,21 @@ def main():
                 "vs_default_mixed": round(tmx / tb, 3),
                 "state_finite": bool(np.all(np.isfinite(bstep.get_state_complex())))}
             bstep.close()
+            # the mixed step with composition (Sc = 1, no-flux conditions)
+            cstep = gdhip.CNAB2Step(lmax, nlat, nlon, nr, precision="fp32", **kw)
+            cstep.enable_composition(1.0)
+            cstep.set_state_complex(st)
+            cstep.set_composition(gdhip.seeded_state(SEED + 1, lmax, nr, nfields=1, amp=AMP)[0])
+            cstep.run(args.warmup, graph=True)
+            cstep.synchronize()
+            t0 = time.perf_counter()
+            cstep.run(args.steps, graph=True)
+            cstep.synchronize()
+            tc = time.perf_counter() - t0
+            out["scalar_bc"]["step_with_composition"] = {
+                "mixed_steps_per_s": round(args.steps / tc, 2), "vs_default_mixed": round(tmx / tc, 3),
+                "composition_finite": bool(np.all(np.isfinite(cstep.get_composition())))}
+            cstep.close()
         except Exception as exc:  # a secondary measurement never hides the headline line
             out["scalar_bc"] = {"error": repr(exc)[:300]}
     if not args.no_restart:

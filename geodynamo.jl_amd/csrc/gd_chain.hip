@@ -26,8 +26,13 @@ template <int BW, int D, typename TS, bool DIRECT> __global__ void __launch_boun
     constexpr int W = 2 * BW + 1;
     const int n = c.n, nlm = c.nlm;
     const int idx = blockIdx.x * 64 + threadIdx.x;
-    const int f = c.f0 + blockIdx.y, ri = blockIdx.z; // field, 0 re / 1 im
-    const int k = c.kdiff > 0 ? c.kdiff - 1 : (f < 2 ? 0 : (f < 4 ? 1 : 2));
+    // field, 0 re / 1 im; field 5 is the composition: laid out and treated
+    // as the temperature (field 4) of its own buffers, diffusivity slot 3,
+    // boundary rows preset by its condition stage
+    const int fq = c.f0 + blockIdx.y, ri = blockIdx.z;
+    const bool isC = fq == 5;
+    const int f = isC ? 4 : fq;
+    const int k = isC ? 3 : (f < 2 ? 0 : (f < 4 ? 1 : 2));
     // level-uniform tables in LDS: the explicit operator by row (ER doubles per
     // row i: the coefficient of u row i-BW+q in slot q, 0 where that row does
     // not exist, and 1/r_i^2 in slot W), D1 and D2 bands, 1/r, 1/r^2
@@ -59,19 +64,19 @@ template <int BW, int D, typename TS, bool DIRECT> __global__ void __launch_boun
     const double *LU = c.LUt + (size_t)k * W * n * c.nl + l;
     const double *rdg = c.rdiag + (size_t)k * n * c.nl + l;
     const size_t fs = (size_t)nlm * n, off = (size_t)(2 * f + ri) * fs + idx;
-    const double *u = c.st_in + off;
+    const double *u = (isC ? c.c_in : c.st_in) + off;
     // a field whose nonlinear terms are known to be zero reads one zero row (L2-resident)
-    const bool nlz = (c.nl_zero_mask >> f) & 1;
-    const double *pn = nlz ? c.zero_row + idx : c.nl_arr + off;
-    const double *pp = nlz ? c.zero_row + idx : c.np_arr + off;
+    const bool nlz = !isC && ((c.nl_zero_mask >> f) & 1);
+    const double *pn = nlz ? c.zero_row + idx : (isC ? c.c_nl : c.nl_arr) + off;
+    const double *pp = nlz ? c.zero_row + idx : (isC ? c.c_np : c.np_arr) + off;
     const size_t nstride = nlz ? 0 : (size_t)nlm;
-    double *o = c.st_out + off;
+    double *o = (isC ? c.c_out : c.st_out) + off;
     // bit 5 of bc_mask: a boundary-condition stage has already set the
     // temperature rows in st_in; read them as stored, and Dirichlet rhs rows
     // of the temperature take the stored boundary value
-    const bool tkeep = (f == 4) && ((c.bc_mask >> 5) & 1);
+    const bool tkeep = isC || ((f == 4) && ((c.bc_mask >> 5) & 1));
     const bool tbc = (f == 4) && !tkeep;
-    const bool bcr = (c.bc_mask >> f) & 1;
+    const bool bcr = isC ? c.c_bcr != 0 : ((c.bc_mask >> f) & 1);
     const double inv_dt = 1.0 / c.dt, lw = 1.0 - c.theta;
     auto cl = [&](int j) { return j < 0 ? 0 : (j > n - 1 ? n - 1 : j); }; // a valid row
     auto urow = [&](int j) { return j >= 0 && j < n && !(tbc && (j == 0 || j == n - 1)); };
@@ -138,9 +143,10 @@ template <int BW, int D, typename TS, bool DIRECT> __global__ void __launch_boun
     const bool need1 = (f == 4) ? wS : (pol && wB), need2 = pol && wB; // radial derivatives used
     TS *pA = nullptr, *pB = nullptr, *pS = nullptr;
     if (syn) {
-        pA = static_cast<TS *>(c.synV) + (size_t)(colA[f] + ri) * nlm + idx;
-        pB = static_cast<TS *>(c.synV) + (size_t)(colB[f] + ri) * nlm + idx;
-        pS = static_cast<TS *>(c.synS) + (size_t)(colS[f] + ri) * nlm + idx;
+        TS *sv = static_cast<TS *>(isC ? c.c_synV : c.synV), *ss = static_cast<TS *>(isC ? c.c_synS : c.synS);
+        pA = sv + (size_t)(colA[f] + ri) * nlm + idx;
+        pB = sv + (size_t)(colB[f] + ri) * nlm + idx;
+        pS = ss + (size_t)(colS[f] + ri) * nlm + idx;
     }
     double by[D], bd[D], bu[D][BW];
     auto bload = [&](int s, int i) {
@@ -255,9 +261,12 @@ int gd_launch_cnab2_chain(hipStream_t st, const ChainArgs &c, bool syn_fp64)
     GD_CHECK(!c.nl_zero_mask || c.zero_row, GD_ERR_ARG, "cnab2_chain: nl_zero_mask needs zero_row");
     GD_CHECK(!c.synV || (c.synS && c.D1 && c.D2 && c.rtab), GD_ERR_ARG, "cnab2_chain: incomplete synthesis outputs");
     GD_CHECK(gd_cnab2_chain_supported(c.n, c.bw), GD_ERR_ARG, "cnab2_chain: n=%d bw=%d unsupported", c.n, c.bw);
-    GD_CHECK(c.f0 >= 0 && c.nf >= 0 && c.f0 + (c.nf ? c.nf : 5) <= 5, GD_ERR_ARG, "cnab2_chain: fields %d+%d",
-             c.f0, c.nf);
-    dim3 grid((c.nlm + 63) / 64, c.nf ? c.nf : 5, 2);
+    const int nfield = c.nf ? c.nf : (c.c_in ? 6 : 5);
+    GD_CHECK(c.f0 >= 0 && c.nf >= 0 && c.f0 + nfield <= (c.c_in ? 6 : 5), GD_ERR_ARG, "cnab2_chain: fields %d+%d",
+             c.f0, nfield);
+    GD_CHECK(!c.c_in || (c.c_out && c.c_nl && c.c_np && (!c.synV || c.c_synV)), GD_ERR_ARG,
+             "cnab2_chain: incomplete composition buffers");
+    dim3 grid((c.nlm + 63) / 64, nfield, 2);
     const size_t lds = sizeof(double) * (size_t)((((2 * c.bw + 1) + 2) & ~1) + 2 * (2 * c.bw + 1) + 2) * c.n;
     GD_CHECK(lds <= 64 * 1024, GD_ERR_ARG, "cnab2_chain: n=%d bw=%d needs %zu B of LDS", c.n, c.bw, lds);
     const bool direct = c.synV && c.syn_tang;

@@ -37,7 +37,14 @@ struct ChainArgs {
     int syn_tang; // only the u and T/r synthesis columns are needed (the tangential step)
     void *synV, *synS;
     int f0, nf;   // fields f0 .. f0 + nf - 1 (nf = 0: all five)
-    int kdiff;    // 0: diffusivity slot by field (0 0 1 1 2); k + 1: slot k for every field
+    // field 5 (the composition, when c_in is set): five-field buffers of which
+    // planes 8, 9 are used, diffusivity slot 3, rows preset by its conditions,
+    // Dirichlet rows of S_l when c_bcr; nf = 0 then means all six
+    const double *c_in;
+    double *c_out;
+    const double *c_nl, *c_np;
+    void *c_synV, *c_synS;
+    int c_bcr;
 };
 int gd_cnab2_chain_supported(int n, int bw);
 // rdiag[(k*n + i)*nl + l] = 1 / LU_t[k][(i*W + bw)][l] for nk diffusivities

@@ -390,6 +390,15 @@ static int temperature_bc(gd_step *s, int parity, hipStream_t st)
     return gd_scalar_bc_launch(s->tbc, st, cur + 8 * s->fs, cur + 9 * s->fs, 1);
 }
 
+// the composition's conditions (compositional.jl:318-370; default no flux at
+// both boundaries) on the composition state the solve reads
+static int composition_bc(gd_step *s, int parity, hipStream_t st)
+{
+    if (!s->comp) return GD_OK;
+    double *cc = s->cstate[parity].as<double>();
+    return gd_scalar_bc_launch(s->cbc, st, cc + 8 * s->fs, cc + 9 * s->fs, 1);
+}
+
 // one full step with the nl buffers chosen by parity; boot = AB1 bootstrap.
 // Tangential step with the chain kernel (and no per-class timing), s->overlap:
 //   2  the scalar analysis runs on the side stream beside the vector analysis
@@ -437,6 +446,15 @@ static int enqueue_step(gd_step *s, int parity, bool boot, hipEvent_t *ev = null
         c.synV = s->synV.p;
         c.synS = s->synS.p;
         c.syn_tang = tangential(s) ? 1 : 0;
+        if (s->comp) { // the composition as column group 5 of the same launch
+            c.c_in = s->cstate[parity].as<double>();
+            c.c_out = s->cstate[parity ^ 1].as<double>();
+            c.c_nl = s->cnl[parity].as<double>();
+            c.c_np = boot ? c.c_nl : s->cnl[parity ^ 1].as<double>();
+            c.c_synV = s->csynV.p;
+            c.c_synS = s->csynS.p;
+            c.c_bcr = s->d.bc_rows == 1;
+        }
     }
     const bool fp64 = s->d.prec == GD_PREC_FP64;
     if (split) {
@@ -456,6 +474,13 @@ static int enqueue_step(gd_step *s, int parity, bool boot, hipEvent_t *ev = null
             ct.f0 = 4;
             ct.nf = 1;
             GD_TRY(gd_launch_cnab2_chain(st, cu, fp64));
+            if (s->comp) {
+                ChainArgs cc = c;
+                cc.f0 = 5;
+                cc.nf = 1;
+                GD_TRY(composition_bc(s, parity, st));
+                GD_TRY(gd_launch_cnab2_chain(st, cc, fp64));
+            }
             GD_TRY(gd_launch_cnab2_chain(s->side, ct, fp64));
             GD_HIP(hipEventRecord(s->ev_join, s->side));
             GD_HIP(hipStreamWaitEvent(st, s->ev_join, 0));
@@ -465,30 +490,11 @@ static int enqueue_step(gd_step *s, int parity, bool boot, hipEvent_t *ev = null
                 GD_HIP(hipStreamWaitEvent(st, s->ev_join, 0));
             }
             GD_TRY(temperature_bc(s, parity, st));
+            GD_TRY(composition_bc(s, parity, st));
             GD_TRY(gd_launch_cnab2_chain(st, c, fp64));
         }
         s->syn_ok = true; // synV/synS now describe state[parity ^ 1]
-        if (s->comp) {
-            // the composition's conditions (compositional.jl:318-370, default
-            // no flux at both boundaries) on the state the solve reads, then
-            // its CNAB2 column solves with diffusivity slot 3
-            double *cc = s->cstate[parity].as<double>();
-            GD_TRY(gd_scalar_bc_launch(s->cbc, st, cc + 8 * s->fs, cc + 9 * s->fs, 1));
-            ChainArgs ck = c;
-            ck.st_in = cc;
-            ck.st_out = s->cstate[parity ^ 1].as<double>();
-            ck.nl_arr = s->cnl[parity].as<double>();
-            ck.np_arr = boot ? ck.nl_arr : s->cnl[parity ^ 1].as<double>();
-            ck.f0 = 4;
-            ck.nf = 1;
-            ck.kdiff = 4; // slot 3
-            ck.nl_zero_mask = 0;
-            ck.bc_mask = (s->d.bc_rows == 1 ? 0x10 : 0) | 0x20;
-            ck.synV = s->csynV.p;
-            ck.synS = s->csynS.p;
-            GD_TRY(gd_launch_cnab2_chain(st, ck, fp64));
-            s->csyn_ok = true;
-        }
+        if (s->comp) s->csyn_ok = true;
     } else {
         GD_TRY(temperature_bc(s, parity, st));
         GD_TRY(gd_launch_cnab2_step(st, s->nuLap.as<double>(), s->rtab.as<double>() + s->d.nr,
