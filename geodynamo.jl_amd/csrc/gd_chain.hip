@@ -143,8 +143,8 @@ template <int BW, int D, typename TS, bool DIRECT> __global__ void __launch_boun
     const bool need1 = (f == 4) ? wS : (pol && wB), need2 = pol && wB; // radial derivatives used
     TS *pA = nullptr, *pB = nullptr, *pS = nullptr;
     if (syn) {
-        TS *sv = static_cast<TS *>(isC ? c.c_synV : c.synV), *ss = static_cast<TS *>(isC ? c.c_synS : c.synS);
-        pA = sv + (size_t)(colA[f] + ri) * nlm + idx;
+        TS *sv = static_cast<TS *>(c.synV), *ss = static_cast<TS *>(c.synS);
+        pA = sv + (size_t)((isC ? SYNV_COLS - 2 : colA[f]) + ri) * nlm + idx;
         pB = sv + (size_t)(colB[f] + ri) * nlm + idx;
         pS = ss + (size_t)(colS[f] + ri) * nlm + idx;
     }
@@ -184,7 +184,7 @@ template <int BW, int D, typename TS, bool DIRECT> __global__ void __launch_boun
             xw[0] = v;
             if (i >= 0) o[(size_t)nlm * i] = v;
             if constexpr (DIRECT) {
-                if (wA && i >= 0) pA[(size_t)i * 18 * nlm] = (TS)(f == 4 ? v * rinv_[i] : v);
+                if (wA && i >= 0) pA[(size_t)i * SYNV_COLS * nlm] = (TS)(f == 4 ? v * rinv_[i] : v);
             } else {
             const int r = i + BW; // xw[q] = x[r - BW + q]
             if (syn && wA && r >= 0 && r < n) {
@@ -221,7 +221,7 @@ template <int BW, int D, typename TS, bool DIRECT> __global__ void __launch_boun
                     bv = -lf * rinv2 * x;
                     sv = lf * bv * rinv2;
                 }
-                const size_t ov = (size_t)r * 18 * nlm, os = (size_t)r * 10 * nlm;
+                const size_t ov = (size_t)r * SYNV_COLS * nlm, os = (size_t)r * 10 * nlm;
                 pA[ov] = (TS)a;
                 if (wB) pB[ov] = (TS)bv;
                 if (wS) pS[os] = (TS)sv;
@@ -264,7 +264,7 @@ int gd_launch_cnab2_chain(hipStream_t st, const ChainArgs &c, bool syn_fp64)
     const int nfield = c.nf ? c.nf : (c.c_in ? 6 : 5);
     GD_CHECK(c.f0 >= 0 && c.nf >= 0 && c.f0 + nfield <= (c.c_in ? 6 : 5), GD_ERR_ARG, "cnab2_chain: fields %d+%d",
              c.f0, nfield);
-    GD_CHECK(!c.c_in || (c.c_out && c.c_nl && c.c_np && (!c.synV || c.c_synV)), GD_ERR_ARG,
+    GD_CHECK(!c.c_in || (c.c_out && c.c_nl && c.c_np && (!c.synV || c.syn_tang)), GD_ERR_ARG,
              "cnab2_chain: incomplete composition buffers");
     dim3 grid((c.nlm + 63) / 64, nfield, 2);
     const size_t lds = sizeof(double) * (size_t)((((2 * c.bw + 1) + 2) & ~1) + 2 * (2 * c.bw + 1) + 2) * c.n;

@@ -20,6 +20,10 @@ int gd_launch_cnab2_step(hipStream_t st, const double *nuLap, const double *rinv
 // Arguments of the column-register CNAB2 kernel (gd_radial.hip).  synV/synS
 // (element type float or double): when set, the next step's synthesis columns
 // are written from the new state.
+// columns per radial level of the step's Legendre-synthesis input synV:
+// 0..17 as listed in gd_step.hip, 18, 19 the composition's C/r (re, im)
+constexpr int SYNV_COLS = 20;
+
 struct ChainArgs {
     const double *nuLap, *rinv2, *nus, *LUt;
     const int *l_of_idx;
@@ -43,8 +47,7 @@ struct ChainArgs {
     const double *c_in;
     double *c_out;
     const double *c_nl, *c_np;
-    void *c_synV, *c_synS;
-    int c_bcr;
+    int c_bcr;                 // its synthesis columns: SYNV_COLS - 2, - 1 of synV
 };
 int gd_cnab2_chain_supported(int n, int bw);
 // rdiag[(k*n + i)*nl + l] = 1 / LU_t[k][(i*W + bw)][l] for nk diffusivities

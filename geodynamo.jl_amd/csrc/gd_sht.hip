@@ -741,9 +741,10 @@ template <typename T> int gd_launch_synth_vec(hipStream_t st, const SynthArgs &a
     GD_TRY(synth_check(a, "synth_vec"));
     GD_CHECK(a.slot0 + 8 * a.nf <= a.nslot, GD_ERR_ARG, "synth_vec: slots %d+%d exceed %d", a.slot0, 8 * a.nf, a.nslot);
     if (a.nf == 5) launch_synth<T, true, 5, 5, 16>(st, a);
+    else if (a.nf == 3) launch_synth<T, true, 3, 3, 16>(st, a); // 12 columns per level, 3 waves = 4 levels
     else if (a.nf == 2) launch_synth<T, true, 2, 4, 16>(st, a);
     else if (a.nf == 1) launch_synth<T, true, 1, 1, 16>(st, a);
-    else GD_CHECK(false, GD_ERR_ARG, "synth_vec: nf=%d (1, 2 or 5)", a.nf);
+    else GD_CHECK(false, GD_ERR_ARG, "synth_vec: nf=%d (1, 2, 3 or 5)", a.nf);
     return gd_check_launch("synth_kernel<vec>");
 }
 
@@ -794,6 +795,7 @@ template <typename T> static int launch_ana(hipStream_t st, const AnaArgs &a, bo
     if (vec && a.nf == 2 && ana_res<T, true, 2>(st, a)) return gd_check_launch("ana_res_kernel<vec>");
     if (vec && a.nf == 1 && ana_res<T, true, 1>(st, a)) return gd_check_launch("ana_res_kernel<vec>");
     if (!vec && a.nf == 1 && ana_res<T, false, 1>(st, a)) return gd_check_launch("ana_res_kernel<scal>");
+    if (!vec && a.nf == 2 && ana_res<T, false, 2>(st, a)) return gd_check_launch("ana_res_kernel<scal>");
     // streaming form (large nlat, or LDS beyond 64 KB)
     const int VC = vec ? 4 * a.nf : 2 * a.nf;
     const int LT = 16 * a.nf / VC;
@@ -1080,11 +1082,11 @@ template <typename T, int MODE, int NC> __global__ void __launch_bounds__(256) r
     }
     __syncthreads();
     // fields in / products out: compile-time for the step modes
-    constexpr int NFC = MODE == RING_STEP ? 15 : MODE == RING_STEP_TANG ? 4 : 0;
-    constexpr int NPC = MODE == RING_STEP ? 5 : MODE == RING_STEP_TANG ? 3 : 0;
+    constexpr int NFC = MODE == RING_STEP ? 15 : MODE == RING_STEP_TANG ? 4 : MODE == RING_STEP_TANG_C ? 6 : 0;
+    constexpr int NPC = MODE == RING_STEP ? 5 : MODE == RING_STEP_TANG ? 3 : MODE == RING_STEP_TANG_C ? 4 : 0;
     const int NF = NFC ? NFC : a.nfield;
 
-    if (MODE == RING_SYNTH || MODE == RING_STEP || MODE == RING_STEP_TANG) {
+    if (MODE == RING_SYNTH || MODE == RING_STEP || MODE == RING_STEP_TANG || MODE == RING_STEP_TANG_C) {
         // Hermitian spectra of the north (X) and south (Y) rings packed as Z = X + iY
         const T *src = static_cast<const T *>(a.fb) + ((size_t)i * a.nr + r) * nm * a.nslot_f;
         // fixed trip count, fully unrolled: every thread's global loads are in
@@ -1136,20 +1138,20 @@ template <typename T, int MODE, int NC> __global__ void __launch_bounds__(256) r
     }
 
     int NP;
-    if constexpr (MODE == RING_STEP_TANG) {
+    if constexpr (MODE == RING_STEP_TANG || MODE == RING_STEP_TANG_C) {
         // the products of RING_STEP with u_r = omega_r = B_r = j_r = 0: the
         // tangential parts of u x omega, j x B and u x B vanish and u_r dT/dr
         // drops out; the Coriolis term and the tangential advection of T stay,
         // evaluated in the same order as there
-        // fields: 0 u_t 1 u_p 2 gT_t 3 gT_p
-        NP = 3;
+        // fields: 0 u_t 1 u_p 2 gT_t 3 gT_p (4 gC_t 5 gC_p: -u.grad C as product 3)
+        NP = NPC;
         const T ro = (T)a.ro;
         const T *costh = static_cast<const T *>(a.costh);
         for (int j = tid; j < n; j += nthr) {
-            C2<T> f[4];
+            C2<T> f[NFC];
 #pragma unroll
-            for (int F = 0; F < 4; ++F) f[F] = buf[F * ld + sk(j)];
-            C2<T> o[3];
+            for (int F = 0; F < NFC; ++F) f[F] = buf[F * ld + sk(j)];
+            C2<T> o[NPC];
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int ti = a.coriolis_index ? (h ? is : i) : (j < a.nlat - 1 ? j : a.nlat - 1);
@@ -1169,9 +1171,15 @@ template <typename T, int MODE, int NC> __global__ void __launch_bounds__(256) r
                     o[1].x = ap;
                     o[2].x = aT;
                 }
+                if constexpr (NFC == 6) {
+                    const T ct = h ? f[4].y : f[4].x, cp = h ? f[5].y : f[5].x;
+                    const T aC = -(ut * ct + up * cp);
+                    if (h) o[3].y = aC;
+                    else o[3].x = aC;
+                }
             }
 #pragma unroll
-            for (int P = 0; P < 3; ++P) buf[P * ld + sk(j)] = o[P];
+            for (int P = 0; P < NPC; ++P) buf[P * ld + sk(j)] = o[P];
         }
         __syncthreads();
     } else if (MODE == RING_ANALYSIS) {
@@ -1281,7 +1289,7 @@ template <typename T, int MODE> static void ring_dispatch(hipStream_t st, dim3 g
 template <typename T> int gd_launch_ring(hipStream_t st, int mode, const RingArgs &a)
 {
     const int nthr = RING_THREADS;
-    int nf = (mode == RING_STEP) ? 15 : (mode == RING_STEP_TANG) ? 4 : a.nfield;
+    int nf = (mode == RING_STEP) ? 15 : (mode == RING_STEP_TANG) ? 4 : (mode == RING_STEP_TANG_C) ? 6 : a.nfield;
     GD_CHECK(nf >= 1 && nf <= 15, GD_ERR_ARG, "ring: nfield=%d", nf);
     GD_CHECK((long)nf * (a.mmax + 1) <= 8L * nthr, GD_ERR_ARG, "ring: %d fields x %d orders exceed one load batch",
              nf, a.mmax + 1);
@@ -1297,6 +1305,7 @@ template <typename T> int gd_launch_ring(hipStream_t st, int mode, const RingArg
     case RING_SYNTH: ring_dispatch<T, RING_SYNTH>(st, grid, lds, a); break;
     case RING_ANALYSIS: ring_dispatch<T, RING_ANALYSIS>(st, grid, lds, a); break;
     case RING_STEP_TANG: ring_dispatch<T, RING_STEP_TANG>(st, grid, lds, a); break;
+    case RING_STEP_TANG_C: ring_dispatch<T, RING_STEP_TANG_C>(st, grid, lds, a); break;
     default: ring_dispatch<T, RING_STEP>(st, grid, lds, a); break;
     }
     return gd_check_launch("ring_kernel");

@@ -65,7 +65,9 @@ struct RingArgs {
 // B_r, j_r, dT/dr), 5 products out.  RING_STEP_TANG: the step when every
 // radial component is zero -- 4 fields in (u_theta, u_phi, (grad T)_theta,
 // (grad T)_phi), 3 products out (the force and -u.grad T; u x B vanishes).
-enum { RING_SYNTH = 0, RING_ANALYSIS = 1, RING_STEP = 2, RING_STEP_TANG = 3 };
+// RING_STEP_TANG_C: the same with the composition's gradient as fields 4, 5
+// and -u.grad C as a fourth product.
+enum { RING_SYNTH = 0, RING_ANALYSIS = 1, RING_STEP = 2, RING_STEP_TANG = 3, RING_STEP_TANG_C = 4 };
 
 int gd_launch_plm(hipStream_t st, const double *x, const int *moff, int lmax, int mmax, int nlat2, double *tab64,
                   float *tab32, int nlm);

@@ -64,10 +64,10 @@ struct gd_step {
     // step): state and nonlinear history in the planes 8, 9 (field 4) of
     // five-field buffers whose other planes stay zero, so the CNAB2 kernel
     // treats it as a temperature-like field with diffusivity slot 3 (Pm/Sc);
-    // its own synthesis columns and ring output
+    // its own synthesis columns
     bool comp = false;
     double Sc = 1.0;
-    DevBuf cstate[2], cnl[2], csynV, csynS, cgb;
+    DevBuf cstate[2], cnl[2];
     gd_scalar_bc *cbc = nullptr;
     bool csyn_ok = false;
 };
@@ -75,6 +75,8 @@ struct gd_step {
 // ---------------------------------------------------------------- spectral preparation
 // One thread per (idx, level).  synV columns per level (c = 2k + re/im):
 //   k: 0 u_pol 1 u_tor 2 w_pol 3 w_tor 4 B_pol 5 B_tor 6 j_pol 7 j_tor 8 T/r
+//   9 C/r (the composition, when enabled; written by comp_pre_kernel and the
+//   CNAB2 kernel)
 // synS columns per level: k: 0 u_r 1 w_r 2 B_r 3 j_r 4 dT/dr
 template <typename T>
 __global__ void __launch_bounds__(256) spectral_pre_kernel(const double *__restrict__ st, const int *__restrict__ l_of_idx,
@@ -112,7 +114,7 @@ __global__ void __launch_bounds__(256) spectral_pre_kernel(const double *__restr
         const double wpol = -lf * rinv2 * ut;
         const double jtor = lf * rinv2 * Bp - d2B - 2.0 * rinv * d1B;
         const double jpol = -lf * rinv2 * Bt;
-        T *v = synV + (size_t)r * 18 * nlm + idx;
+        T *v = synV + (size_t)r * SYNV_COLS * nlm + idx;
         v[(0 + reim) * (size_t)nlm] = (T)up;
         v[(2 + reim) * (size_t)nlm] = (T)ut;
         v[(4 + reim) * (size_t)nlm] = (T)wpol;
@@ -167,10 +169,27 @@ template <typename T> static int spectral_pre(gd_step *s)
     return GD_OK;
 }
 
-// the composition's columns (C/r in columns 16, 17 of csynV)
+// the composition's synthesis columns C/r (SYNV_COLS - 2, - 1 of synV) from
+// planes 8, 9 of its state buffer, as the CNAB2 kernel writes them each step
+template <typename T>
+__global__ void __launch_bounds__(256) comp_pre_kernel(const double *__restrict__ cst, const double *__restrict__ rtab,
+                                                       int nlm, int nr, T *__restrict__ synV)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+    if (idx >= nlm) return;
+    const size_t fs = (size_t)nlm * nr, o = (size_t)r * nlm + idx;
+    const double rinv = rtab[r + nr * 2];
+    T *v = synV + (size_t)r * SYNV_COLS * nlm + idx;
+    v[(size_t)(SYNV_COLS - 2) * nlm] = (T)(cst[8 * fs + o] * rinv);
+    v[(size_t)(SYNV_COLS - 1) * nlm] = (T)(cst[9 * fs + o] * rinv);
+}
+
 template <typename T> static int spectral_pre_comp(gd_step *s)
 {
-    GD_TRY(spectral_pre_into<T>(s, s->cstate[s->step & 1].as<double>(), s->csynV, s->csynS));
+    const int nr = s->d.nr, nlm = s->nlm;
+    hipLaunchKernelGGL((comp_pre_kernel<T>), dim3((nlm + 255) / 256, nr), dim3(256), 0, s->ctx->stream,
+                       s->cstate[s->step & 1].as<double>(), s->rtab.as<double>(), nlm, nr, s->synV.as<T>());
+    GD_TRY(gd_check_launch("comp_pre_kernel"));
     s->csyn_ok = true;
     return GD_OK;
 }
@@ -205,7 +224,7 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEven
     a.mmax = p->mmax;
     a.out = s->fb.p;
     a.in = s->synV.p;
-    a.cin = 18;
+    a.cin = SYNV_COLS;
     a.tab0 = gd_sht_tab(p, 1);
     a.tab1 = gd_sht_tab(p, 2);
     a.slot0 = 0;
@@ -232,54 +251,25 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEven
         // slots, 24 with the composition's gradient (S = C/r) in slots 16..23
         const int nslot = s->comp ? 24 : 16;
         a.nslot = nslot;
-        a.nf = 2;
-        static const int vmap[8] = {0, 1, 2, 3, 16, 17, -1, -1};
-        for (int k = 0; k < 20; ++k) a.colmap[k] = k < 8 ? vmap[k] : -1;
+        a.nf = s->comp ? 3 : 2;
+        static const int vmap[12] = {0, 1, 2, 3, 16, 17, -1, -1, SYNV_COLS - 2, SYNV_COLS - 1, -1, -1};
+        for (int k = 0; k < 20; ++k) a.colmap[k] = k < 4 * a.nf ? vmap[k] : -1;
+        if (s->comp && !s->csyn_ok) GD_TRY(spectral_pre_comp<T>(s));
         GD_TRY(gd_launch_synth_vec<T>(st, a));
-        if (s->comp) {
-            if (!s->csyn_ok) GD_TRY(spectral_pre_comp<T>(s));
-            SynthArgs ac = a;
-            ac.in = s->csynV.p;
-            ac.nf = 1;
-            for (int k = 0; k < 20; ++k) ac.colmap[k] = -1;
-            ac.colmap[0] = 16;
-            ac.colmap[1] = 17;
-            ac.slot0 = 16;
-            GD_TRY(gd_launch_synth_vec<T>(st, ac));
-        }
         GD_MARK(2);
         GD_MARK(3);
         r.nslot_f = nslot;
-        r.nslot_g = 12;
-        r.nfield = 4;
-        static const int fn[4] = {0, 2, 8, 10}, fsou[4] = {4, 6, 12, 14};
-        for (int F = 0; F < 4; ++F) {
+        // with composition the ring also takes grad C (fields 4, 5) and forms
+        // -u.grad C (compute_composition_nonlinear!, compositional.jl:219-260)
+        // as a fourth product; GB records of 12 slots, 16 with it
+        r.nfield = s->comp ? 6 : 4;
+        r.nslot_g = s->comp ? 16 : 12;
+        static const int fn[6] = {0, 2, 8, 10, 16, 18}, fsou[6] = {4, 6, 12, 14, 20, 22};
+        for (int F = 0; F < r.nfield; ++F) {
             r.fn_off[F] = fn[F];
             r.fs_off[F] = fsou[F];
         }
-        GD_TRY(gd_launch_ring<T>(st, RING_STEP_TANG, r));
-        if (s->comp) {
-            // the same ring pass with grad C in place of grad T: its third
-            // product is -u.grad C (compute_composition_nonlinear!,
-            // compositional.jl:219-260); the Coriolis products are not used
-            RingArgs rc = r;
-            rc.gb = s->cgb.p;
-            rc.fn_off[2] = 16;
-            rc.fn_off[3] = 18;
-            rc.fs_off[2] = 20;
-            rc.fs_off[3] = 22;
-            GD_TRY(gd_launch_ring<T>(st, RING_STEP_TANG, rc));
-            AnaArgs gc = g;
-            gc.gb = s->cgb.p;
-            gc.nslot_g = 12;
-            gc.gslot0 = 8;
-            gc.nf = 1;
-            gc.tab0 = gd_sht_tab(p, 0);
-            gc.tab1 = nullptr;
-            gc.out[0] = s->cnl[s->step & 1].as<double>() + 8 * fs;
-            gc.scale[0] = 1;
-            GD_TRY(gd_launch_ana<T>(st, gc, false, maxgroup));
-        }
+        GD_TRY(gd_launch_ring<T>(st, s->comp ? RING_STEP_TANG_C : RING_STEP_TANG, r));
         GD_MARK(4);
         if (side) {
             GD_HIP(hipEventRecord(s->ev_ring, st));
@@ -287,7 +277,7 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEven
         }
         // force -> (pol, tor) of velocity nl (velocity.jl:331); the magnetic nl
         // is u x B = 0 here and its planes stay zero
-        g.nslot_g = 12;
+        g.nslot_g = r.nslot_g;
         g.nf = 1;
         g.gslot0 = 0;
         g.tab0 = gd_sht_tab(p, 1);
@@ -361,6 +351,11 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEven
     g.tab1 = nullptr;
     g.out[0] = nl_out + 8 * fs; // -u.grad T -> T
     g.scale[0] = 1;
+    if (s->comp && tangential(s)) { // and -u.grad C -> the composition's nl (GB slots 12..15)
+        g.nf = 2;
+        g.out[1] = s->cnl[s->step & 1].as<double>() + 8 * fs;
+        g.scale[1] = 1;
+    }
     GD_TRY(gd_launch_ana<T>(side ? s->side : st, g, false, maxgroup));
     GD_MARK(6);
     return GD_OK;
@@ -451,8 +446,6 @@ static int enqueue_step(gd_step *s, int parity, bool boot, hipEvent_t *ev = null
             c.c_out = s->cstate[parity ^ 1].as<double>();
             c.c_nl = s->cnl[parity].as<double>();
             c.c_np = boot ? c.c_nl : s->cnl[parity ^ 1].as<double>();
-            c.c_synV = s->csynV.p;
-            c.c_synS = s->csynS.p;
             c.c_bcr = s->d.bc_rows == 1;
         }
     }
@@ -530,7 +523,7 @@ static int step_release(gd_step *s)
     s->tbc = nullptr;
     if (s->cbc) (void)gd_scalar_bc_destroy(s->cbc);
     s->cbc = nullptr;
-    for (DevBuf *b : {&s->cstate[0], &s->cstate[1], &s->cnl[0], &s->cnl[1], &s->csynV, &s->csynS, &s->cgb})
+    for (DevBuf *b : {&s->cstate[0], &s->cstate[1], &s->cnl[0], &s->cnl[1]})
         b->release();
     if (s->side) (void)hipStreamDestroy(s->side);
     s->side = nullptr;
@@ -613,7 +606,7 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
     GD_TRY(s->state[1].ensure(sizeof(double) * s->fs * 10));
     GD_TRY(s->nlbuf[0].ensure(sizeof(double) * s->fs * 10));
     GD_TRY(s->nlbuf[1].ensure(sizeof(double) * s->fs * 10));
-    GD_TRY(s->synV.ensure(esz * s->fs * 18));
+    GD_TRY(s->synV.ensure(esz * s->fs * SYNV_COLS));
     GD_TRY(s->synS.ensure(esz * s->fs * 10));
     GD_TRY(s->fb.ensure(esz * (size_t)s->sht.nlat2 * nr * nm * 60));
     GD_TRY(s->gb.ensure(esz * (size_t)nm * s->sht.nlat2 * nr * 20));
@@ -924,16 +917,10 @@ extern "C" int gd_step_enable_composition(gd_step *s, double Sc)
     GD_CHECK(stv == 0, GD_ERR_NUMERIC, "gd_step_enable_composition: zero or non-finite pivot in LU(S_l)");
     // buffers (zeroed: the planes other than 8, 9 must stay zero)
     const size_t esz = d->prec == GD_PREC_FP64 ? sizeof(double) : sizeof(float);
-    const int nm = d->mmax + 1;
     for (DevBuf *b : {&s->cstate[0], &s->cstate[1], &s->cnl[0], &s->cnl[1]}) {
         GD_TRY(b->ensure(sizeof(double) * s->fs * 10));
         GD_HIP(hipMemsetAsync(b->p, 0, sizeof(double) * s->fs * 10, st));
     }
-    GD_TRY(s->csynV.ensure(esz * s->fs * 18));
-    GD_TRY(s->csynS.ensure(esz * s->fs * 10));
-    GD_TRY(s->cgb.ensure(esz * (size_t)nm * s->sht.nlat2 * nr * 12));
-    GD_HIP(hipMemsetAsync(s->csynV.p, 0, esz * s->fs * 18, st));
-    GD_HIP(hipMemsetAsync(s->csynS.p, 0, esz * s->fs * 10, st));
     if (!s->cbc) GD_TRY(gd_scalar_bc_create(s->ctx, d->lmax, d->mmax, nr, bw, r.data(), d->fd_mode, &s->cbc));
     std::vector<int> two((size_t)s->nlm, 2);
     std::vector<double> zero((size_t)2 * s->nlm, 0.0);
