@@ -442,7 +442,8 @@ std::string geometry_of(const gd_restart_meta *meta)
 
 extern "C" int gd_restart_write_arrays(const char *path, const gd_restart_meta *meta, int lmax, int mmax, int nr,
                                        const double *r, int step, const double *re, const double *im,
-                                       const double *nl_re, const double *nl_im, int precision)
+                                       const double *nl_re, const double *nl_im, int precision,
+                                       const gd_restart_comp *comp)
 {
     GD_CHECK(path && re && im && r, GD_ERR_ARG, "gd_restart_write_arrays: null argument");
     GD_CHECK(lmax >= 0 && mmax >= 0 && mmax <= lmax && nr >= 2, GD_ERR_ARG,
@@ -453,6 +454,9 @@ extern "C" int gd_restart_write_arrays(const char *path, const gd_restart_meta *
     GD_CHECK((nl_re == nullptr) == (nl_im == nullptr), GD_ERR_ARG,
              "gd_restart_write_arrays: give both history planes or neither");
     GD_CHECK(!nl_re || step > 0, GD_ERR_ARG, "gd_restart_write_arrays: a nonlinear history needs step > 0");
+    const bool wc = comp && comp->re;
+    GD_CHECK(!wc || comp->im, GD_ERR_ARG, "gd_restart_write_arrays: composition needs both planes");
+    const bool wch = wc && comp->nl_re && comp->nl_im && nl_re;
     const int ft = precision == GD_PREC_FP64 ? NC_DOUBLE : NC_FLOAT;
     std::vector<int32_t> lv, mv;
     lm_pairs(lmax, mmax, lv, mv);
@@ -559,6 +563,23 @@ extern "C" int gd_restart_write_arrays(const char *path, const gd_restart_meta *
     };
     field_vars("", re, im, "spectral_space");
     if (nl_re) field_vars("_prev_nonlinear", nl_re, nl_im, "spectral_space_ab2_history");
+    // composition_spectral (setup_field_variables!, outputs_writer.jl:507-522)
+    auto comp_vars = [&](const std::string &stem, const double *pr, const double *pi, const char *what) {
+        for (int ri = 0; ri < 2; ++ri) {
+            WVar f;
+            f.name = std::string("composition_spectral") + stem + (ri ? "_imag" : "_real");
+            f.dims = {d_r, d_lm};
+            f.type = ft;
+            f.f64 = ri ? pi : pr;
+            f.count = fs;
+            f.atts = {att_text("long_name", std::string("composition_spectral") + stem +
+                                                (ri ? "_imaginary_coefficients" : "_real_coefficients")),
+                      att_text("representation", what)};
+            add(f);
+        }
+    };
+    if (wc) comp_vars("", comp->re, comp->im, "spectral_space");
+    if (wch) comp_vars("_prev_nonlinear", comp->nl_re, comp->nl_im, "spectral_space_ab2_history");
     // restart scalars (write_restart!, :1252-1256)
     v = WVar{};
     v.name = "last_output_time";
@@ -585,7 +606,7 @@ extern "C" int gd_restart_write_arrays(const char *path, const gd_restart_meta *
 
 extern "C" int gd_restart_read_arrays(const char *path, int lmax, int mmax, int nr, double *r, int *step,
                                       double *re, double *im, double *nl_re, double *nl_im, int *has_history,
-                                      gd_restart_meta *meta)
+                                      gd_restart_meta *meta, gd_restart_comp *comp)
 {
     GD_CHECK(path && re && im && step && has_history, GD_ERR_ARG, "gd_restart_read_arrays: null argument");
     GD_CHECK(lmax >= 0 && mmax >= 0 && mmax <= lmax && nr >= 2, GD_ERR_ARG,
@@ -655,6 +676,21 @@ extern "C" int gd_restart_read_arrays(const char *path, int lmax, int mmax, int 
             GD_TRY(plane(std::string(COMPONENTS[k]) + "_prev_nonlinear_real", nl_re + (size_t)k * fs));
             GD_TRY(plane(std::string(COMPONENTS[k]) + "_prev_nonlinear_imag", nl_im + (size_t)k * fs));
         }
+    if (comp) {
+        comp->present = rd.var("composition_spectral_real") && rd.var("composition_spectral_imag") ? 1 : 0;
+        comp->has_history = comp->present && hist && rd.var("composition_spectral_prev_nonlinear_real") &&
+                                    rd.var("composition_spectral_prev_nonlinear_imag")
+                                ? 1
+                                : 0;
+        if (comp->present && comp->re && comp->im) {
+            GD_TRY(plane("composition_spectral_real", comp->re));
+            GD_TRY(plane("composition_spectral_imag", comp->im));
+        }
+        if (comp->has_history && comp->nl_re && comp->nl_im) {
+            GD_TRY(plane("composition_spectral_prev_nonlinear_real", comp->nl_re));
+            GD_TRY(plane("composition_spectral_prev_nonlinear_imag", comp->nl_im));
+        }
+    }
     if (r) {
         const RVar *v = rd.var("r");
         GD_CHECK(v && v->count == (size_t)nr, GD_ERR_ARG, "restart %s: variable r missing", path);
@@ -698,7 +734,7 @@ extern "C" int gd_restart_write(gd_step *s, const char *path, const gd_restart_m
     GD_TRY(gd_step_shape(s, &lmax, &mmax, &nr));
     int nlm = 0;
     for (int l = 0; l <= lmax; ++l) nlm += (l < mmax ? l : mmax) + 1;
-    const size_t n = (size_t)5 * nlm * nr;
+    const size_t fs = (size_t)nlm * nr, n = 5 * fs;
     std::vector<double> re(n), im(n), r(nr), nre, nim;
     GD_TRY(gd_step_get_state(s, re.data(), im.data(), GD_HOST));
     const int step = gd_step_count(s);
@@ -708,8 +744,23 @@ extern "C" int gd_restart_write(gd_step *s, const char *path, const gd_restart_m
         GD_TRY(gd_step_get_history(s, nre.data(), nim.data(), GD_HOST));
     }
     GD_TRY(gd_step_radii(s, r.data()));
+    // the composition, when the step carries one
+    gd_restart_comp comp{};
+    std::vector<double> c(4 * fs);
+    const bool has_c = gd_step_has_composition(s) != 0;
+    if (has_c) {
+        comp.re = c.data();
+        comp.im = c.data() + fs;
+        GD_TRY(gd_step_get_composition(s, comp.re, comp.im, GD_HOST));
+        if (step > 0) {
+            comp.nl_re = c.data() + 2 * fs;
+            comp.nl_im = c.data() + 3 * fs;
+            GD_TRY(gd_step_get_composition_history(s, comp.nl_re, comp.nl_im, GD_HOST));
+        }
+    }
     return gd_restart_write_arrays(path, meta, lmax, mmax, nr, r.data(), step, re.data(), im.data(),
-                                   step > 0 ? nre.data() : nullptr, step > 0 ? nim.data() : nullptr, precision);
+                                   step > 0 ? nre.data() : nullptr, step > 0 ? nim.data() : nullptr, precision,
+                                   has_c ? &comp : nullptr);
 }
 
 extern "C" int gd_restart_read(gd_step *s, const char *path, gd_restart_meta *meta)
@@ -719,14 +770,29 @@ extern "C" int gd_restart_read(gd_step *s, const char *path, gd_restart_meta *me
     GD_TRY(gd_step_shape(s, &lmax, &mmax, &nr));
     int nlm = 0;
     for (int l = 0; l <= lmax; ++l) nlm += (l < mmax ? l : mmax) + 1;
-    const size_t n = (size_t)5 * nlm * nr;
-    std::vector<double> re(n), im(n), nre(n), nim(n);
+    const size_t fs = (size_t)nlm * nr, n = 5 * fs;
+    std::vector<double> re(n), im(n), nre(n), nim(n), c(4 * fs);
     int step = 0, hist = 0;
+    const bool has_c = gd_step_has_composition(s) != 0;
+    gd_restart_comp comp{};
+    comp.re = c.data();
+    comp.im = c.data() + fs;
+    comp.nl_re = c.data() + 2 * fs;
+    comp.nl_im = c.data() + 3 * fs;
     GD_TRY(gd_restart_read_arrays(path, lmax, mmax, nr, nullptr, &step, re.data(), im.data(), nre.data(), nim.data(),
-                                  &hist, meta));
+                                  &hist, meta, has_c ? &comp : nullptr));
+    // the reference reads composition_spectral only where the run has one
+    // (read_restart!, outputs_writer.jl:1327-1339); a step with composition
+    // needs it from the file
+    GD_CHECK(!has_c || comp.present, GD_ERR_ARG, "gd_restart_read: %s holds no composition_spectral", path);
+    const bool h = hist && (!has_c || comp.has_history);
     // without a stored history the next step is an AB1 bootstrap, as at step 0
-    return gd_step_restore(s, re.data(), im.data(), hist ? nre.data() : nullptr, hist ? nim.data() : nullptr,
-                           hist ? step : 0, GD_HOST);
+    GD_TRY(gd_step_restore(s, re.data(), im.data(), h ? nre.data() : nullptr, h ? nim.data() : nullptr, h ? step : 0,
+                           GD_HOST));
+    if (has_c)
+        GD_TRY(gd_step_restore_composition(s, comp.re, comp.im, h ? comp.nl_re : nullptr, h ? comp.nl_im : nullptr,
+                                           GD_HOST));
+    return GD_OK;
 }
 
 extern "C" int gd_restart_filename(const char *output_dir, const char *prefix, const char *file_type,

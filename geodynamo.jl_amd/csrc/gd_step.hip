@@ -746,7 +746,6 @@ extern "C" int gd_step_restore(gd_step *s, const double *re, const double *im, c
                                const double *nl_im, int step, int flags)
 {
     GD_CHECK(s && re && im, GD_ERR_ARG, "gd_step_restore: null argument");
-    GD_CHECK(!s->comp, GD_ERR_ARG, "gd_step_restore: the composition field is not part of a restore");
     GD_CHECK(step >= 0, GD_ERR_ARG, "gd_step_restore: step %d < 0", step);
     GD_CHECK(step == 0 || (nl_re && nl_im), GD_ERR_ARG, "gd_step_restore: step %d > 0 needs the nonlinear history",
              step);
@@ -977,6 +976,67 @@ extern "C" int gd_step_get_composition(gd_step *s, double *re, double *im, int f
         GD_TRY(gd_from_device(s->ctx, re, a, sizeof(double) * s->fs));
         GD_TRY(gd_from_device(s->ctx, im, b, sizeof(double) * s->fs));
     }
+    return GD_OK;
+}
+
+extern "C" int gd_step_has_composition(const gd_step *s) { return s && s->comp ? 1 : 0; }
+
+// planes 8, 9 of a composition buffer <-> (nlm, nr) l-major host or device planes
+static int comp_planes(gd_step *s, double *buf, double *re, double *im, int flags, bool to_device)
+{
+    hipStream_t st = s->ctx->stream;
+    double *a = re, *b = im;
+    if (flags & GD_HOST) {
+        GD_TRY(s->scratch.ensure(sizeof(double) * s->fs * 2));
+        a = s->scratch.as<double>();
+        b = a + s->fs;
+        if (to_device) {
+            GD_HIP(hipMemcpyAsync(a, re, sizeof(double) * s->fs, hipMemcpyHostToDevice, st));
+            GD_HIP(hipMemcpyAsync(b, im, sizeof(double) * s->fs, hipMemcpyHostToDevice, st));
+        }
+    }
+    if (to_device) {
+        const int *map = s->sht.d_lmaj_of_idx.as<int>();
+        GD_TRY(gd_launch_perm(st, a, map, s->nlm, s->d.nr, buf + 8 * s->fs));
+        GD_TRY(gd_launch_perm(st, b, map, s->nlm, s->d.nr, buf + 9 * s->fs));
+        if (flags & GD_HOST) GD_HIP(hipStreamSynchronize(st));
+    } else {
+        const int *map = s->sht.d_idx_of_lmaj.as<int>();
+        GD_TRY(gd_launch_perm(st, buf + 8 * s->fs, map, s->nlm, s->d.nr, a));
+        GD_TRY(gd_launch_perm(st, buf + 9 * s->fs, map, s->nlm, s->d.nr, b));
+        if (flags & GD_HOST) {
+            GD_TRY(gd_from_device(s->ctx, re, a, sizeof(double) * s->fs));
+            GD_TRY(gd_from_device(s->ctx, im, b, sizeof(double) * s->fs));
+        }
+    }
+    return GD_OK;
+}
+
+extern "C" int gd_step_get_composition_history(gd_step *s, double *nl_re, double *nl_im, int flags)
+{
+    GD_CHECK(s && nl_re && nl_im, GD_ERR_ARG, "gd_step_get_composition_history: null argument");
+    GD_CHECK(s->comp, GD_ERR_ARG, "gd_step_get_composition_history: composition not enabled");
+    GD_CHECK(s->step > 0, GD_ERR_ARG, "gd_step_get_composition_history: no history before the first step");
+    GD_HIP(hipSetDevice(s->ctx->device));
+    return comp_planes(s, s->cnl[(s->step & 1) ^ 1].as<double>(), nl_re, nl_im, flags, false);
+}
+
+// after gd_step_restore: the composition at the restored step and, for a step
+// > 0, its AB2 history
+extern "C" int gd_step_restore_composition(gd_step *s, const double *re, const double *im, const double *nl_re,
+                                           const double *nl_im, int flags)
+{
+    GD_CHECK(s && re && im, GD_ERR_ARG, "gd_step_restore_composition: null argument");
+    GD_CHECK(s->comp, GD_ERR_ARG, "gd_step_restore_composition: composition not enabled");
+    GD_CHECK(s->step == 0 || (nl_re && nl_im), GD_ERR_ARG,
+             "gd_step_restore_composition: step %d > 0 needs the composition's history", s->step);
+    GD_HIP(hipSetDevice(s->ctx->device));
+    GD_TRY(comp_planes(s, s->cstate[s->step & 1].as<double>(), const_cast<double *>(re), const_cast<double *>(im),
+                       flags, true));
+    if (s->step > 0)
+        GD_TRY(comp_planes(s, s->cnl[(s->step & 1) ^ 1].as<double>(), const_cast<double *>(nl_re),
+                           const_cast<double *>(nl_im), flags, true));
+    s->csyn_ok = false;
     return GD_OK;
 }
 

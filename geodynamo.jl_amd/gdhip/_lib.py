@@ -35,6 +35,7 @@ EXPORTS = [
     "gd_batch_create", "gd_batch_run", "gd_batch_destroy",
     "gd_step_get_history", "gd_step_restore", "gd_step_shape", "gd_step_radii", "gd_step_set_temperature_bc",
     "gd_step_enable_composition", "gd_step_set_composition", "gd_step_get_composition", "gd_step_set_composition_bc",
+    "gd_step_has_composition", "gd_step_get_composition_history", "gd_step_restore_composition",
     "gd_restart_write", "gd_restart_read", "gd_restart_write_arrays", "gd_restart_read_arrays",
     "gd_restart_filename",
     "gd_scalar_bc_create", "gd_scalar_bc_set", "gd_scalar_bc_apply", "gd_scalar_bc_tau", "gd_scalar_bc_destroy",
@@ -56,6 +57,11 @@ class RestartMeta(ctypes.Structure):
                 ("output_count", ctypes.c_int), ("restart_count", ctypes.c_int),
                 ("grid_file_written", ctypes.c_int), ("mpi_rank", ctypes.c_int),
                 ("mpi_total_ranks", ctypes.c_int), ("geometry", ctypes.c_char * 32)]
+
+
+class RestartComp(ctypes.Structure):
+    _fields_ = [("re", ctypes.c_void_p), ("im", ctypes.c_void_p), ("nl_re", ctypes.c_void_p),
+                ("nl_im", ctypes.c_void_p), ("present", ctypes.c_int), ("has_history", ctypes.c_int)]
 
 
 class StepDesc(ctypes.Structure):
@@ -148,12 +154,15 @@ def lib():
             "gd_step_set_composition": (i, [vp, vp, vp, i]),
             "gd_step_get_composition": (i, [vp, vp, vp, i]),
             "gd_step_set_composition_bc": (i, [vp, vp, vp, vp]),
+            "gd_step_has_composition": (i, [vp]),
+            "gd_step_get_composition_history": (i, [vp, vp, vp, i]),
+            "gd_step_restore_composition": (i, [vp] * 5 + [i]),
             "gd_restart_write": (i, [vp, ctypes.c_char_p, ctypes.POINTER(RestartMeta), i]),
             "gd_restart_read": (i, [vp, ctypes.c_char_p, ctypes.POINTER(RestartMeta)]),
             "gd_restart_write_arrays": (i, [ctypes.c_char_p, ctypes.POINTER(RestartMeta), i, i, i, vp, i]
-                                        + [vp] * 4 + [i]),
+                                        + [vp] * 4 + [i, ctypes.POINTER(RestartComp)]),
             "gd_restart_read_arrays": (i, [ctypes.c_char_p, i, i, i, vp, vp] + [vp] * 4
-                                       + [vp, ctypes.POINTER(RestartMeta)]),
+                                       + [vp, ctypes.POINTER(RestartMeta), ctypes.POINTER(RestartComp)]),
             "gd_restart_filename": (i, [ctypes.c_char_p] * 4 + [d, i, i, ctypes.c_char_p, i]),
             "gd_scalar_bc_create": (i, [vp, i, i, i, i, vp, i, pp]),
             "gd_scalar_bc_set": (i, [vp, vp, vp, vp]),

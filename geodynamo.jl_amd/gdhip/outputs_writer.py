@@ -16,7 +16,7 @@ import re
 
 import numpy as np
 
-from ._lib import GD_PREC_FP32, GD_PREC_FP64, RestartMeta, check, lib
+from ._lib import GD_PREC_FP32, GD_PREC_FP64, RestartComp, RestartMeta, check, lib
 
 RANK_TIME, TIME_RANK = 0, 1  # FileNaming (outputs_writer.jl:25-28)
 
@@ -134,8 +134,10 @@ def read_restart(step, tracker, restart_dir, restart_time, config, rank=0):
 
 
 def write_restart_arrays(path, lmax, nr, r, state, history=None, step=0, time=0.0, tracker=None,
-                         precision="fp64", mmax=None, geometry="shell"):
-    """Host-only writer on l-major complex arrays (5, nr, nlm); no GPU used."""
+                         precision="fp64", mmax=None, geometry="shell", composition=None,
+                         composition_history=None):
+    """Host-only writer on l-major complex arrays (5, nr, nlm), the composition
+    (nr, nlm) optional; no GPU used."""
     mmax = lmax if mmax is None else mmax
     tracker = tracker or TimeTracker()
     m = _meta(time, tracker, geometry, 0, 1)
@@ -147,12 +149,24 @@ def write_restart_arrays(path, lmax, nr, r, state, history=None, step=0, time=0.
         hp = (hr.ctypes.data, hi.ctypes.data)
     else:
         hp = (None, None)
+    comp, keep = None, []
+    if composition is not None:
+        comp = RestartComp()
+        parts = [composition.real, composition.imag]
+        if composition_history is not None:
+            parts += [composition_history.real, composition_history.imag]
+        keep = [np.ascontiguousarray(x, dtype=np.float64) for x in parts]
+        comp.re, comp.im = keep[0].ctypes.data, keep[1].ctypes.data
+        if len(keep) == 4:
+            comp.nl_re, comp.nl_im = keep[2].ctypes.data, keep[3].ctypes.data
     check(lib().gd_restart_write_arrays(path.encode(), ctypes.byref(m), lmax, mmax, nr, rr.ctypes.data, int(step),
-                                        re_.ctypes.data, im_.ctypes.data, hp[0], hp[1], _precision(precision)))
+                                        re_.ctypes.data, im_.ctypes.data, hp[0], hp[1], _precision(precision),
+                                        ctypes.byref(comp) if comp is not None else None))
 
 
 def read_restart_arrays(path, lmax, nr, mmax=None):
-    """Host-only reader: (state, history or None, step, r, metadata dict)."""
+    """Host-only reader: (state, history or None, step, r, metadata dict); the
+    dict holds "composition" and "composition_history" (None when absent)."""
     mmax = lmax if mmax is None else mmax
     nlm = sum(min(l, mmax) + 1 for l in range(lmax + 1))
     shape = (5, nr, nlm)
@@ -160,8 +174,14 @@ def read_restart_arrays(path, lmax, nr, mmax=None):
     r = np.zeros(nr)
     step, hist = ctypes.c_int(), ctypes.c_int()
     m = RestartMeta()
+    cplanes = [np.zeros((nr, nlm)) for _ in range(4)]
+    comp = RestartComp()
+    comp.re, comp.im, comp.nl_re, comp.nl_im = (x.ctypes.data for x in cplanes)
     check(lib().gd_restart_read_arrays(path.encode(), lmax, mmax, nr, r.ctypes.data, ctypes.byref(step),
                                        re_.ctypes.data, im_.ctypes.data, hr.ctypes.data, hi.ctypes.data,
-                                       ctypes.byref(hist), ctypes.byref(m)))
+                                       ctypes.byref(hist), ctypes.byref(m), ctypes.byref(comp)))
     history = hr + 1j * hi if hist.value else None
-    return re_ + 1j * im_, history, step.value, r, _meta_dict(m)
+    md = _meta_dict(m)
+    md["composition"] = cplanes[0] + 1j * cplanes[1] if comp.present else None
+    md["composition_history"] = cplanes[2] + 1j * cplanes[3] if comp.has_history else None
+    return re_ + 1j * im_, history, step.value, r, md

@@ -103,6 +103,23 @@ class CNAB2Step:
         check(lib().gd_step_get_composition(self._h, a, b, flags))
         return re + 1j * im
 
+    def get_composition_history(self):
+        re, im = np.zeros((self.nr, self.nlm)), np.zeros((self.nr, self.nlm))
+        ctx, flags, (a, b) = prepare(re, im, device=self.device)
+        check(lib().gd_step_get_composition_history(self._h, a, b, flags))
+        return re + 1j * im
+
+    def restore_composition(self, C, history=None):
+        """After restore(): the composition at the restored step and its history."""
+        re, im = np.ascontiguousarray(C.real), np.ascontiguousarray(C.imag)
+        if history is not None:
+            hre, him = np.ascontiguousarray(history.real), np.ascontiguousarray(history.imag)
+            ctx, flags, (a, b, c, d) = prepare(re, im, hre, him, device=self.device)
+        else:
+            ctx, flags, (a, b) = prepare(re, im, device=self.device)
+            c = d = None
+        check(lib().gd_step_restore_composition(self._h, a, b, c, d, flags))
+
     def set_composition_bc(self, bc_type_inner, bc_type_outer, boundary_values):
         ti = np.ascontiguousarray(bc_type_inner, dtype=np.int32)
         to = np.ascontiguousarray(bc_type_outer, dtype=np.int32)

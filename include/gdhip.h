@@ -255,12 +255,18 @@ int gd_step_set_temperature_bc(gd_step *s, const int *bc_inner, const int *bc_ou
  * temperature), CNAB2 with diffusivity Pm/Sc (simulation.jl:130-131), its
  * conditions applied before its solve (default: no flux at both boundaries,
  * value 0, compositional.jl:110-115; set with gd_step_set_composition_bc).
- * Planes (nlm, nr) re/im, l-major; set after gd_step_set_state.  Restores and
- * restart files do not carry it (gd_step_restore refuses). */
+ * Planes (nlm, nr) re/im, l-major; set after gd_step_set_state.  Restart files
+ * carry it as composition_spectral (with its history). */
 int gd_step_enable_composition(gd_step *s, double Sc);
 int gd_step_set_composition(gd_step *s, const double *re, const double *im, int flags);
 int gd_step_get_composition(gd_step *s, double *re, double *im, int flags);
 int gd_step_set_composition_bc(gd_step *s, const int *bc_inner, const int *bc_outer, const double *values);
+int gd_step_has_composition(const gd_step *s);
+/* its AB2 history (GD_ERR_ARG before the first step) and its restore after a
+ * gd_step_restore (same planes; NULL history when that restore was at step 0) */
+int gd_step_get_composition_history(gd_step *s, double *nl_re, double *nl_im, int flags);
+int gd_step_restore_composition(gd_step *s, const double *re, const double *im, const double *nl_re,
+                                const double *nl_im, int flags);
 /* radial grid r_i of the step's shell (fields.jl:108-131), host out[nr] */
 int gd_step_radii(const gd_step *s, double *r);
 
@@ -287,14 +293,25 @@ int gd_restart_write(gd_step *s, const char *path, const gd_restart_meta *meta, 
  * restores state and history (a file without history restores at step 0);
  * meta (may be NULL) receives time, tracker scalars and geometry. */
 int gd_restart_read(gd_step *s, const char *path, gd_restart_meta *meta);
+/* The composition's planes of a restart file, (nlm, nr) each, l-major
+ * (composition_spectral_{real,imag} and its _prev_nonlinear history).  Write:
+ * re/im NULL = none; the history is written with the fields' history.  Read:
+ * present / has_history report the file, the planes are filled when given. */
+typedef struct {
+    double *re, *im, *nl_re, *nl_im;
+    int present, has_history;
+} gd_restart_comp;
+
 /* host-only forms of the two on l-major planes 5 x (nlm, nr) (no GPU used);
- * nl_re/nl_im may be NULL (no history written / not returned) */
+ * nl_re/nl_im may be NULL (no history written / not returned); comp may be
+ * NULL (no composition) */
 int gd_restart_write_arrays(const char *path, const gd_restart_meta *meta, int lmax, int mmax, int nr,
                             const double *r, int step, const double *re, const double *im,
-                            const double *nl_re, const double *nl_im, int precision);
+                            const double *nl_re, const double *nl_im, int precision,
+                            const gd_restart_comp *comp);
 int gd_restart_read_arrays(const char *path, int lmax, int mmax, int nr, double *r, int *step,
                            double *re, double *im, double *nl_re, double *nl_im, int *has_history,
-                           gd_restart_meta *meta);
+                           gd_restart_meta *meta, gd_restart_comp *comp);
 /* replaces generate_filename (outputs_writer.jl:338-361):
  * <dir>/<prefix>_<type>_geom_<geometry>_rank_XXXX_time_<t>.nc with t = %.6f, "." -> "p", or
  * time before rank when time_first = 1 (TIME_RANK naming) */

@@ -122,4 +122,40 @@ def test_fp32_close_to_fp64_and_errors():
     s = _step("fp64")
     s.set_state_complex(st)
     with pytest.raises(gdhip.GDError):
-        s.restore(st)
+        s.get_composition_history()  # no history before the first step
+
+
+def test_restart_carries_composition(tmp_path):
+    """write_restart! / read_restart! with composition_spectral and its AB2
+    history: 4 steps, write, read into a fresh step, 3 more steps equal 7
+    uninterrupted steps bit for bit (fields and composition)."""
+    import gdhip
+
+    print("case: restart with composition", flush=True)
+    st, C0 = _inputs()
+    a = _step("fp64")
+    a.set_state_complex(st)
+    a.set_composition(C0)
+    a.run(7, graph=True)
+    b = _step("fp64")
+    b.set_state_complex(st)
+    b.set_composition(C0)
+    b.run(4, graph=True)
+    cfg = gdhip.OutputConfig(output_dir=str(tmp_path), filename_prefix="gdc")
+    gdhip.write_restart(b, gdhip.TimeTracker(), {"current_time": 4e-4}, cfg)
+    c = _step("fp64")
+    md = gdhip.read_restart(c, gdhip.TimeTracker(), str(tmp_path), 4e-4, cfg)
+    assert md["current_step"] == 4
+    assert np.array_equal(c.get_composition(), b.get_composition())
+    assert np.array_equal(c.get_composition_history(), b.get_composition_history())
+    c.run(3, graph=True)
+    assert np.array_equal(c.get_state_complex(), a.get_state_complex())
+    assert np.array_equal(c.get_composition(), a.get_composition())
+    # a file without composition cannot restore a step that has one
+    plain = _step("fp64", comp=False)
+    plain.set_state_complex(st)
+    plain.run(2)
+    cfg2 = gdhip.OutputConfig(output_dir=str(tmp_path / "plain"), filename_prefix="gdp")
+    gdhip.write_restart(plain, gdhip.TimeTracker(), {"current_time": 2e-4}, cfg2)
+    with pytest.raises(gdhip.GDError):
+        gdhip.read_restart(_step("fp64"), gdhip.TimeTracker(), str(tmp_path / "plain"), 2e-4, cfg2)

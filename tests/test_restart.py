@@ -195,3 +195,29 @@ def test_generate_filename_and_find_restart_files(tmp_path):
     assert got == ["run_restart_geom_shell_rank_0003_time_1p000000.nc",
                    "run_restart_geom_shell_rank_0003_time_0p500000.nc",
                    "run_restart_geom_shell_rank_0003_time_2p000000.nc"]
+
+
+def test_composition_round_trip_and_scipy(tmp_path):
+    """composition_spectral (outputs_writer.jl:507-522, read back at :1327-1339)
+    and its AB2 history."""
+    L, nr = 9, 7
+    st, h = _fields(10, L, nr), _fields(11, L, nr)
+    c, ch = _fields(12, L, nr)[0], _fields(13, L, nr)[0]
+    r = np.linspace(0.5, 1.5, nr)
+    path = str(tmp_path / "c.nc")
+    gdhip.write_restart_arrays(path, L, nr, r, st, h, step=3, composition=c, composition_history=ch)
+    f = netcdf_file(path, "r", mmap=False)
+    try:
+        assert f.variables["composition_spectral_real"].dimensions == ("r", "spectral_mode")
+        assert np.array_equal(f.variables["composition_spectral_imag"].data, c.imag)
+        assert np.array_equal(f.variables["composition_spectral_prev_nonlinear_real"].data, ch.real)
+    finally:
+        f.close()
+    s2, h2, step, _, md = gdhip.read_restart_arrays(path, L, nr)
+    assert step == 3 and np.array_equal(s2, st) and np.array_equal(h2, h)
+    assert np.array_equal(md["composition"], c) and np.array_equal(md["composition_history"], ch)
+    # without composition: reported absent
+    p0 = str(tmp_path / "n.nc")
+    gdhip.write_restart_arrays(p0, L, nr, r, st, h, step=3)
+    _, _, _, _, md0 = gdhip.read_restart_arrays(p0, L, nr)
+    assert md0["composition"] is None and md0["composition_history"] is None
