@@ -40,10 +40,11 @@ template <int BW, int D, typename TS, bool DIRECT> __global__ void __launch_boun
     extern __shared__ double smem[];
     double *Ex = smem, *sD1 = Ex + ER * n, *sD2 = sD1 + W * n, *rinv_ = sD2 + W * n, *rinv2_ = rinv_ + n;
     const double *Lapk = c.nuLap + (size_t)k * W * n;
-    for (int t = threadIdx.x; t < W * n; t += 64) {
-        sD1[t] = c.D1[t];
-        sD2[t] = c.D2[t];
-    }
+    if constexpr (!DIRECT) // the radial derivatives serve only the general synthesis columns
+        for (int t = threadIdx.x; t < W * n; t += 64) {
+            sD1[t] = c.D1[t];
+            sD2[t] = c.D2[t];
+        }
     for (int t = threadIdx.x; t < ER * n; t += 64) {
         const int i = t / ER, q = t - i * ER, j = i - BW + q;
         double v = 0.0;
