@@ -114,12 +114,36 @@ template <> struct Vec16<double> { using type = double2; };
 // (vector: S_re S_im T_re T_im per field, scalar: re im), NW waves of 16
 // columns each, NTILE = 16 NW columns = LT levels; RS output slots per
 // (node, level) record.
-template <bool VEC, int NF, int NW> struct SynthTile {
-    static constexpr int VC = VEC ? 4 * NF : 2 * NF;
+// NG gradient fields (vector only: S alone, T = 0) follow the NF full fields
+// with 2 columns each: both components come from the S columns, so the zero
+// T columns are not contracted.  A full field's 4 columns must start at a
+// multiple of 4 (its epilogue exchanges values within a lane quad); when a
+// level's column count is not a multiple of 4, two levels form one group:
+// their full fields first, then their gradient fields.
+template <bool VEC, int NF, int NW, int NG = 0> struct SynthTile {
+    static constexpr int VC = VEC ? 4 * NF + 2 * NG : 2 * NF; // columns per level
+    static constexpr int PAIR = (VEC && VC % 4 != 0) ? 2 : 1;  // levels per column group
+    static constexpr int GC = VC * PAIR;                        // columns per group
     static constexpr int NTILE = 16 * NW;
-    static constexpr int LT = NTILE / VC;
-    static constexpr int RS = VEC ? 8 * NF : 4 * NF;
+    static constexpr int LT = NTILE / GC * PAIR;
+    static constexpr int RS = VEC ? 8 * (NF + NG) : 4 * NF;
     static constexpr int NT = VEC ? 2 : 1; // tables: (dP, mP/sin) or P
+    // virtual column -> (level within the tile, column within the level)
+    __device__ static void decode(int vcol, int &lv, int &fv)
+    {
+        const int g = vcol / GC, w = vcol - g * GC;
+        if constexpr (PAIR == 1) {
+            lv = g;
+            fv = w;
+        } else if (w < 8 * NF) {
+            lv = 2 * g + w / (4 * NF);
+            fv = w % (4 * NF);
+        } else {
+            const int w2 = w - 8 * NF;
+            lv = 2 * g + w2 / (2 * NG);
+            fv = 4 * NF + w2 % (2 * NG);
+        }
+    }
 };
 
 // Synthesis contraction for one (m, node tile, level tile), both parity
@@ -135,14 +159,14 @@ template <bool VEC, int NF, int NW> struct SynthTile {
 //     (lane / 16) takes degrees 4g .. 4g+3 over the chunk's 4 MFMA steps (the
 //     same permutation indexes A), so each lane reads 4 consecutive degrees.
 constexpr int TMA = TM + 4;
-template <typename T, bool VEC, int NF, int NW, int KC, bool VL>
+template <typename T, bool VEC, int NF, int NW, int KC, bool VL, int NG>
 __device__ __forceinline__ void synth_pipeline(const SynthArgs &a, const T *__restrict__ in, const T *__restrict__ tab0,
                                                const T *__restrict__ tab1, T *sA, const int *scol, int m,
-                                               int i0, int r0, acc_t<T> (&accE)[SynthTile<VEC, NF, NW>::NT][2],
-                                               acc_t<T> (&accO)[SynthTile<VEC, NF, NW>::NT][2])
+                                               int i0, int r0, acc_t<T> (&accE)[SynthTile<VEC, NF, NW, NG>::NT][2],
+                                               acc_t<T> (&accO)[SynthTile<VEC, NF, NW, NG>::NT][2])
 {
-    using TL = SynthTile<VEC, NF, NW>;
-    static_assert(TL::NTILE % TL::VC == 0, "a block tile holds whole levels");
+    using TL = SynthTile<VEC, NF, NW, NG>;
+    static_assert(TL::NTILE % TL::GC == 0, "a block tile holds whole column groups");
     static_assert(KC % 16 == 0, "chunks are whole 16-degree MFMA groups");
     constexpr int NT = TL::NT, VC = TL::VC;
     constexpr int nthr = 64 * NW;
@@ -174,7 +198,8 @@ __device__ __forceinline__ void synth_pipeline(const SynthArgs &a, const T *__re
     // this lane's B column
     const int c16 = lane & 15, g = lane >> 4;
     const int vcol = wave * 16 + c16;
-    const int lv = vcol / VC, fv = vcol - lv * VC;
+    int lv, fv;
+    TL::decode(vcol, lv, fv);
     const int col = scol[fv], r = r0 + lv;
     const bool vb = col >= 0 && r < a.nr;
     const T *pb = in + (size_t)(vb ? r * a.cin + col : 0) * a.nlm + kb0;
@@ -240,9 +265,9 @@ __device__ __forceinline__ void synth_pipeline(const SynthArgs &a, const T *__re
     }
 }
 
-template <typename T, bool VEC, int NF, int NW, int KC> constexpr size_t synth_lds_bytes()
+template <typename T, bool VEC, int NF, int NW, int KC, int NG = 0> constexpr size_t synth_lds_bytes()
 {
-    using TL = SynthTile<VEC, NF, NW>;
+    using TL = SynthTile<VEC, NF, NW, NG>;
     const size_t pipe = 2 * (size_t)TL::NT * KC * TMA, tile = (size_t)TM * TL::LT * TL::RS;
     return 128 + sizeof(T) * (pipe > tile ? pipe : tile);
 }
@@ -250,10 +275,10 @@ template <typename T, bool VEC, int NF, int NW, int KC> constexpr size_t synth_l
 // The block's output tile -- TM nodes x LT levels, RS contiguous slots of each
 // (node, level, m) record -- is staged in LDS and stored as 16-byte vectors,
 // one record run after another.
-template <typename T, bool VEC, int NF, int NW, int KC, bool VL>
+template <typename T, bool VEC, int NF, int NW, int KC, bool VL, int NG = 0>
 __global__ void __launch_bounds__(64 * NW) synth_kernel(SynthArgs a)
 {
-    using TL = SynthTile<VEC, NF, NW>;
+    using TL = SynthTile<VEC, NF, NW, NG>;
     constexpr int NT = TL::NT, VC = TL::VC, LT = TL::LT, RS = TL::RS;
     extern __shared__ __align__(16) char smem_raw[];
     int *scol = reinterpret_cast<int *>(smem_raw);
@@ -276,12 +301,13 @@ __global__ void __launch_bounds__(64 * NW) synth_kernel(SynthArgs a)
             accE[t][rb] = acc_t<T>{0, 0, 0, 0};
             accO[t][rb] = acc_t<T>{0, 0, 0, 0};
         }
-    synth_pipeline<T, VEC, NF, NW, KC, VL>(a, in, tab0, tab1, sA, scol, m, i0, r0, accE, accO);
+    synth_pipeline<T, VEC, NF, NW, KC, VL, NG>(a, in, tab0, tab1, sA, scol, m, i0, r0, accE, accO);
     __syncthreads(); // the tile below reuses the pipeline's LDS
 
     T *tile = sA; // [node][level][slot]
     const int vcol = wave * 16 + (lane & 15);
-    const int lv = vcol / VC, fv = vcol - lv * VC;
+    int lv, fv;
+    TL::decode(vcol, lv, fv);
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb) {
 #pragma unroll
@@ -295,6 +321,17 @@ __global__ void __launch_bounds__(64 * NW) synth_kernel(SynthArgs a)
                 const T Qn = accE[1][rb][v] + accO[1][rb][v], Qs = accE[1][rb][v] - accO[1][rb][v];
                 const T Qxn = shfl_xor3(Qn), Qxs = shfl_xor3(Qs);
                 const int q = lane & 3; // 0 S_re, 1 S_im, 2 T_re, 3 T_im
+                if (NG > 0 && fv >= 4 * NF) {
+                    // gradient field: theta = dP S, phi = i Q S (Q S_re -> phi_im,
+                    // -Q S_im -> phi_re); the values of the full form with T = 0
+                    const int gq = fv - 4 * NF, q2 = gq & 1;
+                    T *tg = tile + (il * LT + lv) * RS + 8 * (NF + (gq >> 1));
+                    tg[q2] = dPn;
+                    tg[4 + q2] = dPs;
+                    tg[q2 ? 2 : 3] = q2 ? -Qn : Qn;
+                    tg[q2 ? 6 : 7] = q2 ? -Qs : Qs;
+                    continue;
+                }
                 if (q == 0) {
                     on = dPn - Qxn;
                     os = dPs - Qxs;
@@ -715,16 +752,17 @@ template <typename T, bool VEC, int NF, int KN> constexpr size_t ana_res_lds()
 
 } // namespace
 
-template <typename T, bool VEC, int NF, int NW, int KC> static void launch_synth(hipStream_t st, const SynthArgs &a)
+template <typename T, bool VEC, int NF, int NW, int KC, int NG = 0>
+static void launch_synth(hipStream_t st, const SynthArgs &a)
 {
-    using TL = SynthTile<VEC, NF, NW>;
+    using TL = SynthTile<VEC, NF, NW, NG>;
     dim3 grid(((a.nlat2 + TM - 1) / TM) * ((a.nr + TL::LT - 1) / TL::LT) * (a.mmax + 1));
-    constexpr size_t lds = synth_lds_bytes<T, VEC, NF, NW, KC>();
+    constexpr size_t lds = synth_lds_bytes<T, VEC, NF, NW, KC, NG>();
     // 16-byte table loads need every node row and tile start 16-byte aligned
     if (a.nlat2 % 4 == 0 && ((uintptr_t)a.tab0 % 16) == 0 && ((uintptr_t)a.tab1 % 16) == 0)
-        hipLaunchKernelGGL((synth_kernel<T, VEC, NF, NW, KC, true>), grid, dim3(64 * NW), lds, st, a);
+        hipLaunchKernelGGL((synth_kernel<T, VEC, NF, NW, KC, true, NG>), grid, dim3(64 * NW), lds, st, a);
     else
-        hipLaunchKernelGGL((synth_kernel<T, VEC, NF, NW, KC, false>), grid, dim3(64 * NW), lds, st, a);
+        hipLaunchKernelGGL((synth_kernel<T, VEC, NF, NW, KC, false, NG>), grid, dim3(64 * NW), lds, st, a);
 }
 
 // The staged tile is stored in 16-byte vectors: record stride and first slot
@@ -739,8 +777,14 @@ static int synth_check(const SynthArgs &a, const char *what)
 template <typename T> int gd_launch_synth_vec(hipStream_t st, const SynthArgs &a)
 {
     GD_TRY(synth_check(a, "synth_vec"));
-    GD_CHECK(a.slot0 + 8 * a.nf <= a.nslot, GD_ERR_ARG, "synth_vec: slots %d+%d exceed %d", a.slot0, 8 * a.nf, a.nslot);
-    if (a.nf == 5) launch_synth<T, true, 5, 5, 16>(st, a);
+    GD_CHECK(a.slot0 + 8 * (a.nf + a.ng) <= a.nslot, GD_ERR_ARG, "synth_vec: slots %d+%d exceed %d", a.slot0,
+             8 * (a.nf + a.ng), a.nslot);
+    // one full field with 1 or 2 gradient fields: 6 columns per level (two
+    // levels per 12-column group, 3 waves = 8 levels) or 8 (4 waves = 8 levels)
+    if (a.ng == 1 && a.nf == 1) launch_synth<T, true, 1, 3, 16, 1>(st, a);
+    else if (a.ng == 2 && a.nf == 1) launch_synth<T, true, 1, 4, 16, 2>(st, a);
+    else if (a.ng != 0) GD_CHECK(false, GD_ERR_ARG, "synth_vec: nf=%d ng=%d", a.nf, a.ng);
+    else if (a.nf == 5) launch_synth<T, true, 5, 5, 16>(st, a);
     else if (a.nf == 3) launch_synth<T, true, 3, 3, 16>(st, a); // 12 columns per level, 3 waves = 4 levels
     else if (a.nf == 2) launch_synth<T, true, 2, 4, 16>(st, a);
     else if (a.nf == 1) launch_synth<T, true, 1, 1, 16>(st, a);

@@ -27,6 +27,8 @@ struct SynthArgs {
     int nlat2, nr, nlm, mmax;
     void *out;            // T*, FB
     int nslot, slot0;     // vector field f -> slot0 + 8f + 4h + (2c + reim); scalar -> slot0 + 4f + 2h + reim
+    int ng;               // vector: gradient fields (S only, T = 0) after the nf full ones; colmap
+                          // then lists 4 nf + 2 ng columns (S_re S_im per gradient field)
 };
 
 struct AnaArgs {

@@ -60,6 +60,9 @@ struct gd_step {
     // per-mode temperature conditions (gd_step_set_temperature_bc); null: the
     // boundary rows of T are read as 0 (thermal.jl defaults: Dirichlet, value 0)
     gd_scalar_bc *tbc = nullptr;
+    // synthesis of the tangential step with grad T (and grad C) as gradient
+    // fields (1) or as full fields with zero toroidal columns (0, same values)
+    int synth_grad = 1;
     // optional passive composition (gd_step_enable_composition, tangential
     // step): state and nonlinear history in the planes 8, 9 (field 4) of
     // five-field buffers whose other planes stay zero, so the CNAB2 kernel
@@ -251,9 +254,18 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEven
         // slots, 24 with the composition's gradient (S = C/r) in slots 16..23
         const int nslot = s->comp ? 24 : 16;
         a.nslot = nslot;
-        a.nf = s->comp ? 3 : 2;
-        static const int vmap[12] = {0, 1, 2, 3, 16, 17, -1, -1, SYNV_COLS - 2, SYNV_COLS - 1, -1, -1};
-        for (int k = 0; k < 20; ++k) a.colmap[k] = k < 4 * a.nf ? vmap[k] : -1;
+        if (s->synth_grad) {
+            // u as a full field, grad T (and grad C) as gradient fields: their
+            // zero toroidal columns are not contracted
+            a.nf = 1;
+            a.ng = s->comp ? 2 : 1;
+            static const int gmap[8] = {0, 1, 2, 3, 16, 17, SYNV_COLS - 2, SYNV_COLS - 1};
+            for (int k = 0; k < 20; ++k) a.colmap[k] = k < 4 + 2 * a.ng ? gmap[k] : -1;
+        } else {
+            a.nf = s->comp ? 3 : 2;
+            static const int vmap[12] = {0, 1, 2, 3, 16, 17, -1, -1, SYNV_COLS - 2, SYNV_COLS - 1, -1, -1};
+            for (int k = 0; k < 20; ++k) a.colmap[k] = k < 4 * a.nf ? vmap[k] : -1;
+        }
         if (s->comp && !s->csyn_ok) GD_TRY(spectral_pre_comp<T>(s));
         GD_TRY(gd_launch_synth_vec<T>(st, a));
         GD_MARK(2);
@@ -633,6 +645,10 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
     s->overlap = d->prec == GD_PREC_FP32 ? 2 : 0;
     const char *ov = getenv("GD_STEP_OVERLAP");
     if (ov && ov[0] >= '0' && ov[0] <= '2') s->overlap = ov[0] - '0';
+    // GD_SYNTH_GRAD = 0 contracts the zero toroidal columns of the gradient
+    // fields as well (the earlier form; the results are the same bits)
+    const char *sg = getenv("GD_SYNTH_GRAD");
+    s->synth_grad = !(sg && sg[0] == '0');
     return GD_OK;
 }
 
