@@ -92,9 +92,14 @@ def kernel_costs(prec_bytes, tangential=True):
 
 
 # kernel of each class on the headline (tangential) path, as rocprofv3 names it
-TANG_KERNELS = {"spectral_pre": "spectral_pre_kernel<{T}>", "synth_vector": "synth_kernel<{T}, true, 2, 4, 16, true>",
+# (synthesis: u and grad T as gradient fields, or, with GD_SYNTH_GRAD=0, as
+# full fields; CNAB2: one lane per column on the FP64 path, the twisted form
+# on the FP32-synthesis path)
+TANG_KERNELS = {"spectral_pre": "spectral_pre_kernel<{T}>",
+                "synth_vector": ("synth_kernel<{T}, true, 1, 3, 16, true, 1>", "synth_kernel<{T}, true, 2, 4, 16, true>"),
                 "ring_fft_products": "ring_kernel<{T}, 3, 384>", "analysis_vector": "ana_res_kernel<{T}, true, 1, 96>",
-                "analysis_scalar": "ana_res_kernel<{T}, false, 1, 96>", "cnab2_solve": "cnab2_chain_kernel<3, 4, {T}, true>"}
+                "analysis_scalar": "ana_res_kernel<{T}, false, 1, 96>",
+                "cnab2_solve": ("cnab2_chain_kernel<3, 4, {T}, true>", "cnab2_twist_kernel<3, 4, {T}>")}
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "pmc_step_lmax127_nr65.json")
 COPY_PROBE_BYTES = 268435456
 
@@ -121,7 +126,8 @@ def derive_traffic(prof_dir):
     out = {"fetch_scale": round(fscale, 4), "write_scale": round(wscale, 4), "fp32": {}, "fp64": {}}
     for prec, T in (("fp32", "float"), ("fp64", "double")):
         for cls, pat in TANG_KERNELS.items():
-            ks = [k for k in val["fetch"] if pat.format(T=T) in k]
+            pats = pat if isinstance(pat, tuple) else (pat,)
+            ks = [k for k in val["fetch"] if any(p.format(T=T) in k for p in pats)]
             if not ks:
                 continue
             rd, wr = val["fetch"][ks[0]] * fscale, val["write"].get(ks[0], 0.0) * wscale

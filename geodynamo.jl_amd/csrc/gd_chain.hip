@@ -1,6 +1,8 @@
 // gd_chain.hip -- per-column CNAB2 kernel of the step (FP64 rhs and banded
 // solve, timestep.jl:610-730, fused with the next step's spectral
 // preparation).
+#include <algorithm>
+#include <cmath>
 #include <type_traits>
 
 #include "gd_radial.hpp"
@@ -232,6 +234,210 @@ template <int BW, int D, typename TS, bool DIRECT> __global__ void __launch_boun
         }
     }
 }
+
+// Twisted form of the tangential (DIRECT) column kernel.  A block of two
+// waves takes 64 columns.  Wave 0 takes rows [0, n1): top-down rhs and L1
+// substitution, then upward back substitution with U1; wave 1 takes rows
+// [n1, n) of the same columns with the bottom block's UL factors: bottom-up
+// rhs and U2 substitution, then downward substitution with L2.  The half is
+// wave-uniform, so every row index and table offset stays on the scalar unit
+// as in the one-lane form.  The waves exchange their last BW intermediate
+// values through LDS (one barrier), solve the interface rows with R_l, and
+// finish outward: the serial chain per column is about half of the one-lane
+// form.  The rhs, the Dirichlet rows and the synthesis columns are those of
+// cnab2_chain_kernel.  Lanes past nlm repeat column nlm-1: they compute the
+// same values as its own lane and store them to the same places, which keeps
+// both waves on the barrier and the loops free of per-lane store guards.
+template <int BW, int D, typename TS> __global__ void __launch_bounds__(128) cnab2_twist_kernel(ChainArgs c)
+{
+    constexpr int W = 2 * BW + 1, M2 = 2 * BW;
+    const int n = c.n, nlm = c.nlm, n1 = (n + 1) / 2, nb = n - n1;
+    const int lane = threadIdx.x & 63;
+    const bool bot = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) != 0;
+    const int idx = min(blockIdx.x * 64 + lane, nlm - 1);
+    const int fq = c.f0 + blockIdx.y, ri = blockIdx.z;
+    const bool isC = fq == 5;
+    const int f = isC ? 4 : fq;
+    const int k = isC ? 3 : (f < 2 ? 0 : (f < 4 ? 1 : 2));
+    constexpr int ER = (W + 2) & ~1;
+    extern __shared__ double smem[];
+    double *Ex = smem, *rinv_ = Ex + ER * n, *xch = rinv_ + n; // xch: [half][BW][64]
+    const double *Lapk = c.nuLap + (size_t)k * W * n;
+    for (int t = threadIdx.x; t < ER * n; t += 128) {
+        const int i = t / ER, q = t - i * ER, j = i - BW + q;
+        double v = 0.0;
+        if (q == BW) v = Lapk[BW + W * i];
+        else if (q < W && j >= 0 && j < n) v = Lapk[(2 * BW - q) + W * j];
+        else if (q == W) v = c.rtab[i + n];
+        Ex[t] = v;
+    }
+    for (int t = threadIdx.x; t < n; t += 128) rinv_[t] = c.rtab[t + 2 * n];
+    __syncthreads();
+    const int l = c.l_of_idx[idx];
+    const double lf = (double)l * (double)(l + 1);
+    const double nulf = c.nus[k] * lf;
+    const double *T = (bot ? c.ULt : c.LUt) + (size_t)k * W * n * c.nl + l;
+    const double *rdg = (bot ? c.rdiag2 : c.rdiag) + (size_t)k * n * c.nl + l;
+    const double *Rl = c.Rint + ((size_t)k * c.nl + l) * (M2 * M2);
+    const size_t fs = (size_t)nlm * n, off = (size_t)(2 * f + ri) * fs + idx;
+    const double *u = (isC ? c.c_in : c.st_in) + off;
+    const bool nlz = !isC && ((c.nl_zero_mask >> f) & 1);
+    const double *pn = nlz ? c.zero_row + idx : (isC ? c.c_nl : c.nl_arr) + off;
+    const double *pp = nlz ? c.zero_row + idx : (isC ? c.c_np : c.np_arr) + off;
+    const size_t nstride = nlz ? 0 : (size_t)nlm;
+    double *o = (isC ? c.c_out : c.st_out) + off;
+    const bool tkeep = isC || ((f == 4) && ((c.bc_mask >> 5) & 1));
+    const bool tbc = (f == 4) && !tkeep;
+    const bool bcr = isC ? c.c_bcr != 0 : ((c.bc_mask >> f) & 1);
+    const double inv_dt = 1.0 / c.dt, lw = 1.0 - c.theta;
+    const int dir = bot ? -1 : 1;
+    auto cl = [&](int j) { return j < 0 ? 0 : (j > n - 1 ? n - 1 : j); };
+    auto urow = [&](int j) { return j >= 0 && j < n && !(tbc && (j == 0 || j == n - 1)); };
+    // band entry (i, j) of this half's table; j outside 0..n-1 reads the
+    // (finite) diagonal instead.  No mask is needed: the forward rows outside
+    // 0..n-1 meet yw entries that start at 0, and the back substitution's rows
+    // stay inside its half.  Loads stay unconditional and their values are
+    // used D steps later, so the prefetch is never drained by a select.
+    auto band = [&](int ic, int j) {
+        const int jc = (j >= 0 && j < n) ? j : ic;
+        return T[((size_t)jc * W + (BW + ic - jc)) * c.nl];
+    };
+    const bool wA = f == 0 || f == 1 || f == 4;
+    TS *pA = static_cast<TS *>(c.synV) + (size_t)((isC ? SYNV_COLS - 2 : (f == 0 ? 2 : (f == 1 ? 0 : 16))) + ri) * nlm + idx;
+    auto put = [&](int i, double v) { // x row i: state and (u, T/r, C/r) synthesis column
+        o[(size_t)nlm * i] = v;
+        if (wA) pA[(size_t)i * SYNV_COLS * nlm] = (TS)(f == 4 ? v * rinv_[i] : v);
+    };
+
+    // ---- outward-in: rhs rows and the unit-triangular substitution
+    const int nfw = bot ? nb : n1; // valid steps of this lane
+    double w[W];                   // u rows (top: i-BW..i+BW; bottom: i+BW..i-BW)
+    {
+        const int i0 = bot ? n - 1 : 0;
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
+            const int j = i0 + dir * (q - BW);
+            const double v = u[(size_t)nlm * cl(j)];
+            w[q] = urow(j) ? v : 0.0;
+        }
+    }
+    double su[D], sn[D], sp[D], sl[D][BW];
+    auto fload = [&](int sl_, int s) {
+        const int i = bot ? n - 1 - s : s, ic = cl(i), jn = i + dir * (BW + 1);
+        su[sl_] = u[(size_t)nlm * cl(jn)]; // masked with urow at its use
+        sn[sl_] = pn[nstride * ic];
+        sp[sl_] = pp[nstride * ic];
+#pragma unroll
+        for (int q = 0; q < BW; ++q) {
+            sl[sl_][q] = band(ic, i + dir * (q - BW)); // the q-th of the BW rows already eliminated
+        }
+    };
+    double yw[BW];
+#pragma unroll
+    for (int q = 0; q < BW; ++q) yw[q] = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) fload(d, d);
+    // step s of this half from prefetch slot d; `valid` is false only in the
+    // tail block, so full blocks shift the windows as plain register renames
+    auto fstep = [&](int d, int s, bool valid) {
+        const int i = bot ? n - 1 - s : s, ic = cl(i);
+        const double *er = Ex + ER * ic;
+        double lr = 0.0;
+#pragma unroll
+        for (int q = 0; q < W; ++q) lr += ((q == BW) ? er[BW] - nulf * er[W] : er[bot ? 2 * BW - q : q]) * w[q];
+        double v = inv_dt * w[BW] + 1.5 * sn[d] - 0.5 * sp[d];
+        v += lw * lr;
+        v = (bcr && (i == 0 || i == n - 1)) ? (tkeep ? w[BW] : 0.0) : v; // Dirichlet value
+#pragma unroll
+        for (int q = 0; q < BW; ++q) v -= sl[d][q] * yw[q];
+        if (valid) {
+#pragma unroll
+            for (int q = 0; q < BW - 1; ++q) yw[q] = yw[q + 1];
+            yw[BW - 1] = v;
+#pragma unroll
+            for (int q = 0; q < W - 1; ++q) w[q] = w[q + 1];
+            w[W - 1] = urow(i + dir * (BW + 1)) ? su[d] : 0.0;
+            o[(size_t)nlm * i] = v;
+        }
+        fload(d, s + D);
+    };
+    int s0 = 0;
+    for (; s0 + D <= nfw; s0 += D) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) fstep(d, s0 + d, true);
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) fstep(d, s0 + d, s0 + d < nfw); // the last nfw % D steps
+
+    // ---- interface: top yw[q] = y(n1-BW+q), bottom yw[q] = z(n1+BW-1-q)
+    double g[M2];
+#pragma unroll
+    for (int q = 0; q < BW; ++q) xch[((bot ? BW : 0) + q) * 64 + lane] = yw[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < BW; ++q) {
+        const double other = xch[((bot ? 0 : BW) + q) * 64 + lane];
+        const double mine = yw[q];
+        // g[t] = y(n1-BW+t); g[BW+t] = z(n1+t) = bottom's yw[BW-1-t]
+        if (bot) {
+            g[q] = other;
+            g[M2 - 1 - q] = mine;
+        } else {
+            g[q] = mine;
+            g[M2 - 1 - q] = other;
+        }
+    }
+    double xb[BW]; // top: x(n1-BW+t); bottom: x(n1+t)
+#pragma unroll
+    for (int t = 0; t < BW; ++t) {
+        const int a = (bot ? BW : 0) + t;
+        double v = 0.0;
+#pragma unroll
+        for (int b = 0; b < M2; ++b) v += Rl[a * M2 + b] * g[b];
+        xb[t] = v;
+    }
+#pragma unroll
+    for (int t = 0; t < BW; ++t) put(bot ? n1 + t : n1 - BW + t, xb[t]);
+
+    // ---- inside-out: back substitution (top: U1 upward; bottom: L2 downward)
+    const int nbk = bot ? nb - BW : n1 - BW; // valid steps of this lane
+    double xw[BW];                           // the BW rows solved last, nearest first
+#pragma unroll
+    for (int q = 0; q < BW; ++q) xw[q] = bot ? xb[BW - 1 - q] : xb[q];
+    double by[D], bd[D], bu[D][BW];
+    auto bload = [&](int sl_, int t) {
+        const int i = bot ? n1 + BW + t : n1 - BW - 1 - t, ic = cl(i);
+        by[sl_] = o[(size_t)nlm * ic];
+        bd[sl_] = rdg[(size_t)ic * c.nl];
+#pragma unroll
+        for (int q = 0; q < BW; ++q) {
+            bu[sl_][q] = band(ic, i + dir * (1 + q)); // the q-th nearest row solved before (top: below, bottom: above)
+        }
+    };
+#pragma unroll
+    for (int d = 0; d < D; ++d) bload(d, d);
+    auto bstep = [&](int d, int t, bool valid) { // as fstep: only the tail block is guarded
+        const int i = bot ? n1 + BW + t : n1 - BW - 1 - t;
+        double v = by[d];
+#pragma unroll
+        for (int q = 0; q < BW; ++q) v -= bu[d][q] * xw[q];
+        v *= bd[d];
+        if (valid) {
+#pragma unroll
+            for (int q = BW - 1; q > 0; --q) xw[q] = xw[q - 1];
+            xw[0] = v;
+            put(i, v);
+        }
+        bload(d, t + D);
+    };
+    int t0 = 0;
+    for (; t0 + D <= nbk; t0 += D) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) bstep(d, t0 + d, true);
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) bstep(d, t0 + d, t0 + d < nbk); // the last nbk % D steps
+}
 } // namespace
 
 __global__ void recip_diag_kernel(const double *__restrict__ LUt, int nk, int n, int bw, int nl, double *__restrict__ rd)
@@ -271,6 +477,19 @@ int gd_launch_cnab2_chain(hipStream_t st, const ChainArgs &c, bool syn_fp64)
     const size_t lds = sizeof(double) * (size_t)((((2 * c.bw + 1) + 2) & ~1) + 2 * (2 * c.bw + 1) + 2) * c.n;
     GD_CHECK(lds <= 64 * 1024, GD_ERR_ARG, "cnab2_chain: n=%d bw=%d needs %zu B of LDS", c.n, c.bw, lds);
     const bool direct = c.synV && c.syn_tang;
+    const size_t tlds = sizeof(double) * ((size_t)((((2 * c.bw + 1) + 2) & ~1) + 1) * c.n + 2 * (size_t)c.bw * 64);
+    if (direct && c.ULt && c.rdiag2 && c.Rint && c.bw <= 4 && c.n >= 2 * c.bw + 2 && tlds <= 64 * 1024) {
+        // twisted form: 64 columns per block, one wave per half
+        const dim3 tgrid((c.nlm + 63) / 64, grid.y, 2);
+#define GD_TWIST(B)                                                                                              \
+    if (c.bw == B) {                                                                                             \
+        if (syn_fp64) hipLaunchKernelGGL((cnab2_twist_kernel<B, 4, double>), tgrid, dim3(128), tlds, st, c);     \
+        else hipLaunchKernelGGL((cnab2_twist_kernel<B, 4, float>), tgrid, dim3(128), tlds, st, c);               \
+        return gd_check_launch("cnab2_twist_kernel");                                                            \
+    }
+        GD_TWIST(1) GD_TWIST(2) GD_TWIST(3) GD_TWIST(4) // wider bands: more registers than 3 waves per SIMD allow
+#undef GD_TWIST
+    }
 #define GD_CHAIN(B)                                                                                              \
     if (c.bw == B) {                                                                                             \
         if (syn_fp64 && direct) hipLaunchKernelGGL((cnab2_chain_kernel<B, 4, double, true>), grid, dim3(64), lds, st, c); \
@@ -285,3 +504,100 @@ int gd_launch_cnab2_chain(hipStream_t st, const ChainArgs &c, bool syn_fp64)
     return GD_ERR_ARG;
 }
 
+
+// ---------------------------------------------------------------- twisted column solve (tables)
+// The tangential column kernel can split each column at n1 = ceil(n/2): the
+// top rows keep the LU of S_l (its top-left block is the LU of S_l[0:n1,
+// 0:n1]), the bottom block S_l[n1:, n1:] gets a UL factorisation (eliminated
+// from the bottom up: unit upper U2, lower L2), and the 2BW unknowns next to
+// the split solve a 2BW x 2BW interface system
+//   [ U(top BW x BW)      E1 = L1^-1 B1 (its last BW rows) ] [x(n1-BW..n1-1)]   [y(n1-BW..n1-1)]
+//   [ F2 = U2^-1 C2 (first BW rows)   L2(first BW x BW)    ] [x(n1..n1+BW-1)] = [z(n1..n1+BW-1)]
+// with B1, C2 the coupling blocks of S_l.  Host FP64, once per operator.
+int gd_chain_twist_tables(const double *S, const double *LU, int n, int bw, int nl, double *ul_t, double *rd2,
+                          double *R)
+{
+    GD_CHECK(n >= 2 * bw + 2, GD_ERR_ARG, "twist tables: n=%d too small for bw=%d", n, bw);
+    const int W = 2 * bw + 1, n1 = (n + 1) / 2, m2 = 2 * bw;
+    std::vector<double> A((size_t)W * n), E(n), M((size_t)m2 * m2), Inv((size_t)m2 * m2);
+    auto bi = [&](int i, int j) { return (size_t)j * W + (bw + i - j); }; // band index, |i - j| <= bw
+    auto inb = [&](int i, int j) { return i >= 0 && j >= 0 && i < n && j < n && i - j <= bw && j - i <= bw; };
+    for (int l = 0; l < nl; ++l) {
+        const double *Sl = S + (size_t)l * W * n, *Ll = LU + (size_t)l * W * n;
+        // UL of the bottom block (band copy, entries outside the block ignored)
+        for (size_t q = 0; q < A.size(); ++q) A[q] = Sl[q];
+        for (int kc = n - 1; kc > n1; --kc) {
+            const double piv = A[bi(kc, kc)];
+            GD_CHECK(piv != 0.0 && std::isfinite(piv), GD_ERR_NUMERIC, "twist tables: zero pivot (l=%d)", l);
+            for (int i = std::max(n1, kc - bw); i < kc; ++i) {
+                const double fct = A[bi(i, kc)] / piv;
+                for (int j = std::max(n1, kc - bw); j < kc; ++j)
+                    if (inb(i, j)) A[bi(i, j)] -= fct * A[bi(kc, j)];
+                A[bi(i, kc)] = fct; // the unit-upper multiplier
+            }
+        }
+        for (int j = 0; j < n; ++j)
+            for (int r = 0; r < W; ++r) {
+                const int i = j + r - bw;
+                const bool keep = i >= n1 && j >= n1 && i < n;
+                ul_t[((size_t)j * W + r) * nl + l] = keep ? A[(size_t)j * W + r] : 0.0;
+            }
+        for (int i = 0; i < n; ++i) {
+            const double d = A[bi(i, i)];
+            if (i >= n1) GD_CHECK(d != 0.0 && std::isfinite(d), GD_ERR_NUMERIC, "twist tables: zero pivot (l=%d)", l);
+            rd2[(size_t)i * nl + l] = i >= n1 ? 1.0 / d : 0.0;
+        }
+        // interface matrix
+        for (int c = 0; c < bw; ++c) {
+            // E1 column c: forward substitution of S(:, n1 + c) over rows < n1 with unit L1
+            for (int i = 0; i < n1; ++i) {
+                double v = inb(i, n1 + c) ? Sl[bi(i, n1 + c)] : 0.0;
+                for (int j = std::max(0, i - bw); j < i; ++j) v -= Ll[bi(i, j)] * E[j];
+                E[i] = v;
+            }
+            for (int t = 0; t < bw; ++t) M[(size_t)t * m2 + bw + c] = E[n1 - bw + t];
+            // F2 column c: back substitution of S(:, n1 - bw + c) over rows >= n1 with unit U2
+            for (int i = n - 1; i >= n1; --i) {
+                double v = inb(i, n1 - bw + c) ? Sl[bi(i, n1 - bw + c)] : 0.0;
+                for (int j = i + 1; j <= std::min(n - 1, i + bw); ++j) v -= A[bi(i, j)] * E[j];
+                E[i] = v;
+            }
+            for (int t = 0; t < bw; ++t) M[(size_t)(bw + t) * m2 + c] = E[n1 + t];
+        }
+        for (int t = 0; t < bw; ++t)
+            for (int s = 0; s < bw; ++s) {
+                const int it = n1 - bw + t, is = n1 - bw + s;
+                M[(size_t)t * m2 + s] = s >= t && inb(it, is) ? Ll[bi(it, is)] : 0.0;          // U of the LU
+                const int jt = n1 + t, js = n1 + s;
+                M[(size_t)(bw + t) * m2 + bw + s] = s <= t && inb(jt, js) ? A[bi(jt, js)] : 0.0; // L2
+            }
+        // inverse by Gauss-Jordan with partial pivoting
+        for (int a = 0; a < m2; ++a)
+            for (int b = 0; b < m2; ++b) Inv[(size_t)a * m2 + b] = a == b ? 1.0 : 0.0;
+        for (int col = 0; col < m2; ++col) {
+            int p = col;
+            for (int r = col + 1; r < m2; ++r)
+                if (std::fabs(M[(size_t)r * m2 + col]) > std::fabs(M[(size_t)p * m2 + col])) p = r;
+            GD_CHECK(M[(size_t)p * m2 + col] != 0.0, GD_ERR_NUMERIC, "twist tables: singular interface (l=%d)", l);
+            for (int b = 0; b < m2; ++b) {
+                std::swap(M[(size_t)p * m2 + b], M[(size_t)col * m2 + b]);
+                std::swap(Inv[(size_t)p * m2 + b], Inv[(size_t)col * m2 + b]);
+            }
+            const double d = M[(size_t)col * m2 + col];
+            for (int b = 0; b < m2; ++b) {
+                M[(size_t)col * m2 + b] /= d;
+                Inv[(size_t)col * m2 + b] /= d;
+            }
+            for (int r = 0; r < m2; ++r)
+                if (r != col) {
+                    const double fct = M[(size_t)r * m2 + col];
+                    for (int b = 0; b < m2; ++b) {
+                        M[(size_t)r * m2 + b] -= fct * M[(size_t)col * m2 + b];
+                        Inv[(size_t)r * m2 + b] -= fct * Inv[(size_t)col * m2 + b];
+                    }
+                }
+        }
+        for (int q = 0; q < m2 * m2; ++q) R[(size_t)l * m2 * m2 + q] = Inv[q];
+    }
+    return GD_OK;
+}

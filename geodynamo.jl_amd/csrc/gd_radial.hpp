@@ -48,8 +48,16 @@ struct ChainArgs {
     double *c_out;
     const double *c_nl, *c_np;
     int c_bcr;                 // its synthesis columns: SYNV_COLS - 2, - 1 of synV
+    // twisted column solve of the tangential variant (null: one lane per
+    // column): UL factors of the bottom blocks [k][(col*W+row)][l], their
+    // reciprocal diagonals [k][row][l], interface inverses [k][l][2BW][2BW]
+    const double *ULt, *rdiag2, *Rint;
 };
 int gd_cnab2_chain_supported(int n, int bw);
+// host: the twisted-solve tables of one diffusivity from S_l and LU(S_l) in
+// band layout [l][col * W + row]; needs n >= 2 bw + 2
+int gd_chain_twist_tables(const double *S, const double *LU, int n, int bw, int nl, double *ul_t, double *rd2,
+                          double *R);
 // rdiag[(k*n + i)*nl + l] = 1 / LU_t[k][(i*W + bw)][l] for nk diffusivities
 int gd_launch_recip_diag(hipStream_t st, const double *LUt, int nk, int n, int bw, int nl, double *rdiag);
 int gd_launch_cnab2_chain(hipStream_t st, const ChainArgs &c, bool syn_fp64);

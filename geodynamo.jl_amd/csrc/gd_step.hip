@@ -63,6 +63,10 @@ struct gd_step {
     // synthesis of the tangential step with grad T (and grad C) as gradient
     // fields (1) or as full fields with zero toroidal columns (0, same values)
     int synth_grad = 1;
+    // twisted column solve (gd_chain_twist_tables): tables for the four
+    // diffusivity slots; twist = 0 keeps one lane per column
+    DevBuf ULt, rdiag2, Rint;
+    int twist = 0;
     // optional passive composition (gd_step_enable_composition, tangential
     // step): state and nonlinear history in the planes 8, 9 (field 4) of
     // five-field buffers whose other planes stay zero, so the CNAB2 kernel
@@ -453,6 +457,11 @@ static int enqueue_step(gd_step *s, int parity, bool boot, hipEvent_t *ev = null
         c.synV = s->synV.p;
         c.synS = s->synS.p;
         c.syn_tang = tangential(s) ? 1 : 0;
+        if (s->twist) {
+            c.ULt = s->ULt.as<double>();
+            c.rdiag2 = s->rdiag2.as<double>();
+            c.Rint = s->Rint.as<double>();
+        }
         if (s->comp) { // the composition as column group 5 of the same launch
             c.c_in = s->cstate[parity].as<double>();
             c.c_out = s->cstate[parity ^ 1].as<double>();
@@ -525,7 +534,7 @@ static void destroy_graphs(gd_step *s)
 static int step_release(gd_step *s)
 {
     destroy_graphs(s);
-    DevBuf *bufs[] = {&s->state[0], &s->state[1], &s->nlbuf[0], &s->nlbuf[1], &s->Lt, &s->LUt, &s->nuLap, &s->nus,
+    DevBuf *bufs[] = {&s->state[0], &s->state[1], &s->nlbuf[0], &s->nlbuf[1], &s->Lt, &s->LUt, &s->ULt, &s->rdiag2, &s->Rint, &s->nuLap, &s->nus,
                       &s->D1, &s->D2, &s->rtab, &s->synV,
                       &s->synS, &s->fb, &s->gb, &s->part, &s->eout, &s->scratch, &s->status, &s->zrow, &s->rdiag};
     for (DevBuf *b : bufs) b->release();
@@ -547,6 +556,26 @@ template <typename T> static int upload_vec(gd_ctx *ctx, DevBuf &b, const std::v
 {
     GD_TRY(b.ensure(sizeof(T) * v.size()));
     GD_HIP(hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, ctx->stream));
+    return GD_OK;
+}
+
+// the twisted-solve tables of diffusivity slot k from the operator just built
+// in S (band) and LU (band) on the device
+static int twist_slot(gd_step *s, int k, const DevBuf &S, const DevBuf &LU)
+{
+    if (!s->twist) return GD_OK;
+    const int nr = s->d.nr, bw = s->d.bw, nl = s->nl, m2 = 2 * bw;
+    const size_t nb = (size_t)s->W * nr * nl;
+    std::vector<double> hs(nb), hl(nb), ul(nb), rd((size_t)nr * nl), R((size_t)nl * m2 * m2);
+    GD_HIP(hipStreamSynchronize(s->ctx->stream));
+    GD_HIP(hipMemcpy(hs.data(), S.p, sizeof(double) * nb, hipMemcpyDeviceToHost));
+    GD_HIP(hipMemcpy(hl.data(), LU.p, sizeof(double) * nb, hipMemcpyDeviceToHost));
+    GD_TRY(gd_chain_twist_tables(hs.data(), hl.data(), nr, bw, nl, ul.data(), rd.data(), R.data()));
+    GD_HIP(hipMemcpy(s->ULt.as<double>() + k * nb, ul.data(), sizeof(double) * nb, hipMemcpyHostToDevice));
+    GD_HIP(hipMemcpy(s->rdiag2.as<double>() + (size_t)k * nr * nl, rd.data(), sizeof(double) * rd.size(),
+                     hipMemcpyHostToDevice));
+    GD_HIP(hipMemcpy(s->Rint.as<double>() + (size_t)k * nl * m2 * m2, R.data(), sizeof(double) * R.size(),
+                     hipMemcpyHostToDevice));
     return GD_OK;
 }
 
@@ -597,10 +626,23 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
         if (rc == GD_OK) rc = upload_vec(ctx, s->nuLap, nlap);
     }
     GD_CHECK(d->bc_rows >= 0 && d->bc_rows <= 2, GD_ERR_ARG, "gd_step_create: bc_rows=%d", d->bc_rows);
-    for (int k = 0; k < 3 && rc == GD_OK; ++k)
+    // the twisted column solve on the FP32-synthesis path with bw <= 4 (at
+    // lmax=127, nr=65, bw=3: 68 vs 73 us; on the FP64 path 87 vs 83 us, so
+    // FP64 keeps one lane per column); GD_CHAIN_TWIST = 0 keeps one lane here too
+    const char *tw = getenv("GD_CHAIN_TWIST");
+    s->twist = (!(tw && tw[0] == '0') && d->prec == GD_PREC_FP32 && bw <= 4 && nr >= 2 * bw + 2 &&
+                gd_cnab2_chain_supported(nr, bw)) ? 1 : 0;
+    if (s->twist && rc == GD_OK) {
+        rc = s->ULt.ensure(sizeof(double) * nb * 4);
+        if (rc == GD_OK) rc = s->rdiag2.ensure(sizeof(double) * 4 * nr * s->nl);
+        if (rc == GD_OK) rc = s->Rint.ensure(sizeof(double) * 4 * s->nl * 4 * bw * bw);
+    }
+    for (int k = 0; k < 3 && rc == GD_OK; ++k) {
         rc = gd_cnab2_build(ctx->stream, dlap.as<double>(), s->rtab.as<double>() + nr, nus[k], d->dt, d->theta, s->nl,
                             nr, bw, d->bc_rows == 1 || (d->bc_rows == 2 && k == 2), tmpL.as<double>(), tmpS.as<double>(), tmpLU.as<double>(),
                             s->Lt.as<double>() + k * nb, s->LUt.as<double>() + k * nb, s->status.as<int>());
+        if (rc == GD_OK) rc = twist_slot(s, k, tmpS, tmpLU);
+    }
     int stv = 0;
     if (rc == GD_OK && hipMemcpyAsync(&stv, s->status.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
         rc = GD_ERR_HIP;
@@ -919,6 +961,7 @@ extern "C" int gd_step_enable_composition(gd_step *s, double Sc)
         rc = gd_cnab2_build(st, dlap.as<double>(), s->rtab.as<double>() + nr, nu, d->dt, d->theta, s->nl, nr, bw,
                             d->bc_rows == 1, tmpL.as<double>(), tmpS.as<double>(), tmpLU.as<double>(),
                             s->Lt.as<double>() + 3 * nb, s->LUt.as<double>() + 3 * nb, s->status.as<int>());
+    if (rc == GD_OK) rc = twist_slot(s, 3, tmpS, tmpLU);
     if (rc == GD_OK) rc = gd_launch_recip_diag(st, s->LUt.as<double>(), 4, nr, bw, s->nl, s->rdiag.as<double>());
     int stv = 0;
     if (rc == GD_OK && hipMemcpyAsync(&stv, s->status.p, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)

@@ -1,10 +1,16 @@
-"""Kernel forms of the tangential step that must give the same bits as the
-forms they replaced, checked on whole steps (FP64 and FP32, with and without
+"""Kernel forms of the tangential step checked against the forms they
+replaced, on whole steps (FP64 and FP32, with and without
 composition, a small shell and lmax=127, nr=65):
-  * the synthesis with grad T (and grad C) as gradient fields -- S columns
+  * same bits: the synthesis with grad T (and grad C) as gradient fields -- S columns
     only -- against contracting their zero toroidal columns too
     (GD_SYNTH_GRAD=0): each MFMA output column depends only on its own input
-    column, and the dropped terms are exact zeros."""
+    column, and the dropped terms are exact zeros;
+  * equal to rounding: the twisted CNAB2 column solve (a top and a bottom
+    wave per 64 columns meeting at an interface system; the FP32-synthesis
+    path uses it, the FP64 path keeps one lane per column) against one lane
+    per column (GD_CHAIN_TWIST=0): the same linear system solved in another
+    order, so equal to rounding (max relative difference <= 1e-6 after 3
+    steps of the FP32 path, whose nonlinear terms are FP32)."""
 import os
 
 import numpy as np
@@ -30,11 +36,11 @@ def _pair(var, prec, L, nlat, nlon, nr, comp):
     return out
 
 
-@pytest.mark.parametrize("var", ["GD_SYNTH_GRAD"])
+@pytest.mark.parametrize("var", ["GD_SYNTH_GRAD", "GD_CHAIN_TWIST"])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 @pytest.mark.parametrize("comp", [False, True])
 @pytest.mark.parametrize("L,nlat,nlon,nr", [(31, 48, 96, 17), (127, 192, 384, 65)])
-def test_kernel_forms_same_bits(var, prec, comp, L, nlat, nlon, nr):
+def test_kernel_forms_agree(var, prec, comp, L, nlat, nlon, nr):
     import gdhip
 
     print("case: %s=0 vs default %s comp=%s lmax=%d nr=%d" % (var, prec, comp, L, nr), flush=True)
@@ -45,6 +51,17 @@ def test_kernel_forms_same_bits(var, prec, comp, L, nlat, nlon, nr):
         if comp:
             s.set_composition(gdhip.seeded_state(52, L, nr, nfields=1, amp=1e-3)[0])
         s.run(3, graph=True)
-    assert np.array_equal(old.get_state_complex(), new.get_state_complex())
+    a, b = old.get_state_complex(), new.get_state_complex()
+    if var == "GD_SYNTH_GRAD":
+        assert np.array_equal(a, b)
+    else:
+        tol = 1e-13 if prec == "fp64" else 1e-6
+        for f in range(5):
+            scale = max(np.abs(a[f]).max(), 1e-300)
+            assert np.abs(a[f] - b[f]).max() <= tol * scale, (f, np.abs(a[f] - b[f]).max() / scale)
     if comp:
-        assert np.array_equal(old.get_composition(), new.get_composition())
+        ca, cb = old.get_composition(), new.get_composition()
+        if var == "GD_SYNTH_GRAD":
+            assert np.array_equal(ca, cb)
+        else:
+            assert np.abs(ca - cb).max() <= (1e-13 if prec == "fp64" else 1e-6) * np.abs(ca).max()
