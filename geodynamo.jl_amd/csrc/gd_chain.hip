@@ -248,7 +248,7 @@ template <int BW, int D, typename TS, bool DIRECT> __global__ void __launch_boun
 // cnab2_chain_kernel.  Lanes past nlm repeat column nlm-1: they compute the
 // same values as its own lane and store them to the same places, which keeps
 // both waves on the barrier and the loops free of per-lane store guards.
-template <int BW, int D, typename TS> __global__ void __launch_bounds__(128) cnab2_twist_kernel(ChainArgs c)
+template <int BW, int D, typename TS> __global__ void __launch_bounds__(128, 3) cnab2_twist_kernel(ChainArgs c)
 {
     constexpr int W = 2 * BW + 1, M2 = 2 * BW;
     const int n = c.n, nlm = c.nlm, n1 = (n + 1) / 2, nb = n - n1;
@@ -261,7 +261,9 @@ template <int BW, int D, typename TS> __global__ void __launch_bounds__(128) cna
     const int k = isC ? 3 : (f < 2 ? 0 : (f < 4 ? 1 : 2));
     constexpr int ER = (W + 2) & ~1;
     extern __shared__ double smem[];
-    double *Ex = smem, *rinv_ = Ex + ER * n, *xch = rinv_ + n; // xch: [half][BW][64]
+    // Ex: row i of the explicit operator at [ER*i + q] (column i - BW + q; q = W: 1/r^2);
+    // ExB: the same rows mirrored (column i + BW - q at q), so both halves read er[q]
+    double *Ex = smem, *ExB = Ex + ER * n, *rinv_ = ExB + ER * n, *xch = rinv_ + n; // xch: [half][BW][64]
     const double *Lapk = c.nuLap + (size_t)k * W * n;
     for (int t = threadIdx.x; t < ER * n; t += 128) {
         const int i = t / ER, q = t - i * ER, j = i - BW + q;
@@ -270,6 +272,7 @@ template <int BW, int D, typename TS> __global__ void __launch_bounds__(128) cna
         else if (q < W && j >= 0 && j < n) v = Lapk[(2 * BW - q) + W * j];
         else if (q == W) v = c.rtab[i + n];
         Ex[t] = v;
+        ExB[ER * i + (q < W ? 2 * BW - q : q)] = v;
     }
     for (int t = threadIdx.x; t < n; t += 128) rinv_[t] = c.rtab[t + 2 * n];
     __syncthreads();
@@ -284,7 +287,7 @@ template <int BW, int D, typename TS> __global__ void __launch_bounds__(128) cna
     const bool nlz = !isC && ((c.nl_zero_mask >> f) & 1);
     const double *pn = nlz ? c.zero_row + idx : (isC ? c.c_nl : c.nl_arr) + off;
     const double *pp = nlz ? c.zero_row + idx : (isC ? c.c_np : c.np_arr) + off;
-    const size_t nstride = nlz ? 0 : (size_t)nlm;
+    const unsigned nstride = nlz ? 0u : (unsigned)nlm;
     double *o = (isC ? c.c_out : c.st_out) + off;
     const bool tkeep = isC || ((f == 4) && ((c.bc_mask >> 5) & 1));
     const bool tbc = (f == 4) && !tkeep;
@@ -300,13 +303,13 @@ template <int BW, int D, typename TS> __global__ void __launch_bounds__(128) cna
     // used D steps later, so the prefetch is never drained by a select.
     auto band = [&](int ic, int j) {
         const int jc = (j >= 0 && j < n) ? j : ic;
-        return T[((size_t)jc * W + (BW + ic - jc)) * c.nl];
+        return T[(unsigned)(jc * W + (BW + ic - jc)) * (unsigned)c.nl];
     };
     const bool wA = f == 0 || f == 1 || f == 4;
     TS *pA = static_cast<TS *>(c.synV) + (size_t)((isC ? SYNV_COLS - 2 : (f == 0 ? 2 : (f == 1 ? 0 : 16))) + ri) * nlm + idx;
     auto put = [&](int i, double v) { // x row i: state and (u, T/r, C/r) synthesis column
-        o[(size_t)nlm * i] = v;
-        if (wA) pA[(size_t)i * SYNV_COLS * nlm] = (TS)(f == 4 ? v * rinv_[i] : v);
+        o[(unsigned)nlm * i] = v;
+        if (wA) pA[(unsigned)(i * SYNV_COLS) * (unsigned)nlm] = (TS)(f == 4 ? v * rinv_[i] : v);
     };
 
     // ---- outward-in: rhs rows and the unit-triangular substitution
@@ -317,14 +320,14 @@ template <int BW, int D, typename TS> __global__ void __launch_bounds__(128) cna
 #pragma unroll
         for (int q = 0; q < W; ++q) {
             const int j = i0 + dir * (q - BW);
-            const double v = u[(size_t)nlm * cl(j)];
+            const double v = u[(unsigned)nlm * cl(j)];
             w[q] = urow(j) ? v : 0.0;
         }
     }
     double su[D], sn[D], sp[D], sl[D][BW];
     auto fload = [&](int sl_, int s) {
         const int i = bot ? n - 1 - s : s, ic = cl(i), jn = i + dir * (BW + 1);
-        su[sl_] = u[(size_t)nlm * cl(jn)]; // masked with urow at its use
+        su[sl_] = u[(unsigned)nlm * cl(jn)]; // masked with urow at its use
         sn[sl_] = pn[nstride * ic];
         sp[sl_] = pp[nstride * ic];
 #pragma unroll
@@ -341,10 +344,10 @@ template <int BW, int D, typename TS> __global__ void __launch_bounds__(128) cna
     // tail block, so full blocks shift the windows as plain register renames
     auto fstep = [&](int d, int s, bool valid) {
         const int i = bot ? n - 1 - s : s, ic = cl(i);
-        const double *er = Ex + ER * ic;
+        const double *er = (bot ? ExB : Ex) + ER * ic;
         double lr = 0.0;
 #pragma unroll
-        for (int q = 0; q < W; ++q) lr += ((q == BW) ? er[BW] - nulf * er[W] : er[bot ? 2 * BW - q : q]) * w[q];
+        for (int q = 0; q < W; ++q) lr += ((q == BW) ? er[BW] - nulf * er[W] : er[q]) * w[q];
         double v = inv_dt * w[BW] + 1.5 * sn[d] - 0.5 * sp[d];
         v += lw * lr;
         v = (bcr && (i == 0 || i == n - 1)) ? (tkeep ? w[BW] : 0.0) : v; // Dirichlet value
@@ -357,7 +360,7 @@ template <int BW, int D, typename TS> __global__ void __launch_bounds__(128) cna
 #pragma unroll
             for (int q = 0; q < W - 1; ++q) w[q] = w[q + 1];
             w[W - 1] = urow(i + dir * (BW + 1)) ? su[d] : 0.0;
-            o[(size_t)nlm * i] = v;
+            o[(unsigned)nlm * i] = v;
         }
         fload(d, s + D);
     };
@@ -407,8 +410,8 @@ template <int BW, int D, typename TS> __global__ void __launch_bounds__(128) cna
     double by[D], bd[D], bu[D][BW];
     auto bload = [&](int sl_, int t) {
         const int i = bot ? n1 + BW + t : n1 - BW - 1 - t, ic = cl(i);
-        by[sl_] = o[(size_t)nlm * ic];
-        bd[sl_] = rdg[(size_t)ic * c.nl];
+        by[sl_] = o[(unsigned)nlm * ic];
+        bd[sl_] = rdg[(unsigned)ic * (unsigned)c.nl];
 #pragma unroll
         for (int q = 0; q < BW; ++q) {
             bu[sl_][q] = band(ic, i + dir * (1 + q)); // the q-th nearest row solved before (top: below, bottom: above)
@@ -477,8 +480,11 @@ int gd_launch_cnab2_chain(hipStream_t st, const ChainArgs &c, bool syn_fp64)
     const size_t lds = sizeof(double) * (size_t)((((2 * c.bw + 1) + 2) & ~1) + 2 * (2 * c.bw + 1) + 2) * c.n;
     GD_CHECK(lds <= 64 * 1024, GD_ERR_ARG, "cnab2_chain: n=%d bw=%d needs %zu B of LDS", c.n, c.bw, lds);
     const bool direct = c.synV && c.syn_tang;
-    const size_t tlds = sizeof(double) * ((size_t)((((2 * c.bw + 1) + 2) & ~1) + 1) * c.n + 2 * (size_t)c.bw * 64);
-    if (direct && c.ULt && c.rdiag2 && c.Rint && c.bw <= 4 && c.n >= 2 * c.bw + 2 && tlds <= 64 * 1024) {
+    const size_t tlds = sizeof(double) * ((size_t)(2 * (((2 * c.bw + 1) + 2) & ~1) + 1) * c.n + 2 * (size_t)c.bw * 64);
+    // the twisted kernel addresses rows, synthesis columns and tables with 32-bit offsets
+    const bool off32 = (size_t)SYNV_COLS * c.nlm * c.n < ((size_t)1 << 32) &&
+                       (size_t)(2 * c.bw + 1) * c.n * c.nl < ((size_t)1 << 32);
+    if (direct && c.ULt && c.rdiag2 && c.Rint && c.bw <= 3 && c.n >= 2 * c.bw + 2 && tlds <= 64 * 1024 && off32) {
         // twisted form: 64 columns per block, one wave per half
         const dim3 tgrid((c.nlm + 63) / 64, grid.y, 2);
 #define GD_TWIST(B)                                                                                              \
@@ -487,7 +493,7 @@ int gd_launch_cnab2_chain(hipStream_t st, const ChainArgs &c, bool syn_fp64)
         else hipLaunchKernelGGL((cnab2_twist_kernel<B, 4, float>), tgrid, dim3(128), tlds, st, c);               \
         return gd_check_launch("cnab2_twist_kernel");                                                            \
     }
-        GD_TWIST(1) GD_TWIST(2) GD_TWIST(3) GD_TWIST(4) // wider bands: more registers than 3 waves per SIMD allow
+        GD_TWIST(1) GD_TWIST(2) GD_TWIST(3) // wider bands spill at 3 waves per SIMD
 #undef GD_TWIST
     }
 #define GD_CHAIN(B)                                                                                              \

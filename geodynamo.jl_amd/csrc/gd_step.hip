@@ -626,11 +626,11 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
         if (rc == GD_OK) rc = upload_vec(ctx, s->nuLap, nlap);
     }
     GD_CHECK(d->bc_rows >= 0 && d->bc_rows <= 2, GD_ERR_ARG, "gd_step_create: bc_rows=%d", d->bc_rows);
-    // the twisted column solve on the FP32-synthesis path with bw <= 4 (at
+    // the twisted column solve on the FP32-synthesis path with bw <= 3 (at
     // lmax=127, nr=65, bw=3: 68 vs 73 us; on the FP64 path 87 vs 83 us, so
     // FP64 keeps one lane per column); GD_CHAIN_TWIST = 0 keeps one lane here too
     const char *tw = getenv("GD_CHAIN_TWIST");
-    s->twist = (!(tw && tw[0] == '0') && d->prec == GD_PREC_FP32 && bw <= 4 && nr >= 2 * bw + 2 &&
+    s->twist = (!(tw && tw[0] == '0') && d->prec == GD_PREC_FP32 && bw <= 3 && nr >= 2 * bw + 2 &&
                 gd_cnab2_chain_supported(nr, bw)) ? 1 : 0;
     if (s->twist && rc == GD_OK) {
         rc = s->ULt.ensure(sizeof(double) * nb * 4);
