@@ -23,7 +23,11 @@
 // curl and radial-component factors): when x[i] is known, row i + BW has its
 // whole band window x[i..i+2BW], so the state is not read again.
 namespace {
-template <int BW, int D, typename TS, bool DIRECT> __global__ void __launch_bounds__(64) cnab2_chain_kernel(ChainArgs c)
+// IX: the type of the row, synthesis-column and table offsets from the
+// per-column base pointers -- unsigned where they fit 32 bits (each offset is
+// then one scalar multiply instead of a 64-bit product), size_t otherwise.
+template <int BW, int D, typename TS, bool DIRECT, typename IX>
+__global__ void __launch_bounds__(64) cnab2_chain_kernel(ChainArgs c)
 {
     constexpr int W = 2 * BW + 1;
     const int n = c.n, nlm = c.nlm;
@@ -72,7 +76,7 @@ template <int BW, int D, typename TS, bool DIRECT> __global__ void __launch_boun
     const bool nlz = !isC && ((c.nl_zero_mask >> f) & 1);
     const double *pn = nlz ? c.zero_row + idx : (isC ? c.c_nl : c.nl_arr) + off;
     const double *pp = nlz ? c.zero_row + idx : (isC ? c.c_np : c.np_arr) + off;
-    const size_t nstride = nlz ? 0 : (size_t)nlm;
+    const IX nstride = nlz ? IX(0) : IX(nlm);
     double *o = (isC ? c.c_out : c.st_out) + off;
     // bit 5 of bc_mask: a boundary-condition stage has already set the
     // temperature rows in st_in; read them as stored, and Dirichlet rhs rows
@@ -84,20 +88,20 @@ template <int BW, int D, typename TS, bool DIRECT> __global__ void __launch_boun
     auto cl = [&](int j) { return j < 0 ? 0 : (j > n - 1 ? n - 1 : j); }; // a valid row
     auto urow = [&](int j) { return j >= 0 && j < n && !(tbc && (j == 0 || j == n - 1)); };
     // band element (row i, column j), |i - j| <= BW, of the transposed LU table at valid rows
-    auto lu = [&](int i, int j) { return LU[((size_t)cl(j) * W + (BW + i - j)) * c.nl]; };
+    auto lu = [&](int i, int j) { return LU[IX(cl(j) * W + (BW + i - j)) * IX(c.nl)]; };
 
     // ---- forward: rhs row and L-substitution
     double w[W]; // u rows i-BW..i+BW (temperature boundary rows read as 0)
 #pragma unroll
     for (int q = 0; q < W; ++q) {
         const int j = q - BW;
-        const double v = u[(size_t)nlm * cl(j)];
+        const double v = u[IX(nlm) * IX(cl(j))];
         w[q] = urow(j) ? v : 0.0;
     }
     double su[D], sn[D], sp[D], sl[D][BW];
     auto fload = [&](int s, int i) {
         const int jn = i + BW + 1, ic = cl(i);
-        const double v = u[(size_t)nlm * cl(jn)];
+        const double v = u[IX(nlm) * IX(cl(jn))];
         su[s] = urow(jn) ? v : 0.0;
         sn[s] = pn[nstride * ic];
         sp[s] = pp[nstride * ic];
@@ -132,7 +136,7 @@ template <int BW, int D, typename TS, bool DIRECT> __global__ void __launch_boun
 #pragma unroll
             for (int q = 0; q < W - 1; ++q) w[q] = w[q + 1];
             w[W - 1] = su[d];
-            if (i < n) o[(size_t)nlm * i] = v;
+            if (i < n) o[IX(nlm) * IX(i)] = v;
             fload(d, i + D);
         }
     }
@@ -154,8 +158,8 @@ template <int BW, int D, typename TS, bool DIRECT> __global__ void __launch_boun
     double by[D], bd[D], bu[D][BW];
     auto bload = [&](int s, int i) {
         const int ic = cl(i);
-        by[s] = o[(size_t)nlm * ic];
-        bd[s] = rdg[(size_t)ic * c.nl];
+        by[s] = o[IX(nlm) * IX(ic)];
+        bd[s] = rdg[IX(ic) * IX(c.nl)];
 #pragma unroll
         for (int q = 0; q < BW; ++q) {
             const int j = i + 1 + q;
@@ -185,9 +189,9 @@ template <int BW, int D, typename TS, bool DIRECT> __global__ void __launch_boun
 #pragma unroll
             for (int q = W - 1; q > 0; --q) xw[q] = xw[q - 1];
             xw[0] = v;
-            if (i >= 0) o[(size_t)nlm * i] = v;
+            if (i >= 0) o[IX(nlm) * IX(i)] = v;
             if constexpr (DIRECT) {
-                if (wA && i >= 0) pA[(size_t)i * SYNV_COLS * nlm] = (TS)(f == 4 ? v * rinv_[i] : v);
+                if (wA && i >= 0) pA[IX(i * SYNV_COLS) * IX(nlm)] = (TS)(f == 4 ? v * rinv_[i] : v);
             } else {
             const int r = i + BW; // xw[q] = x[r - BW + q]
             if (syn && wA && r >= 0 && r < n) {
@@ -224,7 +228,7 @@ template <int BW, int D, typename TS, bool DIRECT> __global__ void __launch_boun
                     bv = -lf * rinv2 * x;
                     sv = lf * bv * rinv2;
                 }
-                const size_t ov = (size_t)r * SYNV_COLS * nlm, os = (size_t)r * 10 * nlm;
+                const IX ov = IX(r * SYNV_COLS) * IX(nlm), os = IX(r * 10) * IX(nlm);
                 pA[ov] = (TS)a;
                 if (wB) pB[ov] = (TS)bv;
                 if (wS) pS[os] = (TS)sv;
@@ -481,7 +485,8 @@ int gd_launch_cnab2_chain(hipStream_t st, const ChainArgs &c, bool syn_fp64)
     GD_CHECK(lds <= 64 * 1024, GD_ERR_ARG, "cnab2_chain: n=%d bw=%d needs %zu B of LDS", c.n, c.bw, lds);
     const bool direct = c.synV && c.syn_tang;
     const size_t tlds = sizeof(double) * ((size_t)(2 * (((2 * c.bw + 1) + 2) & ~1) + 1) * c.n + 2 * (size_t)c.bw * 64);
-    // the twisted kernel addresses rows, synthesis columns and tables with 32-bit offsets
+    // row, synthesis-column and table offsets from the per-column base pointers
+    // fit 32 bits (fewer scalar instructions per level); the twisted kernel needs it
     const bool off32 = (size_t)SYNV_COLS * c.nlm * c.n < ((size_t)1 << 32) &&
                        (size_t)(2 * c.bw + 1) * c.n * c.nl < ((size_t)1 << 32);
     if (direct && c.ULt && c.rdiag2 && c.Rint && c.bw <= 3 && c.n >= 2 * c.bw + 2 && tlds <= 64 * 1024 && off32) {
@@ -496,16 +501,21 @@ int gd_launch_cnab2_chain(hipStream_t st, const ChainArgs &c, bool syn_fp64)
         GD_TWIST(1) GD_TWIST(2) GD_TWIST(3) // wider bands spill at 3 waves per SIMD
 #undef GD_TWIST
     }
+#define GD_CHAIN_IX(B, IX)                                                                                       \
+    {                                                                                                            \
+        if (syn_fp64 && direct) hipLaunchKernelGGL((cnab2_chain_kernel<B, 4, double, true, IX>), grid, dim3(64), lds, st, c); \
+        else if (syn_fp64) hipLaunchKernelGGL((cnab2_chain_kernel<B, 4, double, false, IX>), grid, dim3(64), lds, st, c); \
+        else if (direct) hipLaunchKernelGGL((cnab2_chain_kernel<B, 4, float, true, IX>), grid, dim3(64), lds, st, c); \
+        else hipLaunchKernelGGL((cnab2_chain_kernel<B, 4, float, false, IX>), grid, dim3(64), lds, st, c);       \
+    }
 #define GD_CHAIN(B)                                                                                              \
     if (c.bw == B) {                                                                                             \
-        if (syn_fp64 && direct) hipLaunchKernelGGL((cnab2_chain_kernel<B, 4, double, true>), grid, dim3(64), lds, st, c); \
-        else if (syn_fp64) hipLaunchKernelGGL((cnab2_chain_kernel<B, 4, double, false>), grid, dim3(64), lds, st, c); \
-        else if (direct) hipLaunchKernelGGL((cnab2_chain_kernel<B, 4, float, true>), grid, dim3(64), lds, st, c); \
-        else hipLaunchKernelGGL((cnab2_chain_kernel<B, 4, float, false>), grid, dim3(64), lds, st, c);           \
+        if (off32) GD_CHAIN_IX(B, unsigned) else GD_CHAIN_IX(B, size_t)                                          \
         return gd_check_launch("cnab2_chain_kernel");                                                            \
     }
     GD_CHAIN(1) GD_CHAIN(2) GD_CHAIN(3) GD_CHAIN(4) GD_CHAIN(5) GD_CHAIN(6) GD_CHAIN(7)
 #undef GD_CHAIN
+#undef GD_CHAIN_IX
     gd_set_error("cnab2_chain: no kernel for n=%d bw=%d", c.n, c.bw);
     return GD_ERR_ARG;
 }
