@@ -99,7 +99,7 @@ TANG_KERNELS = {"spectral_pre": "spectral_pre_kernel<{T}>",
                 "synth_vector": ("synth_kernel<{T}, true, 1, 3, 16, true, 1>", "synth_kernel<{T}, true, 2, 4, 16, true>"),
                 "ring_fft_products": "ring_kernel<{T}, 3, 384>", "analysis_vector": "ana_res_kernel<{T}, true, 1, 96>",
                 "analysis_scalar": "ana_res_kernel<{T}, false, 1, 96>",
-                "cnab2_solve": ("cnab2_chain_kernel<3, 4, {T}, true>", "cnab2_twist_kernel<3, 4, {T}>")}
+                "cnab2_solve": ("cnab2_chain_kernel<3, 4, {T}, true,", "cnab2_twist_kernel<3, 4, {T}>")}
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "pmc_step_lmax127_nr65.json")
 COPY_PROBE_BYTES = 268435456
 
