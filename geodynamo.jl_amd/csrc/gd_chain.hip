@@ -487,7 +487,7 @@ int gd_launch_cnab2_chain(hipStream_t st, const ChainArgs &c, bool syn_fp64)
     const size_t tlds = sizeof(double) * ((size_t)(2 * (((2 * c.bw + 1) + 2) & ~1) + 1) * c.n + 2 * (size_t)c.bw * 64);
     // row, synthesis-column and table offsets from the per-column base pointers
     // fit 32 bits (fewer scalar instructions per level); the twisted kernel needs it
-    const bool off32 = (size_t)SYNV_COLS * c.nlm * c.n < ((size_t)1 << 32) &&
+    const bool off32 = !c.off64 && (size_t)SYNV_COLS * c.nlm * c.n < ((size_t)1 << 32) &&
                        (size_t)(2 * c.bw + 1) * c.n * c.nl < ((size_t)1 << 32);
     if (direct && c.ULt && c.rdiag2 && c.Rint && c.bw <= 3 && c.n >= 2 * c.bw + 2 && tlds <= 64 * 1024 && off32) {
         // twisted form: 64 columns per block, one wave per half

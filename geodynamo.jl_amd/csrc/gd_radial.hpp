@@ -52,6 +52,7 @@ struct ChainArgs {
     // column): UL factors of the bottom blocks [k][(col*W+row)][l], their
     // reciprocal diagonals [k][row][l], interface inverses [k][l][2BW][2BW]
     const double *ULt, *rdiag2, *Rint;
+    int off64; // nonzero: 64-bit offsets even where 32 bits suffice (one lane per column)
 };
 int gd_cnab2_chain_supported(int n, int bw);
 // host: the twisted-solve tables of one diffusivity from S_l and LU(S_l) in

@@ -63,6 +63,9 @@ struct gd_step {
     // synthesis of the tangential step with grad T (and grad C) as gradient
     // fields (1) or as full fields with zero toroidal columns (0, same values)
     int synth_grad = 1;
+    // CNAB2 column kernels with 64-bit offsets where 32 bits suffice (1; the
+    // form large shells use, the same bits) or as sized (0)
+    int chain_off64 = 0;
     // twisted column solve (gd_chain_twist_tables): tables for the four
     // diffusivity slots; twist = 0 keeps one lane per column
     DevBuf ULt, rdiag2, Rint;
@@ -457,6 +460,7 @@ static int enqueue_step(gd_step *s, int parity, bool boot, hipEvent_t *ev = null
         c.synV = s->synV.p;
         c.synS = s->synS.p;
         c.syn_tang = tangential(s) ? 1 : 0;
+        c.off64 = s->chain_off64;
         if (s->twist) {
             c.ULt = s->ULt.as<double>();
             c.rdiag2 = s->rdiag2.as<double>();
@@ -691,6 +695,9 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
     // fields as well (the earlier form; the results are the same bits)
     const char *sg = getenv("GD_SYNTH_GRAD");
     s->synth_grad = !(sg && sg[0] == '0');
+    // GD_CHAIN_OFF64 = 1 runs the CNAB2 column kernel with 64-bit offsets
+    const char *o64 = getenv("GD_CHAIN_OFF64");
+    s->chain_off64 = (o64 && o64[0] == '1') ? 1 : 0;
     return GD_OK;
 }
 

@@ -10,7 +10,12 @@ composition, a small shell and lmax=127, nr=65):
     path uses it, the FP64 path keeps one lane per column) against one lane
     per column (GD_CHAIN_TWIST=0): the same linear system solved in another
     order, so equal to rounding (max relative difference <= 1e-6 after 3
-    steps of the FP32 path, whose nonlinear terms are FP32)."""
+    steps of the FP32 path, whose nonlinear terms are FP32);
+  * same bits on the FP64 path: the CNAB2 column kernel with 64-bit offsets
+    (GD_CHAIN_OFF64=1, the form shells with SYNV_COLS*nlm*nr >= 2^32 run)
+    against the 32-bit offsets used where they fit; on the FP32 path the
+    64-bit form is one lane per column, so there it is the twisted solve's
+    rounding bound."""
 import os
 
 import numpy as np
@@ -19,11 +24,14 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
+ALT = {"GD_SYNTH_GRAD": "0", "GD_CHAIN_TWIST": "0", "GD_CHAIN_OFF64": "1"}  # the non-default form
+
+
 def _pair(var, prec, L, nlat, nlon, nr, comp):
     import gdhip
 
     out = []
-    for flag in ("0", None):
+    for flag in (ALT[var], None):
         if flag is None:
             os.environ.pop(var, None)
         else:
@@ -36,14 +44,14 @@ def _pair(var, prec, L, nlat, nlon, nr, comp):
     return out
 
 
-@pytest.mark.parametrize("var", ["GD_SYNTH_GRAD", "GD_CHAIN_TWIST"])
+@pytest.mark.parametrize("var", sorted(ALT))
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 @pytest.mark.parametrize("comp", [False, True])
 @pytest.mark.parametrize("L,nlat,nlon,nr", [(31, 48, 96, 17), (127, 192, 384, 65)])
 def test_kernel_forms_agree(var, prec, comp, L, nlat, nlon, nr):
     import gdhip
 
-    print("case: %s=0 vs default %s comp=%s lmax=%d nr=%d" % (var, prec, comp, L, nr), flush=True)
+    print("case: %s=%s vs default %s comp=%s lmax=%d nr=%d" % (var, ALT[var], prec, comp, L, nr), flush=True)
     st = gdhip.seeded_state(51, L, nr, amp=1e-3)
     old, new = _pair(var, prec, L, nlat, nlon, nr, comp)
     for s in (old, new):
@@ -52,7 +60,8 @@ def test_kernel_forms_agree(var, prec, comp, L, nlat, nlon, nr):
             s.set_composition(gdhip.seeded_state(52, L, nr, nfields=1, amp=1e-3)[0])
         s.run(3, graph=True)
     a, b = old.get_state_complex(), new.get_state_complex()
-    if var == "GD_SYNTH_GRAD":
+    bitwise = var == "GD_SYNTH_GRAD" or (var == "GD_CHAIN_OFF64" and prec == "fp64")
+    if bitwise:
         assert np.array_equal(a, b)
     else:
         tol = 1e-13 if prec == "fp64" else 1e-6
@@ -61,7 +70,7 @@ def test_kernel_forms_agree(var, prec, comp, L, nlat, nlon, nr):
             assert np.abs(a[f] - b[f]).max() <= tol * scale, (f, np.abs(a[f] - b[f]).max() / scale)
     if comp:
         ca, cb = old.get_composition(), new.get_composition()
-        if var == "GD_SYNTH_GRAD":
+        if bitwise:
             assert np.array_equal(ca, cb)
         else:
             assert np.abs(ca - cb).max() <= (1e-13 if prec == "fp64" else 1e-6) * np.abs(ca).max()
