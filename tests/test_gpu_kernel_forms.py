@@ -74,3 +74,40 @@ def test_kernel_forms_agree(var, prec, comp, L, nlat, nlon, nr):
             assert np.array_equal(ca, cb)
         else:
             assert np.abs(ca - cb).max() <= (1e-13 if prec == "fp64" else 1e-6) * np.abs(ca).max()
+
+
+@pytest.mark.parametrize("nr,bw", [(8, 3), (9, 3), (12, 3), (13, 3), (6, 2), (11, 2), (4, 1), (17, 1), (33, 2)])
+def test_twisted_solve_edge_shapes(nr, bw):
+    """The twisted solve at the smallest shell it takes (nr = 2 bw + 2), odd and
+    even nr where only the guarded tail blocks run, and every band width it is
+    built for (bw 1..3), with composition: equal to the one-lane solve to the
+    FP32 path's rounding bound after 3 steps."""
+    import gdhip
+
+    print("case: twisted solve edge shape nr=%d bw=%d" % (nr, bw), flush=True)
+    L, nlat, nlon = 15, 24, 48
+    st = gdhip.seeded_state(53, L, nr, amp=1e-3)
+    C0 = gdhip.seeded_state(54, L, nr, nfields=1, amp=1e-3)[0]
+    out = []
+    for flag in ("0", None):
+        if flag is None:
+            os.environ.pop("GD_CHAIN_TWIST", None)
+        else:
+            os.environ["GD_CHAIN_TWIST"] = flag
+        s = gdhip.CNAB2Step(L, nlat, nlon, nr, Ra=500.0, dt=1e-5, precision="fp32", bw=bw)
+        s.enable_composition(2.0)
+        out.append(s)
+    os.environ.pop("GD_CHAIN_TWIST", None)
+    for s in out:
+        s.set_state_complex(st)
+        s.set_composition(C0)
+        s.run(3, graph=True)
+    a, b = out[0].get_state_complex(), out[1].get_state_complex()
+    for f in range(5):
+        scale = max(np.abs(a[f]).max(), 1e-300)
+        assert np.abs(a[f] - b[f]).max() <= 1e-6 * scale, (f, np.abs(a[f] - b[f]).max() / scale)
+    ca, cb = out[0].get_composition(), out[1].get_composition()
+    assert np.abs(ca - cb).max() <= 1e-6 * np.abs(ca).max()
+    assert np.all(np.isfinite(b))
+    # the two solves round differently somewhere: the twisted form did run
+    assert not np.array_equal(a, b)
