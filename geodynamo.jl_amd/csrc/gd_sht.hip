@@ -252,17 +252,23 @@ __device__ __forceinline__ void synth_pipeline(const SynthArgs &a, const T *__re
                 }
         }
     };
-    if (nC > 0) gload(0);
-    for (int c = 0; c < nC; ++c) {
+    // chunk c: stage it, start loading c + 1, contract it into acc.  The even
+    // chunks (c < nE) and the odd ones run as two loops, each with its own
+    // accumulator set: a per-chunk choice between the sets made the compiler
+    // copy every accumulator register on each iteration.
+    auto step = [&](int c, acc_t<T>(&acc)[NT][2]) {
         sstore(c & 1);
         T bcur[RB];
 #pragma unroll
         for (int s = 0; s < RB; ++s) bcur[s] = rb[s];
         __syncthreads();
         if (c + 1 < nC) gload(c + 1);
-        if (c < nE) mma(c & 1, bcur, accE);
-        else mma(c & 1, bcur, accO);
-    }
+        mma(c & 1, bcur, acc);
+    };
+    if (nC > 0) gload(0);
+    int c = 0;
+    for (; c < nE; ++c) step(c, accE);
+    for (; c < nC; ++c) step(c, accO);
 }
 
 template <typename T, bool VEC, int NF, int NW, int KC, int NG = 0> constexpr size_t synth_lds_bytes()
