@@ -89,6 +89,42 @@ __device__ __forceinline__ doublex4 mfma16x16x4(double a, double b, doublex4 c)
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// Split-precision BF16 ("BF16x3") contraction of FP32 operands on the
+// 16x16x16 BF16 MFMA: x = hi + lo + r with hi = bf16(x), lo = bf16(x - hi)
+// (round to nearest even), |r| <= 2^-17 |x|; a b is taken as
+// hi_a hi_b + hi_a lo_b + lo_a hi_b with FP32 accumulation, so each product
+// carries a relative error of about 2^-16.  Lane l supplies A[l%16][4(l/16)+j]
+// and B[4(l/16)+j][l%16], j = 0..3 (the k order of four 16x16x4 FP32 steps
+// s = j whose lane group l/16 takes k = 4(l/16)+s), and receives the FP32
+// accumulator layout of mfma16x16x4.
+typedef short shortx4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ unsigned bf16_rne(float x)
+{
+    const unsigned u = __float_as_uint(x);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+struct Bf16Pair {
+    shortx4 hi, lo;
+};
+__device__ __forceinline__ Bf16Pair bf16_split4(float x0, float x1, float x2, float x3)
+{
+    Bf16Pair p;
+    const float x[4] = {x0, x1, x2, x3};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const unsigned h = bf16_rne(x[j]);
+        p.hi[j] = (short)h;
+        p.lo[j] = (short)bf16_rne(x[j] - __uint_as_float(h << 16));
+    }
+    return p;
+}
+__device__ __forceinline__ floatx4 mfma_bf16x3(const Bf16Pair &a, const Bf16Pair &b, floatx4 c)
+{
+    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a.lo, b.hi, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a.hi, b.lo, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a.hi, b.hi, c, 0, 0, 0);
+}
+
 template <typename T> __device__ __forceinline__ int acc_row(int lane, int v);
 template <> __device__ __forceinline__ int acc_row<float>(int lane, int v) { return 4 * (lane >> 4) + v; }
 template <> __device__ __forceinline__ int acc_row<double>(int lane, int v) { return (lane >> 4) + 4 * v; }

@@ -54,8 +54,8 @@ int gd_sht_plan_build(gd_ctx *ctx, int lmax, int mmax, int nlat, int nlon, int p
     GD_CHECK(lmax >= 1 && mmax >= 0 && mmax <= lmax, GD_ERR_ARG, "sht: lmax=%d mmax=%d", lmax, mmax);
     GD_CHECK(nlat % 2 == 0 && nlat >= 2 && nlat > lmax, GD_ERR_ARG, "sht: nlat=%d must be even and > lmax", nlat);
     GD_CHECK(nlon > 2 * mmax, GD_ERR_ARG, "sht: nlon=%d must exceed 2*mmax", nlon);
-    GD_CHECK(prec == GD_PREC_FP64 || prec == GD_PREC_FP32, GD_ERR_ARG,
-             "sht: precision %d not available (FP64=0, FP32=1)", prec);
+    GD_CHECK(prec == GD_PREC_FP64 || prec == GD_PREC_FP32 || prec == GD_PREC_BF16X3, GD_ERR_ARG,
+             "sht: precision %d not available (FP64=0, FP32=1, BF16X3=2)", prec);
     p->ctx = ctx;
     p->lmax = lmax;
     p->mmax = mmax;
@@ -75,6 +75,7 @@ int gd_sht_plan_build(gd_ctx *ctx, int lmax, int mmax, int nlat, int nlon, int p
     GD_TRY(upload(ctx, p->d_l_of_idx, p->maps.l_of_idx));
     GD_TRY(upload(ctx, p->d_lmaj_of_idx, p->maps.lmaj_of_idx));
     GD_TRY(upload(ctx, p->d_idx_of_lmaj, p->maps.idx_of_lmaj));
+    GD_TRY(upload(ctx, p->d_m_of_idx, p->maps.m_of_idx));
     std::vector<double> xn(p->x.begin(), p->x.begin() + p->nlat2);
     GD_TRY(upload(ctx, p->d_x, xn));
     const size_t ntab = (size_t)3 * p->nlm * p->nlat2;
@@ -98,12 +99,15 @@ int gd_sht_plan_build(gd_ctx *ctx, int lmax, int mmax, int nlat, int nlon, int p
         c[i] = std::cos(p->theta[i]);
     }
     for (int i = 0; i < p->nlat2; ++i) wq[i] = 2.0 * M_PI * p->w[i] / nlon;
+    GD_TRY(upload(ctx, p->wq64, wq));
     if (prec == GD_PREC_FP64) {
+        GD_TRY(upload(ctx, p->wone, std::vector<double>(p->nlat2, 1.0)));
         GD_TRY(upload(ctx, p->tw, tw));
         GD_TRY(upload(ctx, p->sinth, s));
         GD_TRY(upload(ctx, p->costh, c));
         GD_TRY(upload(ctx, p->wq, wq));
     } else {
+        GD_TRY(upload(ctx, p->wone, std::vector<float>(p->nlat2, 1.0f)));
         GD_TRY(upload(ctx, p->tw, std::vector<float>(tw.begin(), tw.end())));
         GD_TRY(upload(ctx, p->sinth, std::vector<float>(s.begin(), s.end())));
         GD_TRY(upload(ctx, p->costh, std::vector<float>(c.begin(), c.end())));
@@ -116,6 +120,7 @@ int gd_sht_plan_build(gd_ctx *ctx, int lmax, int mmax, int nlat, int nlon, int p
 void gd_sht_plan_release(gd_sht_plan *p)
 {
     DevBuf *bufs[] = {&p->d_moff, &p->d_ne, &p->d_l_of_idx, &p->d_lmaj_of_idx, &p->d_idx_of_lmaj, &p->d_x,
+                      &p->d_m_of_idx, &p->wone, &p->wq64,
                       &p->tab64, &p->tab32, &p->tw, &p->sinth, &p->costh, &p->wq, &p->syn, &p->fb,
                       &p->gb, &p->outm, &p->h_in0, &p->h_in1, &p->h_in2, &p->h_in3, &p->h_out0,
                       &p->h_out1, &p->h_out2, &p->h_out3};
@@ -205,9 +210,50 @@ extern "C" int gd_sht_tables(gd_sht_plan *p, double *P, double *dP)
 }
 
 // ---------------------------------------------------------------- transform pipelines
+// ---------------------------------------------------------------- transposes of the transforms
+// On the Gauss grid with real fields f = sum c_m Re(a_lm Y_lm) (c_0 = 1, c_m = 2)
+// the analysis is A = C^-1 S^T W (scalar) and A_v = L^-1 C^-1 S_v^T W (vector,
+// L = l(l+1)), W the quadrature weight 2 pi w_i / nlon of each grid point, S
+// the synthesis.  The transposes therefore run the forward pipelines with
+// the weights moved:
+//   S^T = C A W^-1:   the analysis with unit weights, then x c_m (and the
+//                     vector one without its 1/(l(l+1)) factor);
+//   A^T = W S C^-1:   the synthesis of the columns / c_m (/ l(l+1) for the
+//                     vector one, 0 at l = 0), then x W per grid point.
+namespace {
+template <typename T>
+__global__ void colscale_kernel(T *__restrict__ syn, const int *__restrict__ l_of_idx, const int *__restrict__ m_of_idx,
+                                int nlm, int ll)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x, col = blockIdx.y;
+    if (idx >= nlm) return;
+    const int l = l_of_idx[idx];
+    double f = m_of_idx[idx] ? 0.5 : 1.0;
+    if (ll) f = l ? f / ((double)l * (double)(l + 1)) : 0.0;
+    syn[(size_t)col * nlm + idx] = (T)((double)syn[(size_t)col * nlm + idx] * f);
+}
+
+// grid values (nlat, nlon, nr), colatitude fastest, times their ring's weight
+__global__ void wscale_kernel(double *__restrict__ phys, const double *__restrict__ w, int nlat, size_t n)
+{
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int i = (int)(q % (size_t)nlat), i2 = i < nlat / 2 ? i : nlat - 1 - i;
+    phys[q] *= w[i2];
+}
+
+// m-major planes [plane][idx] times c_m
+__global__ void cm_kernel(double *__restrict__ planes, const int *__restrict__ m_of_idx, int nlm)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nlm) return;
+    if (m_of_idx[idx]) planes[(size_t)blockIdx.y * nlm + idx] *= 2.0;
+}
+} // namespace
+
 template <typename T>
 static int run_synth(gd_sht_plan *p, bool vec, const double *a_re, const double *a_im, const double *b_re,
-                     const double *b_im, int nr, double *o0, double *o1)
+                     const double *b_im, int nr, double *o0, double *o1, bool adj = false)
 {
     hipStream_t st = p->ctx->stream;
     const int nlm = p->nlm, nm = p->mmax + 1;
@@ -222,7 +268,13 @@ static int run_synth(gd_sht_plan *p, bool vec, const double *a_re, const double 
     } else {
         GD_TRY(gd_launch_pack<T>(st, a_re, a_im, p->d_lmaj_of_idx.as<int>(), nlm, nr, cin, 0, syn));
     }
+    if (adj) {
+        hipLaunchKernelGGL((colscale_kernel<T>), dim3((nlm + 255) / 256, nr * cin), dim3(256), 0, st, syn,
+                           p->d_l_of_idx.as<int>(), p->d_m_of_idx.as<int>(), nlm, vec ? 1 : 0);
+        GD_TRY(gd_check_launch("colscale_kernel"));
+    }
     SynthArgs s{};
+    s.mm = p->prec == GD_PREC_BF16X3;
     s.in = syn;
     s.cin = cin;
     s.nf = 1;
@@ -263,12 +315,22 @@ static int run_synth(gd_sht_plan *p, bool vec, const double *a_re, const double 
         r.fs_off[0] = 2;
         r.phys_out[0] = o0;
     }
-    return gd_launch_ring<T>(st, RING_SYNTH, r);
+    GD_TRY(gd_launch_ring<T>(st, RING_SYNTH, r));
+    if (adj) {
+        const size_t n = (size_t)p->nlat * p->nlon * nr;
+        for (double *o : {o0, o1}) {
+            if (!o) continue;
+            hipLaunchKernelGGL(wscale_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, o,
+                               p->wq64.as<double>(), p->nlat, n);
+            GD_TRY(gd_check_launch("wscale_kernel"));
+        }
+    }
+    return GD_OK;
 }
 
 template <typename T>
 static int run_analysis(gd_sht_plan *p, bool vec, const double *f0, const double *f1, int nr, double *o0_re,
-                        double *o0_im, double *o1_re, double *o1_im)
+                        double *o0_im, double *o1_re, double *o1_im, bool adj = false)
 {
     hipStream_t st = p->ctx->stream;
     const int nlm = p->nlm, nm = p->mmax + 1;
@@ -283,8 +345,10 @@ static int run_analysis(gd_sht_plan *p, bool vec, const double *f0, const double
     r.nfield = vec ? 2 : 1;
     r.phys_in[0] = f0;
     r.phys_in[1] = f1;
+    if (adj) r.wq = p->wone.p;
     GD_TRY(gd_launch_ring<T>(st, RING_ANALYSIS, r));
     AnaArgs a{};
+    a.mm = p->prec == GD_PREC_BF16X3;
     a.gb = p->gb.p;
     a.nslot_g = nslot;
     a.gslot0 = 0;
@@ -303,8 +367,13 @@ static int run_analysis(gd_sht_plan *p, bool vec, const double *f0, const double
         a.tab1 = gd_sht_tab(p, 2);
         a.out[0] = om;          // S -> pol
         a.out[1] = om + 2 * fs; // T -> tor
-        a.scale[0] = a.scale[1] = 0;
+        a.scale[0] = a.scale[1] = adj ? 1 : 0;
         GD_TRY(gd_launch_ana<T>(st, a, true, maxgroup));
+        if (adj) {
+            hipLaunchKernelGGL(cm_kernel, dim3((nlm + 255) / 256, 4 * nr), dim3(256), 0, st, om,
+                               p->d_m_of_idx.as<int>(), nlm);
+            GD_TRY(gd_check_launch("cm_kernel"));
+        }
         // o0 = tor, o1 = pol
         GD_TRY(gd_launch_unpack(st, om + 2 * fs, p->d_idx_of_lmaj.as<int>(), nlm, nr, o0_re));
         GD_TRY(gd_launch_unpack(st, om + 3 * fs, p->d_idx_of_lmaj.as<int>(), nlm, nr, o0_im));
@@ -316,6 +385,11 @@ static int run_analysis(gd_sht_plan *p, bool vec, const double *f0, const double
         a.out[0] = om;
         a.scale[0] = 1;
         GD_TRY(gd_launch_ana<T>(st, a, false, maxgroup));
+        if (adj) {
+            hipLaunchKernelGGL(cm_kernel, dim3((nlm + 255) / 256, 2 * nr), dim3(256), 0, st, om,
+                               p->d_m_of_idx.as<int>(), nlm);
+            GD_TRY(gd_check_launch("cm_kernel"));
+        }
         GD_TRY(gd_launch_unpack(st, om, p->d_idx_of_lmaj.as<int>(), nlm, nr, o0_re));
         GD_TRY(gd_launch_unpack(st, om + fs, p->d_idx_of_lmaj.as<int>(), nlm, nr, o0_im));
     }
@@ -432,6 +506,109 @@ extern "C" int gd_sht_vector_analysis(gd_sht_plan *p, const double *vt, const do
         GD_TRY(gd_from_device(p->ctx, tor_im, b, sizeof(double) * ns));
         GD_TRY(gd_from_device(p->ctx, pol_re, c, sizeof(double) * ns));
         GD_TRY(gd_from_device(p->ctx, pol_im, d, sizeof(double) * ns));
+    }
+    return GD_OK;
+}
+
+// ---- transposed transforms (the adjoint of the transform pair)
+extern "C" int gd_sht_scalar_synth_adjoint(gd_sht_plan *p, const double *phys, int nr, double *re, double *im,
+                                           int flags)
+{
+    GD_CHECK(p && phys && re && im && nr >= 1, GD_ERR_ARG, "gd_sht_scalar_synth_adjoint: bad args");
+    GD_HIP(hipSetDevice(p->ctx->device));
+    const size_t ns = (size_t)p->nlm * nr, np = (size_t)p->nlat * p->nlon * nr;
+    int rc = GD_OK;
+    const double *f = stage_in(p, p->h_in0, phys, np, flags, &rc);
+    GD_TRY(rc);
+    double *ore = stage_out(p->h_out0, re, ns, flags, &rc);
+    GD_TRY(rc);
+    double *oim = stage_out(p->h_out1, im, ns, flags, &rc);
+    GD_TRY(rc);
+    if (p->prec == GD_PREC_FP64) GD_TRY(run_analysis<double>(p, false, f, nullptr, nr, ore, oim, nullptr, nullptr, true));
+    else GD_TRY(run_analysis<float>(p, false, f, nullptr, nr, ore, oim, nullptr, nullptr, true));
+    if (flags & GD_HOST) {
+        GD_TRY(gd_from_device(p->ctx, re, ore, sizeof(double) * ns));
+        GD_TRY(gd_from_device(p->ctx, im, oim, sizeof(double) * ns));
+    }
+    return GD_OK;
+}
+
+extern "C" int gd_sht_scalar_analysis_adjoint(gd_sht_plan *p, const double *re, const double *im, int nr,
+                                              double *phys, int flags)
+{
+    GD_CHECK(p && re && im && phys && nr >= 1, GD_ERR_ARG, "gd_sht_scalar_analysis_adjoint: bad args");
+    GD_HIP(hipSetDevice(p->ctx->device));
+    const size_t ns = (size_t)p->nlm * nr, np = (size_t)p->nlat * p->nlon * nr;
+    int rc = GD_OK;
+    const double *dre = stage_in(p, p->h_in0, re, ns, flags, &rc);
+    GD_TRY(rc);
+    const double *dim = stage_in(p, p->h_in1, im, ns, flags, &rc);
+    GD_TRY(rc);
+    double *dph = stage_out(p->h_out0, phys, np, flags, &rc);
+    GD_TRY(rc);
+    if (p->prec == GD_PREC_FP64) GD_TRY(run_synth<double>(p, false, dre, dim, nullptr, nullptr, nr, dph, nullptr, true));
+    else GD_TRY(run_synth<float>(p, false, dre, dim, nullptr, nullptr, nr, dph, nullptr, true));
+    if (flags & GD_HOST) GD_TRY(gd_from_device(p->ctx, phys, dph, sizeof(double) * np));
+    return GD_OK;
+}
+
+extern "C" int gd_sht_vector_synth_adjoint(gd_sht_plan *p, const double *vt, const double *vp, int nr,
+                                           double *tor_re, double *tor_im, double *pol_re, double *pol_im, int flags)
+{
+    GD_CHECK(p && vt && vp && tor_re && tor_im && pol_re && pol_im && nr >= 1, GD_ERR_ARG,
+             "gd_sht_vector_synth_adjoint: bad args");
+    GD_HIP(hipSetDevice(p->ctx->device));
+    const size_t ns = (size_t)p->nlm * nr, np = (size_t)p->nlat * p->nlon * nr;
+    int rc = GD_OK;
+    const double *f0 = stage_in(p, p->h_in0, vt, np, flags, &rc);
+    GD_TRY(rc);
+    const double *f1 = stage_in(p, p->h_in1, vp, np, flags, &rc);
+    GD_TRY(rc);
+    double *a = stage_out(p->h_out0, tor_re, ns, flags, &rc);
+    GD_TRY(rc);
+    double *b = stage_out(p->h_out1, tor_im, ns, flags, &rc);
+    GD_TRY(rc);
+    double *c = stage_out(p->h_out2, pol_re, ns, flags, &rc);
+    GD_TRY(rc);
+    double *d = stage_out(p->h_out3, pol_im, ns, flags, &rc);
+    GD_TRY(rc);
+    if (p->prec == GD_PREC_FP64) GD_TRY(run_analysis<double>(p, true, f0, f1, nr, a, b, c, d, true));
+    else GD_TRY(run_analysis<float>(p, true, f0, f1, nr, a, b, c, d, true));
+    if (flags & GD_HOST) {
+        GD_TRY(gd_from_device(p->ctx, tor_re, a, sizeof(double) * ns));
+        GD_TRY(gd_from_device(p->ctx, tor_im, b, sizeof(double) * ns));
+        GD_TRY(gd_from_device(p->ctx, pol_re, c, sizeof(double) * ns));
+        GD_TRY(gd_from_device(p->ctx, pol_im, d, sizeof(double) * ns));
+    }
+    return GD_OK;
+}
+
+extern "C" int gd_sht_vector_analysis_adjoint(gd_sht_plan *p, const double *tor_re, const double *tor_im,
+                                              const double *pol_re, const double *pol_im, int nr, double *vt,
+                                              double *vp, int flags)
+{
+    GD_CHECK(p && tor_re && tor_im && pol_re && pol_im && vt && vp && nr >= 1, GD_ERR_ARG,
+             "gd_sht_vector_analysis_adjoint: bad args");
+    GD_HIP(hipSetDevice(p->ctx->device));
+    const size_t ns = (size_t)p->nlm * nr, np = (size_t)p->nlat * p->nlon * nr;
+    int rc = GD_OK;
+    const double *a = stage_in(p, p->h_in0, tor_re, ns, flags, &rc);
+    GD_TRY(rc);
+    const double *b = stage_in(p, p->h_in1, tor_im, ns, flags, &rc);
+    GD_TRY(rc);
+    const double *c = stage_in(p, p->h_in2, pol_re, ns, flags, &rc);
+    GD_TRY(rc);
+    const double *d = stage_in(p, p->h_in3, pol_im, ns, flags, &rc);
+    GD_TRY(rc);
+    double *o0 = stage_out(p->h_out0, vt, np, flags, &rc);
+    GD_TRY(rc);
+    double *o1 = stage_out(p->h_out1, vp, np, flags, &rc);
+    GD_TRY(rc);
+    if (p->prec == GD_PREC_FP64) GD_TRY(run_synth<double>(p, true, a, b, c, d, nr, o0, o1, true));
+    else GD_TRY(run_synth<float>(p, true, a, b, c, d, nr, o0, o1, true));
+    if (flags & GD_HOST) {
+        GD_TRY(gd_from_device(p->ctx, vt, o0, sizeof(double) * np));
+        GD_TRY(gd_from_device(p->ctx, vp, o1, sizeof(double) * np));
     }
     return GD_OK;
 }

@@ -10,9 +10,10 @@ struct gd_sht_plan {
     LmMaps maps;
     FftPlan fft;
     std::vector<double> x, w, theta; // host FP64 grid (cos theta, Gauss weights, colatitude)
-    DevBuf d_moff, d_ne, d_l_of_idx, d_lmaj_of_idx, d_idx_of_lmaj, d_x;
+    DevBuf d_moff, d_ne, d_l_of_idx, d_lmaj_of_idx, d_idx_of_lmaj, d_x, d_m_of_idx;
     DevBuf tab64, tab32;             // [3][nlm][nlat2]: P, dP/dtheta, m P / sin
     DevBuf tw, sinth, costh, wq;     // in the plan's arithmetic type
+    DevBuf wone, wq64;               // nlat/2 ones (plan type) and the weights 2 pi w_i / nlon in FP64
     DevBuf syn, fb, gb, outm;        // scratch for the stand-alone transforms
     DevBuf h_in0, h_in1, h_in2, h_in3, h_out0, h_out1, h_out2, h_out3; // GD_HOST staging
 };

@@ -355,7 +355,14 @@ struct Reader {
                     return false;
                 }
                 v.dims.push_back((int)id);
-                v.count *= dims[id].second;
+                // element count bounded by the file size at every factor (no
+                // 64-bit wrap-around from a malformed header)
+                const uint64_t len = dims[id].second;
+                if (len != 0 && v.count > (uint64_t)hdr.size() / len) {
+                    err = "variable " + v.name + " is larger than the file";
+                    return false;
+                }
+                v.count *= len;
             }
             if (!atts(v.atts) || !u32(t) || !u32(vsize)) return false;
             v.type = (int)t;
@@ -365,7 +372,8 @@ struct Reader {
                 v.begin = b;
             } else if (!u64(v.begin))
                 return false;
-            if (!type_size(v.type) || v.begin + v.count * type_size(v.type) > hdr.size()) {
+            const uint64_t ts = type_size(v.type), fsz = hdr.size();
+            if (!ts || v.begin > fsz || v.count > (fsz - v.begin) / ts) {
                 err = "variable " + v.name + " lies outside the file";
                 return false;
             }

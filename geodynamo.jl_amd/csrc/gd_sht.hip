@@ -159,7 +159,7 @@ template <bool VEC, int NF, int NW, int NG = 0> struct SynthTile {
 //     (lane / 16) takes degrees 4g .. 4g+3 over the chunk's 4 MFMA steps (the
 //     same permutation indexes A), so each lane reads 4 consecutive degrees.
 constexpr int TMA = TM + 4;
-template <typename T, bool VEC, int NF, int NW, int KC, bool VL, int NG>
+template <typename T, bool VEC, int NF, int NW, int KC, bool VL, int NG, int MM>
 __device__ __forceinline__ void synth_pipeline(const SynthArgs &a, const T *__restrict__ in, const T *__restrict__ tab0,
                                                const T *__restrict__ tab1, T *sA, const int *scol, int m,
                                                int i0, int r0, acc_t<T> (&accE)[SynthTile<VEC, NF, NW, NG>::NT][2],
@@ -240,6 +240,23 @@ __device__ __forceinline__ void synth_pipeline(const SynthArgs &a, const T *__re
     };
     auto mma = [&](int buf, const T (&bv)[RB], acc_t<T>(&acc)[NT][2]) {
         const T *A = sA + buf * sAsz;
+        if constexpr (MM) {
+            // BF16x3: the 4 steps of each 16-degree group as one split-precision
+            // 16x16x16 BF16 contraction (same k order: lane group g, degrees 4g+j)
+#pragma unroll
+            for (int h = 0; h < KC / 16; ++h) {
+                const Bf16Pair bp = bf16_split4(bv[4 * h], bv[4 * h + 1], bv[4 * h + 2], bv[4 * h + 3]);
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+#pragma unroll
+                    for (int rb_ = 0; rb_ < 2; ++rb_) {
+                        const T *Ar = A + (t * KC + h * 16 + 4 * g) * TMA + rb_ * 16 + c16;
+                        const Bf16Pair ap = bf16_split4(Ar[0], Ar[TMA], Ar[2 * TMA], Ar[3 * TMA]);
+                        acc[t][rb_] = mfma_bf16x3(ap, bp, acc[t][rb_]);
+                    }
+            }
+            return;
+        }
 #pragma unroll
         for (int s = 0; s < RB; ++s) {
             const int k = (s >> 2) * 16 + 4 * g + (s & 3);
@@ -281,9 +298,10 @@ template <typename T, bool VEC, int NF, int NW, int KC, int NG = 0> constexpr si
 // The block's output tile -- TM nodes x LT levels, RS contiguous slots of each
 // (node, level, m) record -- is staged in LDS and stored as 16-byte vectors,
 // one record run after another.
-template <typename T, bool VEC, int NF, int NW, int KC, bool VL, int NG = 0>
+template <typename T, bool VEC, int NF, int NW, int KC, bool VL, int NG = 0, int MM = 0>
 __global__ void __launch_bounds__(64 * NW) synth_kernel(SynthArgs a)
 {
+    static_assert(!MM || sizeof(T) == 4, "the BF16x3 contraction takes FP32 operands");
     using TL = SynthTile<VEC, NF, NW, NG>;
     constexpr int NT = TL::NT, VC = TL::VC, LT = TL::LT, RS = TL::RS;
     extern __shared__ __align__(16) char smem_raw[];
@@ -307,7 +325,7 @@ __global__ void __launch_bounds__(64 * NW) synth_kernel(SynthArgs a)
             accE[t][rb] = acc_t<T>{0, 0, 0, 0};
             accO[t][rb] = acc_t<T>{0, 0, 0, 0};
         }
-    synth_pipeline<T, VEC, NF, NW, KC, VL, NG>(a, in, tab0, tab1, sA, scol, m, i0, r0, accE, accO);
+    synth_pipeline<T, VEC, NF, NW, KC, VL, NG, MM>(a, in, tab0, tab1, sA, scol, m, i0, r0, accE, accO);
     __syncthreads(); // the tile below reuses the pipeline's LDS
 
     T *tile = sA; // [node][level][slot]
@@ -555,9 +573,10 @@ template <typename T, bool VEC, int NF> __global__ void __launch_bounds__(64 * N
 // 48 us: a wave's next-tile operand prefetch then spans fewer tiles; for FP64
 // it would exceed the register budget of a 512-thread block)
 template <typename T> constexpr int ana_waves() { return 4; }
-template <typename T, bool VEC, int NF, int KN>
+template <typename T, bool VEC, int NF, int KN, int MM = 0>
 __global__ void __launch_bounds__(64 * ana_waves<T>()) ana_res_kernel(AnaArgs a)
 {
+    static_assert(!MM || sizeof(T) == 4, "the BF16x3 contraction takes FP32 operands");
     constexpr int ANA_WAVES = ana_waves<T>();
     constexpr int NT = VEC ? 2 : 1;
     constexpr int VC = VEC ? 4 * NF : 2 * NF;
@@ -659,14 +678,34 @@ __global__ void __launch_bounds__(64 * ana_waves<T>()) ana_res_kernel(AnaArgs a)
         acc_t<T> acc[NT][2];
 #pragma unroll
         for (int t = 0; t < NT; ++t) acc[t][0] = acc[t][1] = acc_t<T>{0, 0, 0, 0};
+        if constexpr (MM) {
+            // BF16x3: each 16-node group (4 steps, lane group kq on nodes 4kq+j)
+            // as one split-precision 16x16x16 BF16 contraction
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
+            for (int h = 0; h < KS / 4; ++h) {
 #pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                const T *Ak = sA + (t * KN + node(ks)) * TMA;
-                acc[t][0] = mfma16x16x4(Ak[c], b[t][ks], acc[t][0]);
-                acc[t][1] = mfma16x16x4(Ak[16 + c], b[t][ks], acc[t][1]);
-                b[t][ks] = gload(next, t, ks);
+                for (int t = 0; t < NT; ++t) {
+                    const Bf16Pair bp = bf16_split4(b[t][4 * h], b[t][4 * h + 1], b[t][4 * h + 2], b[t][4 * h + 3]);
+#pragma unroll
+                    for (int rb = 0; rb < 2; ++rb) {
+                        const T *A0 = sA + (t * KN + 16 * h + 4 * kq) * TMA + rb * 16 + c;
+                        const Bf16Pair ap = bf16_split4(A0[0], A0[TMA], A0[2 * TMA], A0[3 * TMA]);
+                        acc[t][rb] = mfma_bf16x3(ap, bp, acc[t][rb]);
+                    }
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) b[t][4 * h + j] = gload(next, t, 4 * h + j);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const T *Ak = sA + (t * KN + node(ks)) * TMA;
+                    acc[t][0] = mfma16x16x4(Ak[c], b[t][ks], acc[t][0]);
+                    acc[t][1] = mfma16x16x4(Ak[16 + c], b[t][ks], acc[t][1]);
+                    b[t][ks] = gload(next, t, ks);
+                }
             }
         }
         if (tile >= ntiles) continue; // uniform over the wave
@@ -758,17 +797,30 @@ template <typename T, bool VEC, int NF, int KN> constexpr size_t ana_res_lds()
 
 } // namespace
 
-template <typename T, bool VEC, int NF, int NW, int KC, int NG = 0>
-static void launch_synth(hipStream_t st, const SynthArgs &a)
+template <typename T, bool VEC, int NF, int NW, int KC, int NG, int MM>
+static void launch_synth_mm(hipStream_t st, const SynthArgs &a)
 {
     using TL = SynthTile<VEC, NF, NW, NG>;
     dim3 grid(((a.nlat2 + TM - 1) / TM) * ((a.nr + TL::LT - 1) / TL::LT) * (a.mmax + 1));
     constexpr size_t lds = synth_lds_bytes<T, VEC, NF, NW, KC, NG>();
     // 16-byte table loads need every node row and tile start 16-byte aligned
     if (a.nlat2 % 4 == 0 && ((uintptr_t)a.tab0 % 16) == 0 && ((uintptr_t)a.tab1 % 16) == 0)
-        hipLaunchKernelGGL((synth_kernel<T, VEC, NF, NW, KC, true, NG>), grid, dim3(64 * NW), lds, st, a);
+        hipLaunchKernelGGL((synth_kernel<T, VEC, NF, NW, KC, true, NG, MM>), grid, dim3(64 * NW), lds, st, a);
     else
-        hipLaunchKernelGGL((synth_kernel<T, VEC, NF, NW, KC, false, NG>), grid, dim3(64 * NW), lds, st, a);
+        hipLaunchKernelGGL((synth_kernel<T, VEC, NF, NW, KC, false, NG, MM>), grid, dim3(64 * NW), lds, st, a);
+}
+
+// a.mm = 1: the BF16x3 contraction (FP32 operands only)
+template <typename T, bool VEC, int NF, int NW, int KC, int NG = 0>
+static void launch_synth(hipStream_t st, const SynthArgs &a)
+{
+    if constexpr (sizeof(T) == 4) {
+        if (a.mm) {
+            launch_synth_mm<T, VEC, NF, NW, KC, NG, 1>(st, a);
+            return;
+        }
+    }
+    launch_synth_mm<T, VEC, NF, NW, KC, NG, 0>(st, a);
 }
 
 // The staged tile is stored in 16-byte vectors: record stride and first slot
@@ -782,6 +834,7 @@ static int synth_check(const SynthArgs &a, const char *what)
 
 template <typename T> int gd_launch_synth_vec(hipStream_t st, const SynthArgs &a)
 {
+    GD_CHECK(!a.mm || sizeof(T) == 4, GD_ERR_ARG, "synth_vec: the BF16x3 contraction takes FP32 operands");
     GD_TRY(synth_check(a, "synth_vec"));
     GD_CHECK(a.slot0 + 8 * (a.nf + a.ng) <= a.nslot, GD_ERR_ARG, "synth_vec: slots %d+%d exceed %d", a.slot0,
              8 * (a.nf + a.ng), a.nslot);
@@ -800,6 +853,7 @@ template <typename T> int gd_launch_synth_vec(hipStream_t st, const SynthArgs &a
 
 template <typename T> int gd_launch_synth_scal(hipStream_t st, const SynthArgs &a)
 {
+    GD_CHECK(!a.mm || sizeof(T) == 4, GD_ERR_ARG, "synth_scal: the BF16x3 contraction takes FP32 operands");
     GD_TRY(synth_check(a, "synth_scal"));
     GD_CHECK(a.slot0 + 4 * a.nf <= a.nslot, GD_ERR_ARG, "synth_scal: slots %d+%d exceed %d", a.slot0, 4 * a.nf, a.nslot);
     if (a.nf == 5) launch_synth<T, false, 5, 5, 16>(st, a);
@@ -813,7 +867,7 @@ template <typename T> int gd_launch_synth_scal(hipStream_t st, const SynthArgs &
 // nodes in chunks of at most KN: the first launch stores, the others add (the
 // sum over nodes is linear; the per-row factors, signs and imag(m=0) := 0
 // apply to every partial alike)
-template <typename T, bool VEC, int NF, int KN> static bool try_ana_res(hipStream_t st, const AnaArgs &a0)
+template <typename T, bool VEC, int NF, int KN, int MM = 0> static bool try_ana_res(hipStream_t st, const AnaArgs &a0)
 {
     constexpr size_t lds = ana_res_lds<T, VEC, NF, KN>();
     if (lds > 64 * 1024) return false;
@@ -824,20 +878,27 @@ template <typename T, bool VEC, int NF, int KN> static bool try_ana_res(hipStrea
         a.k0 = c * KN;
         a.nk = min(KN, a0.nlat2 - a.k0);
         a.accum = c > 0;
-        hipLaunchKernelGGL((ana_res_kernel<T, VEC, NF, KN>), dim3(gx * 2 * (a.mmax + 1)), dim3(64 * ana_waves<T>()),
-                           lds, st, a);
+        hipLaunchKernelGGL((ana_res_kernel<T, VEC, NF, KN, MM>), dim3(gx * 2 * (a.mmax + 1)),
+                           dim3(64 * ana_waves<T>()), lds, st, a);
     }
     return true;
 }
 
+template <typename T, bool VEC, int NF, int MM> static bool ana_res_mm(hipStream_t st, const AnaArgs &a)
+{
+    if (a.nlat2 <= 32) return try_ana_res<T, VEC, NF, 32, MM>(st, a);
+    if (a.nlat2 <= 48) return try_ana_res<T, VEC, NF, 48, MM>(st, a);
+    if (a.nlat2 <= 96) return try_ana_res<T, VEC, NF, 96, MM>(st, a);
+    if constexpr (sizeof(T) == 4) // the FP64 operand registers of a 128-node tile would spill
+        if (a.nlat2 <= 128) return try_ana_res<T, VEC, NF, 128, MM>(st, a);
+    return try_ana_res<T, VEC, NF, 96, MM>(st, a); // node chunks of 96
+}
+
 template <typename T, bool VEC, int NF> static bool ana_res(hipStream_t st, const AnaArgs &a)
 {
-    if (a.nlat2 <= 32) return try_ana_res<T, VEC, NF, 32>(st, a);
-    if (a.nlat2 <= 48) return try_ana_res<T, VEC, NF, 48>(st, a);
-    if (a.nlat2 <= 96) return try_ana_res<T, VEC, NF, 96>(st, a);
-    if constexpr (sizeof(T) == 4) // the FP64 operand registers of a 128-node tile would spill
-        if (a.nlat2 <= 128) return try_ana_res<T, VEC, NF, 128>(st, a);
-    return try_ana_res<T, VEC, NF, 96>(st, a); // node chunks of 96
+    if constexpr (sizeof(T) == 4)
+        if (a.mm) return ana_res_mm<T, VEC, NF, 1>(st, a);
+    return ana_res_mm<T, VEC, NF, 0>(st, a);
 }
 
 template <typename T> static int launch_ana(hipStream_t st, const AnaArgs &a, bool vec, int maxgroup)
@@ -847,6 +908,7 @@ template <typename T> static int launch_ana(hipStream_t st, const AnaArgs &a, bo
     if (!vec && a.nf == 1 && ana_res<T, false, 1>(st, a)) return gd_check_launch("ana_res_kernel<scal>");
     if (!vec && a.nf == 2 && ana_res<T, false, 2>(st, a)) return gd_check_launch("ana_res_kernel<scal>");
     // streaming form (large nlat, or LDS beyond 64 KB)
+    GD_CHECK(!a.mm, GD_ERR_ARG, "ana: the BF16x3 contraction has no streaming form (nlat/2=%d)", a.nlat2);
     const int VC = vec ? 4 * a.nf : 2 * a.nf;
     const int LT = 16 * a.nf / VC;
     dim3 grid(((maxgroup + TM - 1) / TM) * ((a.nr + LT - 1) / LT) * 2 * (a.mmax + 1));
@@ -861,6 +923,7 @@ template <typename T> static int launch_ana(hipStream_t st, const AnaArgs &a, bo
 
 template <typename T> int gd_launch_ana(hipStream_t st, const AnaArgs &a, bool vec, int maxgroup)
 {
+    GD_CHECK(!a.mm || sizeof(T) == 4, GD_ERR_ARG, "ana: the BF16x3 contraction takes FP32 operands");
     GD_CHECK(a.nf >= 1 && a.nf <= (vec ? 2 : 5), GD_ERR_ARG, "ana: nf=%d", a.nf);
     AnaArgs b = a;
     b.maxgroup = maxgroup;
@@ -1132,11 +1195,13 @@ template <typename T, int MODE, int NC> __global__ void __launch_bounds__(256) r
     }
     __syncthreads();
     // fields in / products out: compile-time for the step modes
-    constexpr int NFC = MODE == RING_STEP ? 15 : MODE == RING_STEP_TANG ? 4 : MODE == RING_STEP_TANG_C ? 6 : 0;
-    constexpr int NPC = MODE == RING_STEP ? 5 : MODE == RING_STEP_TANG ? 3 : MODE == RING_STEP_TANG_C ? 4 : 0;
+    constexpr int NFC = MODE == RING_STEP ? 15 : MODE == RING_STEP_TANG ? 4 : MODE == RING_STEP_TANG_C ? 6
+                      : MODE == RING_ADJ_TANG ? 7 : 0;
+    constexpr int NPC = MODE == RING_STEP ? 5 : MODE == RING_STEP_TANG ? 3 : MODE == RING_STEP_TANG_C ? 4
+                      : MODE == RING_ADJ_TANG ? 4 : 0;
     const int NF = NFC ? NFC : a.nfield;
 
-    if (MODE == RING_SYNTH || MODE == RING_STEP || MODE == RING_STEP_TANG || MODE == RING_STEP_TANG_C) {
+    if (MODE != RING_ANALYSIS) {
         // Hermitian spectra of the north (X) and south (Y) rings packed as Z = X + iY
         const T *src = static_cast<const T *>(a.fb) + ((size_t)i * a.nr + r) * nm * a.nslot_f;
         // fixed trip count, fully unrolled: every thread's global loads are in
@@ -1226,6 +1291,38 @@ template <typename T, int MODE, int NC> __global__ void __launch_bounds__(256) r
                     const T aC = -(ut * ct + up * cp);
                     if (h) o[3].y = aC;
                     else o[3].x = aC;
+                }
+            }
+#pragma unroll
+            for (int P = 0; P < NPC; ++P) buf[P * ld + sk(j)] = o[P];
+        }
+        __syncthreads();
+    } else if constexpr (MODE == RING_ADJ_TANG) {
+        // transposed Jacobian of the products above at (u, grad T): with
+        // a_t = ro cth u_p, a_p = -ro cth u_t, a_T = -(u_t g_t + u_p g_p),
+        //   u_t* = -ro cth a_p* - g_t a_T*,  u_p* = ro cth a_t* - g_p a_T*,
+        //   g_t* = -u_t a_T*,                g_p* = -u_p a_T*
+        // fields: 0 u_t 1 u_p 2 a_t* 3 a_p* 4 g_t 5 g_p 6 a_T*
+        NP = NPC;
+        const T ro = (T)a.ro;
+        const T *costh = static_cast<const T *>(a.costh);
+        for (int j = tid; j < n; j += nthr) {
+            C2<T> f[NFC];
+#pragma unroll
+            for (int F = 0; F < NFC; ++F) f[F] = buf[F * ld + sk(j)];
+            C2<T> o[NPC];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int ti = a.coriolis_index ? (h ? is : i) : (j < a.nlat - 1 ? j : a.nlat - 1);
+                const T rc = ro * costh[ti];
+#define FV(F) (h ? f[F].y : f[F].x)
+                const T ut = FV(0), up = FV(1), bt = FV(2), bp = FV(3), gt = FV(4), gp = FV(5), bT = FV(6);
+#undef FV
+                const T vals[NPC] = {-rc * bp - gt * bT, rc * bt - gp * bT, -ut * bT, -up * bT};
+#pragma unroll
+                for (int P = 0; P < NPC; ++P) {
+                    if (h) o[P].y = vals[P];
+                    else o[P].x = vals[P];
                 }
             }
 #pragma unroll
@@ -1339,7 +1436,8 @@ template <typename T, int MODE> static void ring_dispatch(hipStream_t st, dim3 g
 template <typename T> int gd_launch_ring(hipStream_t st, int mode, const RingArgs &a)
 {
     const int nthr = RING_THREADS;
-    int nf = (mode == RING_STEP) ? 15 : (mode == RING_STEP_TANG) ? 4 : (mode == RING_STEP_TANG_C) ? 6 : a.nfield;
+    int nf = (mode == RING_STEP) ? 15 : (mode == RING_STEP_TANG) ? 4 : (mode == RING_STEP_TANG_C) ? 6
+           : (mode == RING_ADJ_TANG) ? 7 : a.nfield;
     GD_CHECK(nf >= 1 && nf <= 15, GD_ERR_ARG, "ring: nfield=%d", nf);
     GD_CHECK((long)nf * (a.mmax + 1) <= 8L * nthr, GD_ERR_ARG, "ring: %d fields x %d orders exceed one load batch",
              nf, a.mmax + 1);
@@ -1356,6 +1454,7 @@ template <typename T> int gd_launch_ring(hipStream_t st, int mode, const RingArg
     case RING_ANALYSIS: ring_dispatch<T, RING_ANALYSIS>(st, grid, lds, a); break;
     case RING_STEP_TANG: ring_dispatch<T, RING_STEP_TANG>(st, grid, lds, a); break;
     case RING_STEP_TANG_C: ring_dispatch<T, RING_STEP_TANG_C>(st, grid, lds, a); break;
+    case RING_ADJ_TANG: ring_dispatch<T, RING_ADJ_TANG>(st, grid, lds, a); break;
     default: ring_dispatch<T, RING_STEP>(st, grid, lds, a); break;
     }
     return gd_check_launch("ring_kernel");

@@ -29,6 +29,7 @@ struct SynthArgs {
     int nslot, slot0;     // vector field f -> slot0 + 8f + 4h + (2c + reim); scalar -> slot0 + 4f + 2h + reim
     int ng;               // vector: gradient fields (S only, T = 0) after the nf full ones; colmap
                           // then lists 4 nf + 2 ng columns (S_re S_im per gradient field)
+    int mm;               // 1: BF16x3 split-precision contraction (FP32 operands), 0: native MFMA
 };
 
 struct AnaArgs {
@@ -45,6 +46,7 @@ struct AnaArgs {
     // table-resident analysis in node chunks (set by its launcher): nodes
     // k0 .. k0 + nk - 1 (nk = 0: all), accum = 1 adds to out instead of storing
     int k0, nk, accum;
+    int mm;               // 1: BF16x3 split-precision contraction (FP32 operands), 0: native MFMA
 };
 
 struct RingArgs {
@@ -69,7 +71,13 @@ struct RingArgs {
 // (grad T)_phi), 3 products out (the force and -u.grad T; u x B vanishes).
 // RING_STEP_TANG_C: the same with the composition's gradient as fields 4, 5
 // and -u.grad C as a fourth product.
-enum { RING_SYNTH = 0, RING_ANALYSIS = 1, RING_STEP = 2, RING_STEP_TANG = 3, RING_STEP_TANG_C = 4 };
+// RING_ADJ_TANG: the transposed Jacobian of RING_STEP_TANG's products at the
+// state (u, grad T) applied to the cotangents of its products -- 7 fields in
+// (u_theta, u_phi, cotangent force theta / phi, (grad T)_theta, (grad T)_phi,
+// cotangent of -u.grad T), 4 out (cotangents of u_theta, u_phi,
+// (grad T)_theta, (grad T)_phi).
+enum { RING_SYNTH = 0, RING_ANALYSIS = 1, RING_STEP = 2, RING_STEP_TANG = 3, RING_STEP_TANG_C = 4,
+       RING_ADJ_TANG = 5 };
 
 int gd_launch_plm(hipStream_t st, const double *x, const int *moff, int lmax, int mmax, int nlat2, double *tab64,
                   float *tab32, int nlm);

@@ -24,63 +24,8 @@
 #include <cmath>
 #include <cstdlib>
 
-#include "gd_plan.hpp"
-#include "gd_radial.hpp"
+#include "gd_step.hpp"
 
-struct gd_step {
-    gd_ctx *ctx = nullptr;
-    gd_step_desc d{};
-    gd_sht_plan sht;
-    int nlm = 0, nl = 0, W = 0;
-    size_t fs = 0;                 // nlm * nr
-    DevBuf state[2], nlbuf[2];     // m-major FP64 [field][re/im][r][idx]; step k reads state[k&1]
-    DevBuf Lt, LUt;                // [nu][(col*W+row)][l], nu = velocity, magnetic, temperature
-    DevBuf nuLap, nus;             // [nu][W*nr] nu * Laplacian band; the three diffusivities
-    DevBuf D1, D2, rtab;
-    DevBuf synV, synS, fb, gb;
-    DevBuf part, eout, scratch;
-    DevBuf status;
-    DevBuf zrow;                   // nlm zeros
-    DevBuf rdiag;                  // [nu][row][l] reciprocal U diagonals
-    int step = 0;
-    bool chain = false;  // the column-register CNAB2 kernel applies (it also refreshes synV/synS)
-    bool syn_ok = false; // synV/synS hold the synthesis columns of the current state
-    hipGraph_t graph[2] = {nullptr, nullptr};
-    hipGraphExec_t gexec[2] = {nullptr, nullptr};
-    // second stream of the tangential step (default priority): the magnetic
-    // solves, then the scalar analysis and the temperature solve, run beside
-    // the synthesis / ring / vector analysis / velocity solves
-    hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_ring = nullptr, ev_join = nullptr;
-    // 0: one stream; 1: magnetic and temperature solves and the scalar analysis on
-    // the side stream; 2: the scalar analysis only.  Measured at lmax 127, nr 65:
-    // 2 is fastest on the FP32 path (4848 vs 4638 steps/s), 0 on the FP64 path
-    // (2807 vs 2721); 1 is slower on both (three partial CNAB2 launches)
-    int overlap = 0;
-    // per-mode temperature conditions (gd_step_set_temperature_bc); null: the
-    // boundary rows of T are read as 0 (thermal.jl defaults: Dirichlet, value 0)
-    gd_scalar_bc *tbc = nullptr;
-    // synthesis of the tangential step with grad T (and grad C) as gradient
-    // fields (1) or as full fields with zero toroidal columns (0, same values)
-    int synth_grad = 1;
-    // CNAB2 column kernels with 64-bit offsets where 32 bits suffice (1; the
-    // form large shells use, the same bits) or as sized (0)
-    int chain_off64 = 0;
-    // twisted column solve (gd_chain_twist_tables): tables for the four
-    // diffusivity slots; twist = 0 keeps one lane per column
-    DevBuf ULt, rdiag2, Rint;
-    int twist = 0;
-    // optional passive composition (gd_step_enable_composition, tangential
-    // step): state and nonlinear history in the planes 8, 9 (field 4) of
-    // five-field buffers whose other planes stay zero, so the CNAB2 kernel
-    // treats it as a temperature-like field with diffusivity slot 3 (Pm/Sc);
-    // its own synthesis columns
-    bool comp = false;
-    double Sc = 1.0;
-    DevBuf cstate[2], cnl[2];
-    gd_scalar_bc *cbc = nullptr;
-    bool csyn_ok = false;
-};
 
 // ---------------------------------------------------------------- spectral preparation
 // One thread per (idx, level).  synV columns per level (c = 2k + re/im):
@@ -226,6 +171,7 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEven
     GD_MARK(1);
 
     SynthArgs a{};
+    a.mm = p->prec == GD_PREC_BF16X3;
     a.moff = p->d_moff.as<int>();
     a.ne = p->d_ne.as<int>();
     a.nlat2 = p->nlat2;
@@ -245,6 +191,7 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEven
     r.ro = s->d.Pm / s->d.E;
     r.coriolis_index = s->d.coriolis_index;
     AnaArgs g{};
+    g.mm = p->prec == GD_PREC_BF16X3;
     g.gb = s->gb.p;
     g.moff = p->d_moff.as<int>();
     g.ne = p->d_ne.as<int>();
@@ -533,11 +480,17 @@ static void destroy_graphs(gd_step *s)
         s->gexec[k] = nullptr;
         s->graph[k] = nullptr;
     }
+    ++s->gen;
 }
+
+int gd_step_enqueue(gd_step *s, int parity, bool boot) { return enqueue_step(s, parity, boot); }
+bool gd_step_tangential(const gd_step *s) { return tangential(s); }
+void gd_step_drop_graphs(gd_step *s) { destroy_graphs(s); }
 
 static int step_release(gd_step *s)
 {
     destroy_graphs(s);
+    gd_adj_work_release(s);
     DevBuf *bufs[] = {&s->state[0], &s->state[1], &s->nlbuf[0], &s->nlbuf[1], &s->Lt, &s->LUt, &s->ULt, &s->rdiag2, &s->Rint, &s->nuLap, &s->nus,
                       &s->D1, &s->D2, &s->rtab, &s->synV,
                       &s->synS, &s->fb, &s->gb, &s->part, &s->eout, &s->scratch, &s->status, &s->zrow, &s->rdiag};
@@ -589,7 +542,7 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
     s->d = *d;
     GD_CHECK(d->nr >= 2 && d->bw >= 1 && d->bw <= 7 && d->dt > 0 && d->E != 0.0 && d->Pr != 0.0, GD_ERR_ARG,
              "gd_step_create: bad parameters");
-    GD_CHECK(d->prec == GD_PREC_FP64 || d->prec == GD_PREC_FP32, GD_ERR_ARG,
+    GD_CHECK(d->prec == GD_PREC_FP64 || d->prec == GD_PREC_FP32 || d->prec == GD_PREC_BF16X3, GD_ERR_ARG,
              "gd_step_create: precision %d not available", d->prec);
     GD_HIP(hipSetDevice(ctx->device));
     GD_TRY(gd_sht_plan_build(ctx, d->lmax, d->mmax, d->nlat, d->nlon, d->prec, &s->sht));
@@ -634,7 +587,7 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
     // lmax=127, nr=65, bw=3: 68 vs 73 us; on the FP64 path 87 vs 83 us, so
     // FP64 keeps one lane per column); GD_CHAIN_TWIST = 0 keeps one lane here too
     const char *tw = getenv("GD_CHAIN_TWIST");
-    s->twist = (!(tw && tw[0] == '0') && d->prec == GD_PREC_FP32 && bw <= 3 && nr >= 2 * bw + 2 &&
+    s->twist = (!(tw && tw[0] == '0') && d->prec != GD_PREC_FP64 && bw <= 3 && nr >= 2 * bw + 2 &&
                 gd_cnab2_chain_supported(nr, bw)) ? 1 : 0;
     if (s->twist && rc == GD_OK) {
         rc = s->ULt.ensure(sizeof(double) * nb * 4);
@@ -688,7 +641,7 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
     for (hipEvent_t *e : {&s->ev_fork, &s->ev_ring, &s->ev_join})
         GD_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
     // GD_STEP_OVERLAP = 0 / 1 / 2 overrides the stream layout (measurement aid)
-    s->overlap = d->prec == GD_PREC_FP32 ? 2 : 0;
+    s->overlap = d->prec != GD_PREC_FP64 ? 2 : 0;
     const char *ov = getenv("GD_STEP_OVERLAP");
     if (ov && ov[0] >= '0' && ov[0] <= '2') s->overlap = ov[0] - '0';
     // GD_SYNTH_GRAD = 0 contracts the zero toroidal columns of the gradient
@@ -827,8 +780,17 @@ extern "C" int gd_step_restore(gd_step *s, const double *re, const double *im, c
                                   s->ctx->stream));
         }
     }
+    if (s->comp && ((s->step ^ step) & 1)) {
+        // the composition state moves with the step parity (as in
+        // gd_step_set_state); gd_step_restore_composition normally follows
+        const size_t pb = sizeof(double) * s->fs * 2;
+        GD_HIP(hipMemcpyAsync(s->cstate[step & 1].as<double>() + 8 * s->fs,
+                              s->cstate[s->step & 1].as<double>() + 8 * s->fs, pb, hipMemcpyDeviceToDevice,
+                              s->ctx->stream));
+    }
     s->step = step;
     s->syn_ok = false;
+    s->csyn_ok = false;
     GD_TRY(convert_state(s, dev[0], dev[1], cur_state(s), true));
     if (step > 0) GD_TRY(convert_state(s, dev[2], dev[3], history(s), true));
     if (flags & GD_HOST) GD_HIP(hipStreamSynchronize(s->ctx->stream));
@@ -1185,14 +1147,34 @@ struct gd_batch {
     hipEvent_t fork = nullptr;
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    // each member's generation when the graphs were captured: a member whose
+    // captured work changed since (conditions set or dropped, composition
+    // enabled, ...) makes the batch recapture before its next replay
+    std::vector<unsigned> gens[2];
 };
 
-static void batch_release(gd_batch *b)
+static void batch_drop_graphs(gd_batch *b)
 {
     for (int k = 0; k < 2; ++k) {
         if (b->gexec[k]) (void)hipGraphExecDestroy(b->gexec[k]);
         if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
+        b->gexec[k] = nullptr;
+        b->graph[k] = nullptr;
+        b->gens[k].clear();
     }
+}
+
+static bool batch_stale(const gd_batch *b, int par)
+{
+    if (b->gens[par].size() != b->steps.size()) return true;
+    for (size_t i = 0; i < b->steps.size(); ++i)
+        if (b->steps[i]->gen != b->gens[par][i]) return true;
+    return false;
+}
+
+static void batch_release(gd_batch *b)
+{
+    batch_drop_graphs(b);
     for (hipEvent_t e : b->joins)
         if (e) (void)hipEventDestroy(e);
     if (b->fork) (void)hipEventDestroy(b->fork);
@@ -1288,6 +1270,13 @@ extern "C" int gd_batch_run(gd_batch *b, int nsteps)
         } else {
             const int par = step & 1;
             for (gd_step *s : b->steps) GD_TRY(ensure_syn(s));
+            if (b->gexec[par] && batch_stale(b, par)) {
+                // a member changed its captured work: the graph launches of
+                // both parities may hold stale arguments (freed condition
+                // tables, missing composition work)
+                GD_HIP(hipStreamSynchronize(b->ctx->stream));
+                batch_drop_graphs(b);
+            }
             if (!b->gexec[par]) {
                 hipStream_t main = b->ctx->stream;
                 GD_HIP(hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal));
@@ -1301,6 +1290,8 @@ extern "C" int gd_batch_run(gd_batch *b, int nsteps)
                 }
                 b->graph[par] = g;
                 GD_HIP(hipGraphInstantiate(&b->gexec[par], g, nullptr, nullptr, 0));
+                b->gens[par].clear();
+                for (gd_step *s : b->steps) b->gens[par].push_back(s->gen);
             }
             GD_HIP(hipGraphLaunch(b->gexec[par], b->ctx->stream));
         }

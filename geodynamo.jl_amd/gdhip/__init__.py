@@ -13,8 +13,10 @@ from .linear_algebra import (BandedLU, BandedMatrix, apply_banded_matrix, apply_
                              create_derivative_matrix, create_radial_laplacian, factorize_banded,
                              solve_banded)
 from .shtnskit_field_functions import (batch_shtnskit_transforms, batch_spectral_to_physical,  # noqa: F401
-                                       shtnskit_physical_to_spectral, shtnskit_spectral_to_physical,
-                                       shtnskit_vector_analysis, shtnskit_vector_synthesis)
+                                       shtnskit_physical_to_spectral, shtnskit_physical_to_spectral_adjoint,
+                                       shtnskit_spectral_to_physical, shtnskit_spectral_to_physical_adjoint,
+                                       shtnskit_vector_analysis, shtnskit_vector_analysis_adjoint,
+                                       shtnskit_vector_synthesis, shtnskit_vector_synthesis_adjoint)
 from .shtnskit_transforms import SHTnsKitConfig, create_shtnskit_config  # noqa: F401
 from .step import CNAB2Step, StepBatch  # noqa: F401
 from .timestep import (ERK2Cache, ERK2FieldBuffers, ETDCache, SHTnsImplicitMatrices,  # noqa: F401
@@ -27,3 +29,4 @@ from .outputs_writer import (RANK_TIME, TIME_RANK, OutputConfig, TimeTracker, fi
                              write_restart_arrays)
 from .scalar_field_common import (ScalarBoundaryConditions, apply_composition_boundary_conditions_spectral,  # noqa: F401
                                   apply_scalar_flux_bc_spectral, apply_temperature_boundary_conditions_spectral)
+from .adjoint import CNAB2Adjoint, nonlinear_adjoint, step_vjp, surface_level  # noqa: F401

@@ -22,26 +22,33 @@ def _is_torch(a):
     return type(a).__module__.startswith("torch")
 
 
-def prepare(*arrays, device=0):
+def prepare(*arrays, device=0, shapes=None):
     """Return (ctx, flags, [pointers]) for arrays that are all host numpy
-    float64 C-contiguous, or all torch float64 contiguous device tensors."""
+    float64 C-contiguous, or all torch float64 contiguous device tensors.
+    shapes (optional): the exact shape each array must have (None: any), checked
+    before any pointer reaches the library, which reads and writes exactly
+    that many values."""
     kinds = {_is_torch(a) for a in arrays if a is not None}
     if len(kinds) > 1:
         raise TypeError("mix of host and device arrays in one call")
     on_device = kinds == {True}
     ptrs = []
-    for a in arrays:
+    for k, a in enumerate(arrays):
         if a is None:
             ptrs.append(None)
             continue
         if on_device:
-            if not a.is_cuda or not a.is_contiguous() or a.dtype.itemsize != 8:
-                raise TypeError("device arrays must be contiguous 8-byte torch tensors on the GPU")
+            import torch
+
+            if not a.is_cuda or not a.is_contiguous() or a.dtype != torch.float64:
+                raise TypeError("device arrays must be contiguous float64 torch tensors on the GPU")
             ptrs.append(ctypes.c_void_p(a.data_ptr()))
         else:
-            if not isinstance(a, np.ndarray) or not a.flags.c_contiguous or a.dtype.itemsize != 8:
-                raise TypeError("host arrays must be C-contiguous 8-byte numpy arrays")
+            if not isinstance(a, np.ndarray) or not a.flags.c_contiguous or a.dtype != np.float64:
+                raise TypeError("host arrays must be C-contiguous float64 numpy arrays")
             ptrs.append(a.ctypes.data_as(ctypes.c_void_p))
+        if shapes is not None and shapes[k] is not None and tuple(a.shape) != tuple(shapes[k]):
+            raise ValueError("array %d has shape %s, the call needs %s" % (k, tuple(a.shape), tuple(shapes[k])))
     if on_device:
         import torch
 

@@ -11,7 +11,7 @@ INCLUDE = os.path.join(ROOT, "include")
 OBJDIR = os.path.join(os.path.dirname(PKG), "_build")
 LIB = os.path.join(PKG, "libgdhip.so")
 SOURCES = ["gd_chain.hip", "gd_core.hip", "gd_etd.hip", "gd_radial.hip", "gd_sht.hip", "gd_plan.hip", "gd_step.hip",
-           "gd_restart.hip", "gd_bc.hip"]
+           "gd_restart.hip", "gd_bc.hip", "gd_adjoint.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GDHIP_ARCH", "gfx950")
 # Packed FP32 instructions are disabled for all device code.

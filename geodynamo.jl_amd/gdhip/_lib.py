@@ -39,6 +39,10 @@ EXPORTS = [
     "gd_restart_write", "gd_restart_read", "gd_restart_write_arrays", "gd_restart_read_arrays",
     "gd_restart_filename",
     "gd_scalar_bc_create", "gd_scalar_bc_set", "gd_scalar_bc_apply", "gd_scalar_bc_tau", "gd_scalar_bc_destroy",
+    "gd_step_nonlinear_adjoint", "gd_step_vjp", "gd_adjoint_create", "gd_adjoint_destroy", "gd_adjoint_forward",
+    "gd_adjoint_misfit", "gd_adjoint_seed", "gd_adjoint_backward",
+    "gd_sht_scalar_synth_adjoint", "gd_sht_scalar_analysis_adjoint", "gd_sht_vector_synth_adjoint",
+    "gd_sht_vector_analysis_adjoint",
 ]
 
 
@@ -169,6 +173,18 @@ def lib():
             "gd_scalar_bc_apply": (i, [vp, vp, vp, i, i]),
             "gd_scalar_bc_tau": (i, [vp, vp, vp]),
             "gd_scalar_bc_destroy": (i, [vp]),
+            "gd_step_nonlinear_adjoint": (i, [vp] * 5 + [i]),
+            "gd_step_vjp": (i, [vp] * 9 + [i]),
+            "gd_adjoint_create": (i, [vp, i, i, pp]),
+            "gd_adjoint_destroy": (i, [vp]),
+            "gd_adjoint_forward": (i, [vp]),
+            "gd_adjoint_misfit": (i, [vp, i, i, vp, ctypes.POINTER(d), i]),
+            "gd_adjoint_seed": (i, [vp, vp, vp, i]),
+            "gd_adjoint_backward": (i, [vp, vp, vp, i]),
+            "gd_sht_scalar_synth_adjoint": (i, [vp, vp, i, vp, vp, i]),
+            "gd_sht_scalar_analysis_adjoint": (i, [vp, vp, vp, i, vp, i]),
+            "gd_sht_vector_synth_adjoint": (i, [vp, vp, vp, i, vp, vp, vp, vp, i]),
+            "gd_sht_vector_analysis_adjoint": (i, [vp, vp, vp, vp, vp, i, vp, vp, i]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

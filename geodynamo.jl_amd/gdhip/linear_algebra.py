@@ -39,8 +39,10 @@ def create_derivative_matrix(order, domain, bw=DEFAULT_BW, literal=False):
     V \\ rhs solve; the default solves V^T w = k! e_k (exact on polynomials)."""
     N = domain.N
     band = np.zeros((N, 2 * bw + 1))
-    check(lib().gd_derivative_matrix(order, N, bw, np.ascontiguousarray(domain.r).ctypes.data,
-                                     _fd_mode(literal), band.ctypes.data))
+    r = np.ascontiguousarray(domain.r, dtype=np.float64)  # kept alive across the call
+    if r.shape != (7, N):
+        raise ValueError("domain.r must be (7, N) = (7, %d), got %s" % (N, r.shape))
+    check(lib().gd_derivative_matrix(order, N, bw, r.ctypes.data, _fd_mode(literal), band.ctypes.data))
     return BandedMatrix(band, bw, N)
 
 
@@ -48,8 +50,10 @@ def create_radial_laplacian(domain, bw=DEFAULT_BW, literal=False):
     """linear_algebra.jl:134-151: d2/dr2 + (2/r) d/dr."""
     N = domain.N
     band = np.zeros((N, 2 * bw + 1))
-    check(lib().gd_radial_laplacian(N, bw, np.ascontiguousarray(domain.r).ctypes.data,
-                                    _fd_mode(literal), band.ctypes.data))
+    r = np.ascontiguousarray(domain.r, dtype=np.float64)  # kept alive across the call
+    if r.shape != (7, N):
+        raise ValueError("domain.r must be (7, N) = (7, %d), got %s" % (N, r.shape))
+    check(lib().gd_radial_laplacian(N, bw, r.ctypes.data, _fd_mode(literal), band.ctypes.data))
     return BandedMatrix(band, bw, N)
 
 

@@ -33,7 +33,7 @@ class CNAB2Step:
         return (5, self.nr, self.nlm)
 
     def set_state(self, re, im):
-        ctx, flags, (a, b) = prepare(re, im, device=self.device)
+        ctx, flags, (a, b) = prepare(re, im, device=self.device, shapes=[self.shape] * 2)
         check(lib().gd_step_set_state(self._h, a, b, flags))
 
     def set_state_complex(self, st):
@@ -42,7 +42,7 @@ class CNAB2Step:
     def get_state(self, re=None, im=None):
         if re is None:
             re, im = np.zeros(self.shape), np.zeros(self.shape)
-        ctx, flags, (a, b) = prepare(re, im, device=self.device)
+        ctx, flags, (a, b) = prepare(re, im, device=self.device, shapes=[self.shape] * 2)
         check(lib().gd_step_get_state(self._h, a, b, flags))
         return re, im
 
@@ -66,9 +66,9 @@ class CNAB2Step:
             if history is None:
                 raise ValueError("step > 0 needs the nonlinear history")
             hre, him = np.ascontiguousarray(history.real), np.ascontiguousarray(history.imag)
-            ctx, flags, (a, b, c, d) = prepare(re, im, hre, him, device=self.device)
+            ctx, flags, (a, b, c, d) = prepare(re, im, hre, him, device=self.device, shapes=[self.shape] * 4)
         else:
-            ctx, flags, (a, b) = prepare(re, im, device=self.device)
+            ctx, flags, (a, b) = prepare(re, im, device=self.device, shapes=[self.shape] * 2)
             c = d = None
         check(lib().gd_step_restore(self._h, a, b, c, d, int(step), flags))
 
@@ -94,7 +94,7 @@ class CNAB2Step:
     def set_composition(self, C):
         """C: complex (nr, nlm), l-major; call after set_state."""
         re, im = np.ascontiguousarray(C.real), np.ascontiguousarray(C.imag)
-        ctx, flags, (a, b) = prepare(re, im, device=self.device)
+        ctx, flags, (a, b) = prepare(re, im, device=self.device, shapes=[(self.nr, self.nlm)] * 2)
         check(lib().gd_step_set_composition(self._h, a, b, flags))
 
     def get_composition(self):
@@ -114,9 +114,10 @@ class CNAB2Step:
         re, im = np.ascontiguousarray(C.real), np.ascontiguousarray(C.imag)
         if history is not None:
             hre, him = np.ascontiguousarray(history.real), np.ascontiguousarray(history.imag)
-            ctx, flags, (a, b, c, d) = prepare(re, im, hre, him, device=self.device)
+            ctx, flags, (a, b, c, d) = prepare(re, im, hre, him, device=self.device,
+                                               shapes=[(self.nr, self.nlm)] * 4)
         else:
-            ctx, flags, (a, b) = prepare(re, im, device=self.device)
+            ctx, flags, (a, b) = prepare(re, im, device=self.device, shapes=[(self.nr, self.nlm)] * 2)
             c = d = None
         check(lib().gd_step_restore_composition(self._h, a, b, c, d, flags))
 

@@ -380,4 +380,95 @@ function read_restart!(s::HIPStep, filename::String)
             current_step=Int(ccall((:gd_step_count, libgdhip), Cint, (Ptr{Cvoid},), s.ptr)))
 end
 
+# ---- reverse mode: the step's adjoint behind ChainRulesCore rrules (the hook
+# the reference's transforms take through SHTnsKit's AD extension,
+# Manifest.toml:657-669).  ChainRulesCore is a dependency of this module.
+import ChainRulesCore
+using ChainRulesCore: NoTangent, unthunk
+
+mutable struct HIPAdjoint
+    ptr::Ptr{Cvoid}
+    step::HIPStep
+end
+
+"The checkpointed adjoint of `nsteps` steps of `s` (checkpoints every `every` steps)."
+function create_adjoint(s::HIPStep, nsteps::Integer; every::Integer=5)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:gd_adjoint_create, libgdhip), Cint, (Ptr{Cvoid}, Cint, Cint, Ref{Ptr{Cvoid}}),
+                s.ptr, nsteps, every, h))
+    a = HIPAdjoint(h[], s)
+    finalizer(q -> ccall((:gd_adjoint_destroy, libgdhip), Cint, (Ptr{Cvoid},), q.ptr), a)
+    return a
+end
+
+adjoint_forward!(a::HIPAdjoint) = check(ccall((:gd_adjoint_forward, libgdhip), Cint, (Ptr{Cvoid},), a.ptr))
+adjoint_seed!(a::HIPAdjoint, re, im) = GC.@preserve re im check(ccall((:gd_adjoint_seed, libgdhip), Cint,
+    (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint), a.ptr, re, im, GD_HOST))
+function adjoint_backward!(a::HIPAdjoint, re, im)
+    GC.@preserve re im check(ccall((:gd_adjoint_backward, libgdhip), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint), a.ptr, re, im, GD_HOST))
+    return re, im
+end
+
+const MISFIT_KIND = Dict(:B_r => Cint(0), :u_r => Cint(1), :T => Cint(2))
+
+"""run_steps(s, re, im, nsteps) -> (re, im) of the state after `nsteps` CNAB2
+steps from the state (re, im) (5 fields, l-major (nlm, nr) planes each)."""
+function run_steps(s::HIPStep, re::Array{Float64}, im::Array{Float64}, nsteps::Integer)
+    set_state!(s, re, im)
+    check(ccall((:gd_step_run, libgdhip), Cint, (Ptr{Cvoid}, Cint), s.ptr, nsteps))
+    ore, oim = similar(re), similar(im)
+    get_state!(s, ore, oim)
+    return ore, oim
+end
+
+function ChainRulesCore.rrule(::typeof(run_steps), s::HIPStep, re::Array{Float64}, im::Array{Float64},
+                              nsteps::Integer; every::Integer=5)
+    a = create_adjoint(s, nsteps; every=every)
+    set_state!(s, re, im)
+    adjoint_forward!(a)
+    ore, oim = similar(re), similar(im)
+    get_state!(s, ore, oim)
+    function run_steps_pullback(dy)
+        dre, dim = unthunk(dy)
+        adjoint_seed!(a, collect(Float64, unthunk(dre)), collect(Float64, unthunk(dim)))
+        gre, gim = adjoint_backward!(a, similar(re), similar(im))
+        return (NoTangent(), NoTangent(), gre, gim, NoTangent())
+    end
+    return (ore, oim), run_steps_pullback
+end
+
+"""surface_misfit(s, re, im, nsteps, target; kind=:B_r, level) = 1/2 sum (X - target)^2
+over the (nlat, nlon) grid, X the field `kind` (B_r = l(l+1) B_pol / r^2,
+magnetic.jl:257-259; u_r; T) at radial level `level` (0-based) after `nsteps`
+steps from (re, im)."""
+function surface_misfit(s::HIPStep, re::Array{Float64}, im::Array{Float64}, nsteps::Integer,
+                        target::Matrix{Float64}; kind::Symbol=:B_r, level::Integer, every::Integer=5)
+    J, _ = _misfit_forward(s, re, im, nsteps, target, kind, level, every)
+    return J
+end
+
+function _misfit_forward(s, re, im, nsteps, target, kind, level, every)
+    a = create_adjoint(s, nsteps; every=every)
+    set_state!(s, re, im)
+    adjoint_forward!(a)
+    J = Ref{Cdouble}(0)
+    GC.@preserve target check(ccall((:gd_adjoint_misfit, libgdhip), Cint,
+        (Ptr{Cvoid}, Cint, Cint, Ptr{Float64}, Ref{Cdouble}, Cint), a.ptr, MISFIT_KIND[kind], level, target, J,
+        GD_HOST))
+    return J[], a
+end
+
+function ChainRulesCore.rrule(::typeof(surface_misfit), s::HIPStep, re::Array{Float64}, im::Array{Float64},
+                              nsteps::Integer, target::Matrix{Float64}; kind::Symbol=:B_r, level::Integer,
+                              every::Integer=5)
+    J, a = _misfit_forward(s, re, im, nsteps, target, kind, level, every)
+    function surface_misfit_pullback(dJ)
+        gre, gim = adjoint_backward!(a, similar(re), similar(im))
+        c = unthunk(dJ)
+        return (NoTangent(), NoTangent(), c .* gre, c .* gim, NoTangent(), NoTangent())
+    end
+    return J, surface_misfit_pullback
+end
+
 end # module

@@ -147,6 +147,22 @@ int gd_sht_vector_synth(gd_sht_plan *p, const double *tor_re, const double *tor_
 int gd_sht_vector_analysis(gd_sht_plan *p, const double *vt, const double *vp, int nr,
                            double *tor_re, double *tor_im, double *pol_re, double *pol_im,
                            int flags);
+/* Transposes of the four transforms above (the adjoint of the SHT pair
+ * shtnskit_field_functions.jl:11-21, 131-141, 252-321, whose derivatives the
+ * reference takes through SHTnsKit's ChainRulesCore extension,
+ * Manifest.toml:657-669): for the real inner products sum re*re' + im*im'
+ * on spectral planes and sum f*f' on grid values, <A x, y> = <x, A^T y>.
+ * Arguments mirror the forward calls with inputs and outputs exchanged. */
+int gd_sht_scalar_synth_adjoint(gd_sht_plan *p, const double *phys, int nr, double *re, double *im,
+                                int flags);
+int gd_sht_scalar_analysis_adjoint(gd_sht_plan *p, const double *re, const double *im, int nr,
+                                   double *phys, int flags);
+int gd_sht_vector_synth_adjoint(gd_sht_plan *p, const double *vt, const double *vp, int nr,
+                                double *tor_re, double *tor_im, double *pol_re, double *pol_im,
+                                int flags);
+int gd_sht_vector_analysis_adjoint(gd_sht_plan *p, const double *tor_re, const double *tor_im,
+                                   const double *pol_re, const double *pol_im, int nr,
+                                   double *vt, double *vp, int flags);
 
 /* ---- the device-resident CNAB2 step ----------------------------------------
  * replaces the bench step of scripts/bench_timestepping.jl:6-18: the
@@ -196,7 +212,9 @@ int gd_step_profile(gd_step *s, int nsteps, double *out);
  * together.  Each graph replay holds every shell's step, the shells spread
  * round-robin over nstreams streams forked from and joined to the context
  * stream.  All shells must be at the same step count; step 0 (the AB1
- * bootstrap) runs eagerly.  A shell's results equal its own gd_step_run_graph. */
+ * bootstrap) runs eagerly.  A shell's results equal its own gd_step_run_graph.
+ * A member whose captured work changes (conditions, composition) makes the
+ * batch recapture; member steps must outlive the batch. */
 typedef struct gd_batch gd_batch;
 int gd_batch_create(gd_ctx *ctx, int n, gd_step *const *steps, int nstreams, gd_batch **out);
 int gd_batch_run(gd_batch *b, int nsteps);
@@ -248,7 +266,7 @@ int gd_step_shape(const gd_step *s, int *lmax, int *mmax, int *nr);
  * temperature's Dirichlet rows of S_l take the boundary values the conditions
  * leave in the state.  NULL arrays: back to the default (boundary rows read as
  * 0, the reference's default Dirichlet 0).  Drops the step's captured graphs;
- * a batch keeps the conditions its graphs were captured with. */
+ * a batch holding the step recaptures its graphs before its next run. */
 int gd_step_set_temperature_bc(gd_step *s, const int *bc_inner, const int *bc_outer, const double *values);
 /* Composition (compositional.jl): a passive scalar C of the tangential step
  * (radials = 0, full_products = 0), advected by u (-u.grad C, as the
@@ -318,6 +336,48 @@ int gd_restart_read_arrays(const char *path, int lmax, int mmax, int nr, double 
 int gd_restart_filename(const char *output_dir, const char *prefix, const char *file_type,
                         const char *geometry, double time, int rank, int time_first, char *buf,
                         int buflen);
+
+/* ---- adjoint of the step ---------------------------------------------------
+ * Reverse-mode derivatives of the tangential step (radials = 0,
+ * full_products = 0; no temperature conditions or composition): the
+ * transposed CNAB2 solve (U^T then L^T of the stored no-pivot LU,
+ * linear_algebra.jl:59-88, timestep.jl:610-730) and the transposed
+ * nonlinear terms through the transposed SHT pair
+ * (shtnskit_field_functions.jl:252-321) and the transposed pointwise products
+ * (velocity.jl:512-527, scalar_field_common.jl:401-407).  The reference
+ * takes such derivatives through SHTnsKit.jl's ChainRulesCore extension
+ * (Manifest.toml:657-669, SHTnsKit AdvancedADExt); the Julia binding wraps
+ * the entries below in rrules.  Planes as gd_step_set_state; a cotangent is
+ * (dJ/d re, dJ/d im) per coefficient. */
+/* lam = N'(x)^T mu at the step's current state x (N: gd_step_nonlinear) */
+int gd_step_nonlinear_adjoint(gd_step *s, const double *mu_re, const double *mu_im, double *lam_re,
+                              double *lam_im, int flags);
+/* the transposed step at the current state (the AB1 bootstrap when the step
+ * count is 0): given the cotangents lam of the new state and eta of this
+ * step's nonlinear terms (the history the next step reads), returns the
+ * cotangents of the state (olam) and of the previous step's nonlinear terms
+ * (oeta; zero at the bootstrap) */
+int gd_step_vjp(gd_step *s, const double *lam_re, const double *lam_im, const double *eta_re,
+                const double *eta_im, double *olam_re, double *olam_im, double *oeta_re, double *oeta_im,
+                int flags);
+/* The adjoint of an nsteps run from step 0 with checkpoints (state and AB2
+ * history) every `every` steps; the backward pass recomputes each segment's
+ * states from its checkpoint.  forward: runs the steps from the step's
+ * current state (it must be at step 0) and keeps the checkpoints.  misfit:
+ * J = 1/2 sum over the nlat x nlon grid of (X - target)^2, X the field
+ * `kind` (0 B_r = l(l+1) B_pol / r^2, magnetic.jl:257-259; 1 u_r from u_pol;
+ * 2 T) at radial level `level` of the final state, synthesised in FP64
+ * (colatitude fastest, as gd_sht_scalar_synth with nr = 1); seeds the final
+ * cotangent.  seed: the final cotangent given directly.  backward: the
+ * gradient of the seeded functional with respect to the initial state; the
+ * step is left at the end of the forward run. */
+typedef struct gd_adjoint gd_adjoint;
+int gd_adjoint_create(gd_step *s, int nsteps, int every, gd_adjoint **out);
+int gd_adjoint_destroy(gd_adjoint *a);
+int gd_adjoint_forward(gd_adjoint *a);
+int gd_adjoint_misfit(gd_adjoint *a, int kind, int level, const double *target, double *J, int flags);
+int gd_adjoint_seed(gd_adjoint *a, const double *re, const double *im, int flags);
+int gd_adjoint_backward(gd_adjoint *a, double *g_re, double *g_im, int flags);
 
 /* replaces compute_timestep_error (timestep.jl:733-753) on device arrays */
 int gd_timestep_error(gd_ctx *ctx, long n, const double *a_re, const double *a_im,

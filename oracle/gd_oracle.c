@@ -25,6 +25,19 @@
 
 #define BAND(ab, bw, row, col) ((ab)[(row) + (2 * (bw) + 1) * (col)])
 
+/* OpenMP threads of the parallel loops below (the CPU baseline is timed on
+ * all host cores and on one); returns the count in effect */
+int gdo_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
 /* ---------------------------------------------------------------------- */
 /* radial grid: fields.jl:108-131                                          */
 /* ---------------------------------------------------------------------- */

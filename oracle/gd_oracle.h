@@ -94,6 +94,8 @@ double gdo_timestep_error(long n, const double *a_re, const double *a_im,
  * at one radial level (for product/unit tests); order documented in .c */
 void gdo_physical_level(gdo_model *, const double *st_re, const double *st_im, int r,
                         double *fields15, double *prods5);
+/* OpenMP thread count of the parallel loops (n > 0 sets it); returns it */
+int gdo_set_threads(int n);
 
 #ifdef __cplusplus
 }

@@ -70,6 +70,7 @@ def lib():
             "gdo_energies": (None, [vp, _dp, _dp, _dp]),
             "gdo_timestep_error": (c_double, [c_long, _dp, _dp, _dp, _dp]),
             "gdo_physical_level": (None, [vp, _dp, _dp, c_int, _dp, _dp]),
+            "gdo_set_threads": (c_int, [c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -81,6 +82,11 @@ def lib():
 
 def _c(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's loops (n > 0 sets them); returns the count."""
+    return lib().gdo_set_threads(int(n))
 
 
 # ---------------------------------------------------------------- radial

@@ -97,3 +97,43 @@ def test_restart_without_history_bootstraps(tmp_path):
     b.run(2)
     c.run(2)
     assert np.array_equal(c.get_state_complex(), b.get_state_complex())
+
+
+@pytest.mark.parametrize("prec,out_prec", [("fp64", "fp64"), ("fp32", "fp32")])
+def test_gpu_written_file_read_by_scipy(tmp_path, prec, out_prec):
+    """A file written from a GPU step, opened by scipy's independent NetCDF
+    classic reader: the fields (outputs_writer.jl:1221-1273) and the AB2
+    history equal the step's own state and history in l-major order, and the
+    l_values / m_values / r / step variables describe the step."""
+    import gdhip
+    from scipy.io import netcdf_file
+
+    print("case: GPU restart file read by scipy %s -> %s" % (prec, out_prec), flush=True)
+    st0 = gdhip.seeded_state(12, CASE["lmax"], CASE["nr"], amp=1e-3)
+    a = _step(prec)
+    a.set_state_complex(st0)
+    a.run(3, graph=True)
+    st, hist = a.get_state_complex(), a.get_history()
+    cfg = gdhip.OutputConfig(output_dir=str(tmp_path), filename_prefix="gd", output_precision=out_prec)
+    path = gdhip.write_restart(a, gdhip.TimeTracker(), {"current_time": 3e-4}, cfg)
+    names = ("velocity_toroidal", "velocity_poloidal", "magnetic_toroidal", "magnetic_poloidal",
+             "temperature_spectral")
+    f = netcdf_file(path, "r", mmap=False)
+    try:
+        cast = np.float64 if out_prec == "fp64" else np.float32
+        for k, nm in enumerate(names):
+            assert f.variables[nm + "_real"].dimensions == ("r", "spectral_mode")
+            for part, ref in (("_real", st[k].real), ("_imag", st[k].imag)):
+                got = f.variables[nm + part].data
+                assert np.array_equal(got, ref.astype(cast)), nm + part
+            for part, ref in (("_real", hist[k].real), ("_imag", hist[k].imag)):
+                got = f.variables[nm + "_prev_nonlinear" + part].data
+                assert np.array_equal(got, ref.astype(cast)), nm + "_prev_nonlinear" + part
+        lv = np.array([l for l in range(CASE["lmax"] + 1) for m in range(l + 1)])
+        mv = np.array([m for l in range(CASE["lmax"] + 1) for m in range(l + 1)])
+        assert np.array_equal(f.variables["l_values"].data, lv)
+        assert np.array_equal(f.variables["m_values"].data, mv)
+        assert np.allclose(f.variables["r"].data, a.radii(), rtol=0, atol=1e-15)
+        assert int(f.variables["step"].data.ravel()[0]) == 3
+    finally:
+        f.close()

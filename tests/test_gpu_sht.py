@@ -10,7 +10,9 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"fp64": 1e-12, "fp32": 2e-5}
+# BF16x3: each product carries ~2^-16 relative error (hi/lo bf16 halves,
+# lo*lo dropped), so the transforms sit near 1e-5 relative
+TOL = {"fp64": 1e-12, "fp32": 2e-5, "bf16x3": 2e-4}
 
 
 def _rel(a, b):
@@ -40,7 +42,7 @@ def test_plm_tables_match_oracle(oracle, lmax, nlat):
     assert np.abs(dP - dPo).max() < 1e-10 * max(1.0, np.abs(dPo).max())
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
 @pytest.mark.parametrize("lmax,nlat,nlon,nr", [(6, 12, 24, 6), (31, 48, 96, 5), (127, 192, 384, 3)])
 def test_scalar_transforms_match_oracle(oracle, prec, lmax, nlat, nlon, nr):
     import gdhip
@@ -63,7 +65,7 @@ def test_scalar_transforms_match_oracle(oracle, prec, lmax, nlat, nlon, nr):
     assert _rel(back.complex(), a) < TOL[prec] * 10
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
 @pytest.mark.parametrize("lmax,nlat,nlon,nr", [(7, 12, 24, 4), (31, 48, 96, 5), (127, 192, 384, 3)])
 def test_vector_transforms_match_oracle(oracle, prec, lmax, nlat, nlon, nr):
     import gdhip

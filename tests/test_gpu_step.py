@@ -107,6 +107,23 @@ def test_steps_fp32_within_1e6_of_oracle(oracle):
     assert max(err) < 1e-6
 
 
+def test_steps_bf16x3_close_to_oracle(oracle):
+    """The step with BF16x3 Legendre contractions (split-precision BF16 MFMA,
+    FP32 accumulation; FFT and products FP32, solves FP64): per-field error
+    against the oracle after 5 steps, bar 1e-4 (each contraction carries
+    ~2^-16 relative error)."""
+    print("case: 5 steps bf16x3 C0", flush=True)
+    st, g, o = _make(oracle, "bf16x3", **C0)
+    s, nlp = st, None
+    for k in range(5):
+        s, nl, nlp = o.step(s, nlp, k)
+    g.run(1)
+    g.run(4, graph=True)
+    err = _relf(g.get_state_complex(), s)
+    print("bf16x3 state rel err", err)
+    assert max(err) < 1e-4
+
+
 def test_step_fp64_matches_oracle_c2_shape(oracle):
     print("case: 1 step fp64 lmax=127 nr=65", flush=True)
     st, g, o = _make(oracle, "fp64", lmax=127, nlat=192, nlon=384, nr=65, Ra=500.0)

@@ -178,6 +178,20 @@ def test_reader_rejects_bad_files(tmp_path):
         gdhip.read_restart_arrays(miss, L, nr)
     with pytest.raises(gdhip.GDError, match="cannot open"):
         gdhip.read_restart_arrays(str(tmp_path / "absent.nc"), L, nr)
+    # a header whose dimension lengths multiply past 2^64: rejected, not read
+    big = str(tmp_path / "big.nc")
+    f = netcdf_file(big, "w", version=2)
+    for d in ("xa", "xb", "xc"):
+        f.createDimension(d, 2)
+    f.createVariable("velocity_toroidal_real", "d", ("xa", "xb", "xc"))[:] = 1.0
+    f.close()
+    raw = bytearray(open(big, "rb").read())
+    for d in (b"xa", b"xb", b"xc"):
+        k = raw.index(d + b"\x00\x00") + 4
+        raw[k:k + 4] = b"\xff\xff\xff\xff"
+    open(big, "wb").write(bytes(raw))
+    with pytest.raises(gdhip.GDError, match="larger than the file|outside the file"):
+        gdhip.read_restart_arrays(big, L, nr)
 
 
 def test_generate_filename_and_find_restart_files(tmp_path):
