@@ -38,7 +38,7 @@ METRIC = "CNAB2 steps/s at lmax=127 nr=65, mixed vs FP64 GPU at ≤1e-6 rel err"
 CFG = dict(lmax=127, nlat=192, nlon=384, nr=65, E=1e-3, Pm=5.0, Pr=1.0, Ra=500.0, rratio=0.35,
            dt=1e-5, theta=0.5, bw=3)
 SEED, AMP = 2024, 1e-3
-PEAK = {"hbm": 8000.0, "fp32": 157.3, "fp64": 78.6}  # GB/s, TFLOP/s (MI355X dense)
+PEAK = {"hbm": 8000.0, "fp32": 157.3, "fp64": 78.6, "bf16": 2500.0}  # GB/s, TFLOP/s (MI355X dense)
 
 
 def kernel_costs(prec_bytes, tangential=True):
@@ -564,7 +564,8 @@ C1 = dict(lmax=255, nlat=384, nlon=768, nr=97)
 def sht_c1(reps):
     """Transform round trip at lmax = mmax = 255, 384 x 768, 97 levels: three
     scalar fields and one (tor, pol) pair, seeded normal x (l+1)^-1.5,
-    synthesis then analysis of each, FP64 and FP32, device tensors.  Per
+    synthesis then analysis of each, FP64, FP32 and BF16x3 (split-precision
+    BF16 Legendre contractions, FP32 accumulation), device tensors.  Per
     precision: relative L2 error of the recovered coefficients (worst field),
     ms per round trip (events around reps round trips) and the Legendre work
     rate: 14 scalar-equivalent contractions (3 + 3 scalar, 4 + 4 for the
@@ -579,7 +580,7 @@ def sht_c1(reps):
     L, nlat, nlon, nr = C1["lmax"], C1["nlat"], C1["nlon"], C1["nr"]
     out = {"workload": "SHT round trip lmax=mmax=255 nlat=384 nlon=768, 97 levels, 3 scalars + 1 (tor,pol) pair"}
     st = gdhip.seeded_state(2025, L, nr, nfields=5, amp=1.0, power=1.5, zero_boundaries=False)
-    for prec, peak in (("fp64", PEAK["fp64"]), ("fp32", PEAK["fp32"])):
+    for prec, peak in (("fp64", PEAK["fp64"]), ("fp32", PEAK["fp32"]), ("bf16x3", PEAK["bf16"])):
         cfg = gdhip.create_shtnskit_config(L, nlat=nlat, nlon=nlon, precision=prec)
         st[3:, :, cfg.l_values == 0] = 0.0
         nlm = cfg.nlm
@@ -624,6 +625,9 @@ def sht_c1(reps):
         out[prec] = {"round_trip_rel_l2_max": float("%.3e" % err), "ms_per_round_trip": round(ms, 3),
                      "legendre_gflop": round(gf, 2), "legendre_tflops": round(gf / ms, 2),
                      "fraction_of_mfma_peak": round(gf / ms / peak, 4)}
+        if prec == "bf16x3":
+            # each product is three BF16 MFMA products (hi hi, hi lo, lo hi)
+            out[prec]["issued_bf16_fraction_of_peak"] = round(3 * gf / ms / peak, 4)
         del ins, outs, phs, vec, cfg
         torch.cuda.empty_cache()
     return out
@@ -741,11 +745,20 @@ def main():
         full_mixed.set_state_complex(st)
         full_mixed.run(args.err_steps, graph=True)
         err_full = rel_err(full_mixed.get_state_complex(), st_mixed)
+    # the step with BF16x3 Legendre contractions (FFT and products FP32, solves FP64)
+    bf = gdhip.CNAB2Step(lmax, nlat, nlon, nr, precision="bf16x3", **kw)
+    bf.set_state_complex(st)
+    bf.run(1)
+    one_step_bf = bf.get_state_complex()
+    bf.run(args.err_steps - 1, graph=True)
+    st_bf = bf.get_state_complex()
+    e_bf = bf.energies()
 
     if args.copy_probe:
         copy_probe(args.copy_probe)
     t64 = timed(fp64, st, args.steps, args.warmup)
     tmx = timed(mixed, st, args.steps, args.warmup)
+    tbf = timed(bf, st, args.steps, args.warmup)
     prof = mixed.profile(5)
     prof64 = fp64.profile(5)
     if not args.no_full:
@@ -778,6 +791,10 @@ def main():
                      [float("%.3e" % e) for e in err_mixed],
                      "energies_fp64": [float("%.10e" % e) for e in e_ref],
                      "energies_mixed_rel_err": [float("%.3e" % (abs(a - b) / abs(b))) for a, b in zip(e_mix, e_ref)]},
+        "bf16x3": {"steps_per_s": round(args.steps / tbf, 2), "over_fp64": round(t64 / tbf, 3),
+                   "vs_fp64_max_rel_per_field_after_%d_steps" % args.err_steps:
+                   [float("%.3e" % e) for e in rel_err(st_bf, ref)],
+                   "kernel_ms": {k: round(v, 5) for k, v in bf.profile(5).items()}},
         "kernel_ms_mixed": {k: round(v, 5) for k, v in prof.items()},
         "kernel_ms_fp64": {k: round(v, 5) for k, v in prof64.items()},
         "roofline": roof,
@@ -846,7 +863,8 @@ def main():
             out["restart_io"] = {"error": repr(exc)[:300]}
     if not args.no_cpu_baseline:
         cb, acc = cpu_baseline(st, args.err_steps, {"fp64": (one_step64, ref, e_ref),
-                                                    "mixed": (one_step_mx, st_mixed, e_mix)})
+                                                    "mixed": (one_step_mx, st_mixed, e_mix),
+                                                    "bf16x3": (one_step_bf, st_bf, e_bf)})
         out["cpu_baseline"] = cb
         out["accuracy"].update(acc)
     print(json.dumps(out), flush=True)
