@@ -4,6 +4,7 @@
 #include <cstdarg>
 
 #include "gd_internal.hpp"
+#include "gd_radial.hpp"
 
 static thread_local char g_err[1024] = "";
 
@@ -64,6 +65,15 @@ int gd_from_device(gd_ctx *ctx, void *host, const void *dev, size_t bytes)
     return GD_OK;
 }
 
+// ---------------------------------------------------------------- allocation-free scratch
+// The partial sums of gd_timestep_error_noalloc: a fixed block of
+// GD_NOALLOC_PARTIALS doubles (2 KB) in every GPU's copy of this array, held
+// until the process ends.  In host code the array's name is not a device
+// address: each context resolves its device's copy once, at creation, with
+// that device current (hipGetSymbolAddress), and kernels receive that address.
+constexpr int GD_NOALLOC_PARTIALS = 256;
+static __device__ __align__(256) double scratch[GD_NOALLOC_PARTIALS];
+
 // ---------------------------------------------------------------- context
 extern "C" int gd_ctx_create(int device, gd_ctx **out)
 {
@@ -84,8 +94,36 @@ extern "C" int gd_ctx_create(int device, gd_ctx **out)
         return GD_ERR_HIP;
     }
     c->stream = c->own_stream;
+    void *sp = nullptr;
+    e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !sp) {
+        (void)hipStreamDestroy(c->own_stream);
+        delete c;
+        gd_set_error("gd_ctx_create: hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        return GD_ERR_HIP;
+    }
+    c->noalloc_scratch = static_cast<double *>(sp);
     *out = c;
     return GD_OK;
+}
+
+// compute_timestep_error (timestep.jl:733-753) without any allocation: device
+// arrays in, the result written to the device double `out` on the context's
+// stream (asynchronous).  The reduction's 256 partial sums go to the
+// context's device copy of `scratch` (every call writes all of them before
+// they are read, so nothing needs clearing); the block count is that of
+// gd_timestep_error, so the two return the same bits.  Any n >= 0: the
+// partial count does not depend on n.  Calls that overlap on one GPU share
+// the array and race.
+extern "C" int gd_timestep_error_noalloc(gd_ctx *ctx, long n, const double *a_re, const double *a_im,
+                                         const double *b_re, const double *b_im, double *out)
+{
+    GD_CHECK(ctx && a_re && a_im && b_re && b_im && out, GD_ERR_ARG, "gd_timestep_error_noalloc: null argument");
+    GD_CHECK(n >= 0, GD_ERR_ARG, "gd_timestep_error_noalloc: n = %ld < 0", n);
+    GD_CHECK(ctx->noalloc_scratch, GD_ERR_ARG, "gd_timestep_error_noalloc: context without scratch");
+    GD_HIP(hipSetDevice(ctx->device));
+    return gd_launch_sqdiff(ctx->stream, n, a_re, a_im, b_re, b_im, ctx->noalloc_scratch, GD_NOALLOC_PARTIALS,
+                            out);
 }
 
 extern "C" int gd_ctx_set_stream(gd_ctx *ctx, void *s)

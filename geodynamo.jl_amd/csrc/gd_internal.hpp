@@ -40,6 +40,9 @@ struct gd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t own_stream = nullptr;
+    // device address of this device's copy of the allocation-free entry
+    // points' scratch array (gd_core.hip), resolved once at creation
+    double *noalloc_scratch = nullptr;
 };
 
 // device buffer helper (raw, explicit lifetime)

@@ -42,7 +42,7 @@ EXPORTS = [
     "gd_step_nonlinear_adjoint", "gd_step_vjp", "gd_adjoint_create", "gd_adjoint_destroy", "gd_adjoint_forward",
     "gd_adjoint_misfit", "gd_adjoint_seed", "gd_adjoint_backward",
     "gd_sht_scalar_synth_adjoint", "gd_sht_scalar_analysis_adjoint", "gd_sht_vector_synth_adjoint",
-    "gd_sht_vector_analysis_adjoint",
+    "gd_sht_vector_analysis_adjoint", "gd_timestep_error_noalloc",
 ]
 
 
@@ -131,6 +131,7 @@ def lib():
             "gd_step_energies": (i, [vp, vp]),
             "gd_step_profile": (i, [vp, i, vp]),
             "gd_timestep_error": (i, [vp, l, vp, vp, vp, vp, vp, i]),
+            "gd_timestep_error_noalloc": (i, [vp, l, vp, vp, vp, vp, vp]),
             "gd_exp_functions": (i, [i, i, i, vp, d, d, i, i, vp]),
             "gd_etd_cache_create": (i, [vp, i, i, i, vp, d, d, i, pp]),
             "gd_etd_cache_get": (i, [vp, vp, vp]),

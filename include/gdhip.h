@@ -382,6 +382,15 @@ int gd_adjoint_backward(gd_adjoint *a, double *g_re, double *g_im, int flags);
 /* replaces compute_timestep_error (timestep.jl:733-753) on device arrays */
 int gd_timestep_error(gd_ctx *ctx, long n, const double *a_re, const double *a_im,
                       const double *b_re, const double *b_im, double *out, int flags);
+/* No counterpart in the reference: gd_timestep_error for callers that cannot
+ * allocate (code under stream/graph capture, tight serving loops).  Device
+ * arrays only; the result is written to the device double `out` on the
+ * context's stream, asynchronously.  The reduction's partial sums use a fixed
+ * 2 KB static device scratch per GPU (no hipMalloc or other allocation), so
+ * any n >= 0 is accepted; results equal gd_timestep_error's bit for bit.
+ * Calls that overlap on one GPU share the scratch and race. */
+int gd_timestep_error_noalloc(gd_ctx *ctx, long n, const double *a_re, const double *a_im,
+                              const double *b_re, const double *b_im, double *out);
 
 /* ---- exponential integrators: EAB2 and staged ERK2 (FP64) -----------------
  * A_l = nu (d2/dr2 + (2/r) d/dr - l(l+1)/r^2) for l = 0..lmax, dense nr x nr
