@@ -96,9 +96,10 @@ def kernel_costs(prec_bytes, tangential=True):
 # full fields; CNAB2: one lane per column on the FP64 path, the twisted form
 # on the FP32-synthesis path)
 TANG_KERNELS = {"spectral_pre": "spectral_pre_kernel<{T}>",
-                "synth_vector": ("synth_kernel<{T}, true, 1, 3, 16, true, 1>", "synth_kernel<{T}, true, 2, 4, 16, true>"),
-                "ring_fft_products": "ring_kernel<{T}, 3, 384>", "analysis_vector": "ana_res_kernel<{T}, true, 1, 96>",
-                "analysis_scalar": "ana_res_kernel<{T}, false, 1, 96>",
+                "synth_vector": ("synth_kernel<{T}, true, 1, 3, 16, true, 1, 0>",
+                                 "synth_kernel<{T}, true, 2, 4, 16, true, 0, 0>"),
+                "ring_fft_products": "ring_kernel<{T}, 3, 384>", "analysis_vector": "ana_res_kernel<{T}, true, 1, 96, 0>",
+                "analysis_scalar": "ana_res_kernel<{T}, false, 1, 96, 0>",
                 "cnab2_solve": ("cnab2_chain_kernel<3, 4, {T}, true,", "cnab2_twist_kernel<3, 4, {T}>")}
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "pmc_step_lmax127_nr65.json")
 COPY_PROBE_BYTES = 268435456
@@ -356,7 +357,11 @@ def adjoint_c4(nsteps=20, every=5, reps=3):
     Reported per path (FP64; mixed = FP32 transforms and products, FP64
     solves): the forward run alone (nsteps eager steps, as the adjoint's
     forward enqueues them), the forward with checkpoints + misfit + backward
-    pass, their ratio, and the mixed gradient's error against the FP64 one."""
+    pass, their ratio, and the mixed gradient's error against the FP64 one.
+    The B_r cotangent never reaches u or T (B is coupled to them through no
+    term of the tangential step), so its reverse pass is the transposed solves
+    alone; the temperature misfit at mid depth times the full reverse pass
+    (segments recomputed, the transposed nonlinear terms at every step)."""
     import numpy as np
     import torch
 
@@ -376,6 +381,7 @@ def adjoint_c4(nsteps=20, every=5, reps=3):
     tphys = gdhip.SHTnsPhysicalField(cfg, 1)
     gdhip.shtnskit_spectral_to_physical(tspec, tphys)
     target = tphys.data.ravel()
+    target_T = np.random.default_rng(7).standard_normal(nlat * nlon) * 1e-3
     out = {"workload": "lmax=mmax=63 nlat=96 nlon=192 nr=65 dt=1e-5, %d steps, checkpoints every %d, "
                        "misfit 1/2||B_r - target||^2 at the outermost solved level" % (nsteps, every)}
     grads = {}
@@ -386,7 +392,7 @@ def adjoint_c4(nsteps=20, every=5, reps=3):
     for prec in ("fp64", "fp32"):
         g = gdhip.CNAB2Step(lmax, nlat, nlon, nr, precision=prec, **kw)
         adj = gdhip.CNAB2Adjoint(g, nsteps, every=every)
-        tf, ta = [], []
+        tf, ta, tt = [], [], []
         for r in range(reps + 1):
             g.set_state(x_re, x_im)
             torch.cuda.synchronize()
@@ -402,11 +408,23 @@ def adjoint_c4(nsteps=20, every=5, reps=3):
             adj.backward(out=(g_re, g_im))
             torch.cuda.synchronize()
             ta.append(time.perf_counter() - t0)
-        tf, ta = min(tf[1:]), min(ta[1:])
-        grads[prec] = (g_re.cpu().numpy() + 1j * g_im.cpu().numpy())
+            grads[prec] = (g_re.cpu().numpy() + 1j * g_im.cpu().numpy())
+            # a functional whose cotangent reaches u and T: the full reverse pass
+            # (recomputed segments, transposed nonlinear terms at every step)
+            g.set_state(x_re, x_im)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            adj.forward()
+            adj.misfit(target_T, kind="T", level=nr // 2)
+            adj.backward(out=(g_re, g_im))
+            torch.cuda.synchronize()
+            tt.append(time.perf_counter() - t0)
+        tf, ta, tt = min(tf[1:]), min(ta[1:]), min(tt[1:])
         name = "mixed" if prec == "fp32" else "fp64"
         out[name] = {"forward_ms": round(tf * 1e3, 3), "forward_adjoint_ms": round(ta * 1e3, 3),
-                     "ratio": round(ta / tf, 3), "misfit": float("%.10e" % J)}
+                     "ratio": round(ta / tf, 3), "misfit": float("%.10e" % J),
+                     "temperature_misfit_forward_adjoint_ms": round(tt * 1e3, 3),
+                     "temperature_misfit_ratio": round(tt / tf, 3)}
         adj.close()
         g.close()
     e = np.abs(grads["fp32"] - grads["fp64"]).max() / np.abs(grads["fp64"]).max()

@@ -56,92 +56,135 @@ struct AdjChainArgs {
 // One thread per (device idx, field, re/im) column.  U^T z = lam runs upward
 // through the rows (U^T is lower triangular), L^T r = z downward (unit upper);
 // z is kept in lam_out and overwritten by E^T r, written BW rows behind the
-// L^T sweep once r of every row of its band is known.
-template <int BW> __global__ void __launch_bounds__(64) cnab2_adj_kernel(AdjChainArgs c)
+// downward sweep once r of every row of its band is known.  Both sweeps are
+// recurrences in r, so memory parallelism comes from within the thread: the
+// loads of row i + D (upward) or i - D (downward) are issued while row i is
+// computed, into the slot row i frees; they read clamped valid rows and are
+// masked where used.  The level-uniform band of nu Lap (column-major, as in
+// the step's tables) and 1/r^2 sit in LDS.
+template <int BW, int D> __global__ void __launch_bounds__(64) cnab2_adj_kernel(AdjChainArgs c)
 {
     constexpr int W = 2 * BW + 1;
     const int n = c.n, nlm = c.nlm;
     const int idx = blockIdx.x * 64 + threadIdx.x;
     const int f = blockIdx.y, ri = blockIdx.z;
-    if (idx >= nlm) return;
     const int k = f < 2 ? 0 : (f < 4 ? 1 : 2);
+    extern __shared__ double smem[];
+    double *Lc = smem, *r2 = smem + W * n;
+    const double *Lap = c.nuLap + (size_t)k * W * n;
+    for (int t = threadIdx.x; t < W * n; t += 64) Lc[t] = Lap[t];
+    for (int t = threadIdx.x; t < n; t += 64) r2[t] = c.rinv2[t];
+    __syncthreads();
+    if (idx >= nlm) return;
     const int l = c.l_of_idx[idx];
     const double nulf = c.nus[k] * (double)l * (double)(l + 1);
     const double *LU = c.LUt + (size_t)k * W * n * c.nl + l;
     const double *rdg = c.rdiag + (size_t)k * n * c.nl + l;
-    const double *Lap = c.nuLap + (size_t)k * W * n;
     const size_t fs = (size_t)nlm * n, off = (size_t)(2 * f + ri) * fs + idx;
     const double *lam = c.lam_in + off;
     double *o = c.lam_out + off;
     double *mu = c.mu_cur + off;
     double *eta = c.mu_prev ? c.mu_prev + off : nullptr;
     const bool tzero = f == 4;              // the temperature's boundary rows enter E x as 0
-    const double c1 = ((c.nl_zero_mask >> f) & 1) ? 0.0 : c.c1;
-    const double c0 = ((c.nl_zero_mask >> f) & 1) ? 0.0 : c.c0;
     const bool bcr = (c.bc_mask >> f) & 1;  // Dirichlet rows: rhs rows 0 and n-1 are constant
+    const bool nlz = (c.nl_zero_mask >> f) & 1;
+    const double c1 = nlz ? 0.0 : c.c1, c0 = nlz ? 0.0 : c.c0;
     const double inv_dt = 1.0 / c.dt, lw = 1.0 - c.theta;
-    auto lu = [&](int i, int j) { return LU[(size_t)(j * W + (BW + i - j)) * c.nl]; };
+    const size_t NL = (size_t)c.nl, NS = (size_t)nlm;
+    auto cl = [&](int j) { return j < 0 ? 0 : (j > n - 1 ? n - 1 : j); };
 
-    double zw[BW]; // z rows i-BW .. i-1, zw[BW-1] = z_{i-1}
+    // ---- U^T z = lam, rows upward; su[s][q] = U(i-1-q, i)
+    double sl[D], sd[D], su[D][BW];
+    auto uload = [&](int s, int i) {
+        const int ic = cl(i);
+        sl[s] = lam[NS * ic];
+        sd[s] = rdg[NL * ic];
+#pragma unroll
+        for (int q = 0; q < BW; ++q) su[s][q] = LU[NL * (size_t)(ic * W + BW - 1 - q)];
+    };
+    double zw[BW]; // zw[q] = z_{i-1-q}
 #pragma unroll
     for (int q = 0; q < BW; ++q) zw[q] = 0.0;
-    for (int i = 0; i < n; ++i) {
-        double v = lam[(size_t)i * nlm];
 #pragma unroll
-        for (int q = 1; q <= BW; ++q)
-            if (i - q >= 0) v -= lu(i - q, i) * zw[BW - q];
-        v *= rdg[(size_t)i * c.nl];
+    for (int d = 0; d < D; ++d) uload(d, d);
+    for (int i0 = 0; i0 < n; i0 += D) {
 #pragma unroll
-        for (int q = 0; q < BW - 1; ++q) zw[q] = zw[q + 1];
-        zw[BW - 1] = v;
-        o[(size_t)i * nlm] = v;
+        for (int d = 0; d < D; ++d) {
+            const int i = i0 + d;
+            double v = sl[d];
+#pragma unroll
+            for (int q = 0; q < BW; ++q) v -= (i - 1 - q >= 0 ? su[d][q] : 0.0) * zw[q];
+            v *= sd[d];
+#pragma unroll
+            for (int q = BW - 1; q > 0; --q) zw[q] = zw[q - 1];
+            zw[0] = v;
+            if (i < n) o[NS * i] = v;
+            uload(d, i + D);
+        }
     }
-    double rw[BW]; // raw r rows i+1 .. i+BW (rw[0] = r_{i+1})
-    double xw[W];  // effective r (0 at constant rhs rows) rows i .. i+2BW
+
+    // ---- L^T r = z, rows downward; row j = i + BW of E^T r once r_{j-BW..j+BW} is known
+    double sz[D], sm[D], sL[D][BW];
+    auto lload = [&](int s, int i) {
+        const int ic = cl(i);
+        sz[s] = o[NS * ic];
+        sm[s] = mu[NS * cl(i + BW)];
+#pragma unroll
+        for (int q = 0; q < BW; ++q) sL[s][q] = LU[NL * (size_t)(ic * W + BW + 1 + q)]; // L(i+1+q, i)
+    };
+    double rw[BW]; // raw r_{i+1+q}
+    double xw[W];  // effective r (0 at constant rhs rows) of rows i .. i+2BW
 #pragma unroll
     for (int q = 0; q < BW; ++q) rw[q] = 0.0;
 #pragma unroll
     for (int q = 0; q < W; ++q) xw[q] = 0.0;
-    for (int i = n - 1; i >= -BW; --i) {
-        double re = 0.0;
-        if (i >= 0) {
-            double v = o[(size_t)i * nlm];
 #pragma unroll
-            for (int q = 1; q <= BW; ++q)
-                if (i + q < n) v -= lu(i + q, i) * rw[q - 1];
+    for (int d = 0; d < D; ++d) lload(d, n - 1 - d);
+    for (int i0 = n - 1; i0 >= -BW; i0 -= D) {
 #pragma unroll
-            for (int q = BW - 1; q > 0; --q) rw[q] = rw[q - 1];
-            rw[0] = v;
-            re = (bcr && (i == 0 || i == n - 1)) ? 0.0 : v;
-        }
+        for (int d = 0; d < D; ++d) {
+            const int i = i0 - d;
+            double re = 0.0;
+            if (i >= 0) {
+                double v = sz[d];
 #pragma unroll
-        for (int q = W - 1; q > 0; --q) xw[q] = xw[q - 1];
-        xw[0] = re;
-        const int j = i + BW; // xw[q] = r_eff of row j - BW + q
-        if (j >= 0 && j < n) {
-            double s = 0.0;
+                for (int q = 0; q < BW; ++q) v -= (i + 1 + q < n ? sL[d][q] : 0.0) * rw[q];
 #pragma unroll
-            for (int q = 0; q < W; ++q) {
-                const int ip = j - BW + q;
-                if (ip >= 0 && ip < n) s += xw[q] * Lap[BW + ip - j + W * j]; // nu Lap[ip][j]
+                for (int q = BW - 1; q > 0; --q) rw[q] = rw[q - 1];
+                rw[0] = v;
+                re = (bcr && (i == 0 || i == n - 1)) ? 0.0 : v;
             }
-            const double rj = xw[BW];
-            double lo = inv_dt * rj + lw * (s - nulf * c.rinv2[j] * rj);
-            if (tzero && (j == 0 || j == n - 1)) lo = 0.0;
-            o[(size_t)j * nlm] = lo;
-            mu[(size_t)j * nlm] += c1 * rj;
-            if (eta) eta[(size_t)j * nlm] = -c0 * rj;
+#pragma unroll
+            for (int q = W - 1; q > 0; --q) xw[q] = xw[q - 1];
+            xw[0] = re;
+            const int j = i + BW; // xw[q] = r_eff of row j - BW + q
+            if (j >= 0 && j < n) {
+                const double *col = Lc + W * j; // nu Lap[j - BW + q][j] at col[q]
+                double sum = 0.0;
+#pragma unroll
+                for (int q = 0; q < W; ++q) sum += xw[q] * col[q]; // rows outside 0..n-1 carry r = 0
+                const double rj = xw[BW];
+                double lo = inv_dt * rj + lw * (sum - nulf * r2[j] * rj);
+                if (tzero && (j == 0 || j == n - 1)) lo = 0.0;
+                o[NS * j] = lo;
+                mu[NS * j] = sm[d] + c1 * rj;
+                if (eta) eta[NS * j] = -c0 * rj;
+            }
+            lload(d, i - D);
         }
     }
 }
 
 template <int BW> void launch_adj_chain(hipStream_t st, const AdjChainArgs &c)
 {
-    hipLaunchKernelGGL((cnab2_adj_kernel<BW>), dim3((c.nlm + 63) / 64, 5, 2), dim3(64), 0, st, c);
+    const size_t lds = sizeof(double) * (size_t)(2 * BW + 2) * c.n;
+    hipLaunchKernelGGL((cnab2_adj_kernel<BW, 4>), dim3((c.nlm + 63) / 64, 5, 2), dim3(64), lds, st, c);
 }
 
 int launch_adj_chain_bw(hipStream_t st, const AdjChainArgs &c, int bw)
 {
+    GD_CHECK(sizeof(double) * (size_t)(2 * bw + 2) * c.n <= 64 * 1024, GD_ERR_ARG,
+             "adjoint: nr=%d bw=%d exceeds the LDS band table", c.n, bw);
     switch (bw) {
     case 1: launch_adj_chain<1>(st, c); break;
     case 2: launch_adj_chain<2>(st, c); break;
@@ -191,9 +234,12 @@ __global__ void __launch_bounds__(256) adj_pre_kernel(const double *__restrict__
 }
 
 // lam += c_m (S(u*) -> u_pol, T(u*) -> u_tor, S(g*) / r -> T); tmp holds the
-// analysis outputs [field][re/im][level][idx] of u* (S, T) and g* (S, T)
+// analysis outputs [field][re/im][level][idx] of u* (S, T) and g* (S, T).
+// With the radial forcing (cb != 0) the transpose of its spectral term
+// nl(u_pol) += cb r^3 T / (l(l+1)) adds cb r^3 / (l(l+1)) mu(u_pol) to T.
 __global__ void __launch_bounds__(256) adj_post_kernel(const double *__restrict__ tmp, const int *__restrict__ m_of_idx,
-                                                       const double *__restrict__ rtab, int nlm, int nr,
+                                                       const int *__restrict__ l_of_idx, const double *__restrict__ rtab,
+                                                       const double *__restrict__ mu, double cb, int nlm, int nr,
                                                        double *__restrict__ lam)
 {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
@@ -201,11 +247,14 @@ __global__ void __launch_bounds__(256) adj_post_kernel(const double *__restrict_
     const size_t fs = (size_t)nlm * nr, o = (size_t)r * nlm + idx;
     const double c = m_of_idx[idx] ? 2.0 : 1.0;
     const double rinv = rtab[r + nr * 2];
+    const int l = l_of_idx[idx];
+    const double rr = rtab[r + nr * 3];
+    const double fb = (cb != 0.0 && l > 0) ? cb * rr * rr * rr / ((double)l * (double)(l + 1)) : 0.0;
 #pragma unroll
     for (int reim = 0; reim < 2; ++reim) {
         lam[(2 + reim) * fs + o] += c * tmp[(0 + reim) * fs + o];
         lam[(0 + reim) * fs + o] += c * tmp[(2 + reim) * fs + o];
-        lam[(8 + reim) * fs + o] += c * rinv * tmp[(4 + reim) * fs + o];
+        lam[(8 + reim) * fs + o] += c * rinv * tmp[(4 + reim) * fs + o] + fb * mu[(2 + reim) * fs + o];
     }
 }
 
@@ -278,12 +327,11 @@ static int nonlinear_adjoint_T(gd_step *s, const double *xs, const double *mu, d
     a.in = w->syn.p;
     a.cin = ADJ_COLS;
     a.nslot = ADJ_NSLOT;
-    // u and the force cotangent as full fields, grad T as a full field with
-    // zero toroidal columns (its contraction gives exact zeros there)
-    a.nf = 3;
-    a.ng = 0;
-    static const int vmap[12] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, -1, -1};
-    for (int k = 0; k < 20; ++k) a.colmap[k] = k < 12 ? vmap[k] : -1;
+    // u and the force cotangent as full fields, grad T as a gradient field
+    // (S = T/r only: its zero toroidal columns are not contracted)
+    a.nf = 2;
+    a.ng = 1;
+    for (int k = 0; k < 20; ++k) a.colmap[k] = k < 10 ? k : -1;
     a.tab0 = gd_sht_tab(p, 1);
     a.tab1 = gd_sht_tab(p, 2);
     a.slot0 = 0;
@@ -331,7 +379,7 @@ static int nonlinear_adjoint_T(gd_step *s, const double *xs, const double *mu, d
     }
     GD_TRY(gd_launch_ana<T>(st, g, true, (p->lmax + 2) / 2));
     hipLaunchKernelGGL(adj_post_kernel, dim3((nlm + 255) / 256, nr), dim3(256), 0, st, tmp, w->m_of_idx.as<int>(),
-                       s->rtab.as<double>(), nlm, nr, lam);
+                       p->d_l_of_idx.as<int>(), s->rtab.as<double>(), mu, gd_step_buoyancy_factor(s), nlm, nr, lam);
     return gd_check_launch("adj_post_kernel");
 }
 
@@ -353,7 +401,7 @@ static int adjoint_supported(const gd_step *s, const char *what)
 // The transposed step k at state xs: lam_out = E^T r + N'(xs)^T mu_cur,
 // mu_cur += c1 r, mu_prev = -c0 r (not written at the bootstrap)
 static int step_vjp(gd_step *s, const double *xs, bool boot, const double *lam_in, double *lam_out, double *mu_cur,
-                    double *mu_prev)
+                    double *mu_prev, bool nonlinear_terms = true)
 {
     AdjChainArgs c{};
     c.nuLap = s->nuLap.as<double>();
@@ -376,7 +424,7 @@ static int step_vjp(gd_step *s, const double *xs, bool boot, const double *lam_i
     c.c0 = boot ? 0.0 : 0.5;
     c.nl_zero_mask = 0x0c; // as the forward tangential step (gd_step.hip, ChainArgs::nl_zero_mask)
     GD_TRY(launch_adj_chain_bw(s->ctx->stream, c, s->d.bw));
-    return nonlinear_adjoint(s, xs, mu_cur, lam_out);
+    return nonlinear_terms ? nonlinear_adjoint(s, xs, mu_cur, lam_out) : GD_OK;
 }
 
 // l-major (nlm, nr) planes of the five fields <-> m-major device planes
@@ -498,6 +546,13 @@ struct gd_adjoint {
     DevBuf seg;                            // states x_{sP} .. x_{sP+P-1} of the segment being reversed
     DevBuf lam[2], mu[2];                  // cotangents of x_k (lam[k & 1]) and nl_k (mu[k & 1])
     bool forward_done = false, seeded = false;
+    bool last_ok = false;                  // seg holds the last segment's states (from the forward pass)
+    // activity of the seeded cotangent: whether it reaches u or T.  In the
+    // tangential step u and T are coupled through the nonlinear terms while
+    // B_tor, B_pol only through their own (linear) solves, so a cotangent
+    // with no u or T part keeps none: the transposed nonlinear terms are then
+    // exact zeros and the states need not be recomputed.
+    bool active_uT = true;
     // surface misfit: FP64 transform plan of the step's grid, grid buffers
     gd_sht_plan mplan;
     bool mplan_ok = false;
@@ -583,11 +638,16 @@ extern "C" int gd_adjoint_forward(gd_adjoint *a)
     GD_CHECK(s->step == 0, GD_ERR_ARG, "gd_adjoint_forward: the step must be at step 0 (it is at %d)", s->step);
     GD_TRY(adjoint_supported(s, "gd_adjoint_forward"));
     GD_HIP(hipSetDevice(s->ctx->device));
+    // the last segment's states are kept as they are produced: the backward
+    // pass reverses that segment first and needs no recomputation for it
+    const int klast = (a->nseg - 1) * a->P;
+    const size_t np = s->fs * 10;
     for (int k = 0; k < a->K; ++k) {
         if (k % a->P == 0) {
             GD_TRY(copy_planes(s, a->ck_state[k / a->P].p, s->state[k & 1].p));
             if (k > 0) GD_TRY(copy_planes(s, a->ck_hist[k / a->P].p, s->nlbuf[(k & 1) ^ 1].p));
         }
+        if (k >= klast) GD_TRY(copy_planes(s, a->seg.as<double>() + (size_t)(k - klast) * np, s->state[k & 1].p));
         GD_TRY(gd_step_enqueue(s, k & 1, k == 0));
         ++s->step;
     }
@@ -595,6 +655,7 @@ extern "C" int gd_adjoint_forward(gd_adjoint *a)
     GD_TRY(copy_planes(s, a->final_hist.p, s->nlbuf[(a->K & 1) ^ 1].p));
     a->forward_done = true;
     a->seeded = false;
+    a->last_ok = true;
     return GD_OK;
 }
 
@@ -612,6 +673,7 @@ extern "C" int gd_adjoint_seed(gd_adjoint *a, const double *re, const double *im
     if (rc == GD_ERR_HIP) gd_set_error("gd_adjoint_seed: HIP failure");
     GD_TRY(rc);
     a->seeded = true;
+    a->active_uT = true;
     return GD_OK;
 }
 
@@ -759,6 +821,7 @@ extern "C" int gd_adjoint_misfit(gd_adjoint *a, int kind, int level, const doubl
     GD_TRY(gd_check_launch("misfit_seed_kernel"));
     GD_HIP(hipStreamSynchronize(st));
     a->seeded = true;
+    a->active_uT = kind != 0; // B_r seeds B_pol only
     return GD_OK;
 }
 
@@ -778,9 +841,12 @@ extern "C" int gd_adjoint_backward(gd_adjoint *a, double *g_re, double *g_im, in
     const size_t np = s->fs * 10;
     for (int sg = a->nseg - 1; sg >= 0; --sg) {
         const int k0 = sg * a->P, k1 = k0 + a->P < a->K ? k0 + a->P : a->K;
-        // recompute the segment's states x_{k0} .. x_{k1-1}
-        GD_TRY(copy_planes(s, seg, a->ck_state[sg].p));
-        if (k1 - k0 > 1) {
+        // recompute the segment's states x_{k0} .. x_{k1-1} (the last segment's
+        // are still in seg from the forward pass)
+        const bool kept = sg == a->nseg - 1 && a->last_ok;
+        const bool need = a->active_uT; // the states enter only the transposed nonlinear terms
+        if (need && !kept) GD_TRY(copy_planes(s, seg, a->ck_state[sg].p));
+        if (need && !kept && k1 - k0 > 1) {
             GD_TRY(set_step_at(s, k0, a->ck_state[sg].p, a->ck_hist[sg].p));
             for (int k = k0; k < k1 - 1; ++k) {
                 GD_TRY(gd_step_enqueue(s, k & 1, k == 0));
@@ -791,8 +857,11 @@ extern "C" int gd_adjoint_backward(gd_adjoint *a, double *g_re, double *g_im, in
         for (int k = k1 - 1; k >= k0; --k)
             GD_TRY(step_vjp(s, seg + (size_t)(k - k0) * np, k == 0, a->lam[(k + 1) & 1].as<double>(),
                             a->lam[k & 1].as<double>(), a->mu[k & 1].as<double>(),
-                            k > 0 ? a->mu[(k - 1) & 1].as<double>() : nullptr));
+                            k > 0 ? a->mu[(k - 1) & 1].as<double>() : nullptr, need));
     }
+    // seg now holds segment 0's states (the last one's only if it is segment 0,
+    // or when no segment was recomputed)
+    a->last_ok = a->last_ok && (a->nseg == 1 || !a->active_uT);
     // the step returns to the end of the forward run
     GD_TRY(set_step_at(s, a->K, a->final_state.p, a->final_hist.p));
     DevBuf io;

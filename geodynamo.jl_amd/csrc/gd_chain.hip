@@ -26,16 +26,18 @@ namespace {
 // IX: the type of the row, synthesis-column and table offsets from the
 // per-column base pointers -- unsigned where they fit 32 bits (each offset is
 // then one scalar multiply instead of a 64-bit product), size_t otherwise.
+// (blk_x, blk_y, blk_z): the block's column block, field offset and re/im (the
+// launch's own grid, or its share of a batched launch)
 template <int BW, int D, typename TS, bool DIRECT, typename IX>
-__global__ void __launch_bounds__(64) cnab2_chain_kernel(ChainArgs c)
+__device__ __forceinline__ void cnab2_chain_body(const ChainArgs &c, int blk_x, int blk_y, int blk_z)
 {
     constexpr int W = 2 * BW + 1;
     const int n = c.n, nlm = c.nlm;
-    const int idx = blockIdx.x * 64 + threadIdx.x;
+    const int idx = blk_x * 64 + threadIdx.x;
     // field, 0 re / 1 im; field 5 is the composition: laid out and treated
     // as the temperature (field 4) of its own buffers, diffusivity slot 3,
     // boundary rows preset by its condition stage
-    const int fq = c.f0 + blockIdx.y, ri = blockIdx.z;
+    const int fq = c.f0 + blk_y, ri = blk_z;
     const bool isC = fq == 5;
     const int f = isC ? 4 : fq;
     const int k = isC ? 3 : (f < 2 ? 0 : (f < 4 ? 1 : 2));
@@ -239,6 +241,33 @@ __global__ void __launch_bounds__(64) cnab2_chain_kernel(ChainArgs c)
     }
 }
 
+template <int BW, int D, typename TS, bool DIRECT, typename IX>
+__global__ void __launch_bounds__(64) cnab2_chain_kernel(ChainArgs c)
+{
+    cnab2_chain_body<BW, D, TS, DIRECT, IX>(c, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// the block's (column block, field, re/im) from its index within a shell's
+// share of a batched launch (column block fastest, as the shell's own grid)
+__device__ __forceinline__ void chain_decode(const ChainArgs &c, int loc, int &bx, int &by, int &bz)
+{
+    const int gx = (c.nlm + 63) / 64, nfield = c.nf ? c.nf : (c.c_in ? 6 : 5);
+    bx = loc % gx;
+    by = (loc / gx) % nfield;
+    bz = loc / (gx * nfield);
+}
+
+// one grid over the CNAB2 solves of several shells (a batched step)
+template <int BW, int D, typename TS, bool DIRECT, typename IX>
+__global__ void __launch_bounds__(64) cnab2_chain_batch_kernel(BatchRef<ChainArgs> b)
+{
+    const int sh = b.shell_of_block[blockIdx.x];
+    const ChainArgs &c = b.args[sh];
+    int bx, by, bz;
+    chain_decode(c, blockIdx.x - b.block_off[sh], bx, by, bz);
+    cnab2_chain_body<BW, D, TS, DIRECT, IX>(c, bx, by, bz);
+}
+
 // Twisted form of the tangential (DIRECT) column kernel.  A block of two
 // waves takes 64 columns.  Wave 0 takes rows [0, n1): top-down rhs and L1
 // substitution, then upward back substitution with U1; wave 1 takes rows
@@ -252,14 +281,15 @@ __global__ void __launch_bounds__(64) cnab2_chain_kernel(ChainArgs c)
 // cnab2_chain_kernel.  Lanes past nlm repeat column nlm-1: they compute the
 // same values as its own lane and store them to the same places, which keeps
 // both waves on the barrier and the loops free of per-lane store guards.
-template <int BW, int D, typename TS> __global__ void __launch_bounds__(128, 3) cnab2_twist_kernel(ChainArgs c)
+template <int BW, int D, typename TS>
+__device__ __forceinline__ void cnab2_twist_body(const ChainArgs &c, int blk_x, int blk_y, int blk_z)
 {
     constexpr int W = 2 * BW + 1, M2 = 2 * BW;
     const int n = c.n, nlm = c.nlm, n1 = (n + 1) / 2, nb = n - n1;
     const int lane = threadIdx.x & 63;
     const bool bot = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) != 0;
-    const int idx = min(blockIdx.x * 64 + lane, nlm - 1);
-    const int fq = c.f0 + blockIdx.y, ri = blockIdx.z;
+    const int idx = min(blk_x * 64 + lane, nlm - 1);
+    const int fq = c.f0 + blk_y, ri = blk_z;
     const bool isC = fq == 5;
     const int f = isC ? 4 : fq;
     const int k = isC ? 3 : (f < 2 ? 0 : (f < 4 ? 1 : 2));
@@ -444,6 +474,21 @@ template <int BW, int D, typename TS> __global__ void __launch_bounds__(128, 3) 
     }
 #pragma unroll
     for (int d = 0; d < D; ++d) bstep(d, t0 + d, t0 + d < nbk); // the last nbk % D steps
+}
+
+template <int BW, int D, typename TS> __global__ void __launch_bounds__(128, 3) cnab2_twist_kernel(ChainArgs c)
+{
+    cnab2_twist_body<BW, D, TS>(c, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+template <int BW, int D, typename TS>
+__global__ void __launch_bounds__(128, 3) cnab2_twist_batch_kernel(BatchRef<ChainArgs> b)
+{
+    const int sh = b.shell_of_block[blockIdx.x];
+    const ChainArgs &c = b.args[sh];
+    int bx, by, bz;
+    chain_decode(c, blockIdx.x - b.block_off[sh], bx, by, bz);
+    cnab2_twist_body<BW, D, TS>(c, bx, by, bz);
 }
 } // namespace
 

@@ -142,6 +142,16 @@ struct FftPlan {
     int build(int n_);
 };
 
+// A batched launch: one grid over the blocks that several shells' launches of
+// one kernel form (the same compile-time form for all); block b belongs to
+// shell shell_of_block[b] and is that shell's block b - block_off[shell],
+// and args[shell] are that shell's arguments (device arrays).
+template <class A> struct BatchRef {
+    const A *args;
+    const unsigned short *shell_of_block;
+    const int *block_off; // nshell + 1 entries
+};
+
 // launch helper: returns GD_ERR_HIP on launch failure
 int gd_check_launch(const char *what);
 

@@ -298,8 +298,10 @@ template <typename T, bool VEC, int NF, int NW, int KC, int NG = 0> constexpr si
 // The block's output tile -- TM nodes x LT levels, RS contiguous slots of each
 // (node, level, m) record -- is staged in LDS and stored as 16-byte vectors,
 // one record run after another.
-template <typename T, bool VEC, int NF, int NW, int KC, bool VL, int NG = 0, int MM = 0>
-__global__ void __launch_bounds__(64 * NW) synth_kernel(SynthArgs a)
+// blk, nblk: the block's index and the block count of this synthesis (the
+// launch's own, or its share of a batched launch over several shells)
+template <typename T, bool VEC, int NF, int NW, int KC, bool VL, int NG, int MM>
+__device__ __forceinline__ void synth_body(const SynthArgs &a, int blk, int nblk)
 {
     static_assert(!MM || sizeof(T) == 4, "the BF16x3 contraction takes FP32 operands");
     using TL = SynthTile<VEC, NF, NW, NG>;
@@ -308,7 +310,7 @@ __global__ void __launch_bounds__(64 * NW) synth_kernel(SynthArgs a)
     int *scol = reinterpret_cast<int *>(smem_raw);
     T *sA = reinterpret_cast<T *>(smem_raw + 128);
     const int gx = (a.nlat2 + TM - 1) / TM, gy = (a.nr + LT - 1) / LT, nm = a.mmax + 1;
-    const int L = xcd_logical(blockIdx.x, gx * gy * nm);
+    const int L = xcd_logical(blk, nblk);
     const int bx = L % gx, by = (L / gx) % gy;
     const int m = paired_m(L / (gx * gy), nm), i0 = bx * TM, r0 = by * LT;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -395,6 +397,20 @@ __global__ void __launch_bounds__(64 * NW) synth_kernel(SynthArgs a)
             *reinterpret_cast<V *>(dst) = *reinterpret_cast<const V *>(tile + e);
         }
     }
+}
+
+template <typename T, bool VEC, int NF, int NW, int KC, bool VL, int NG = 0, int MM = 0>
+__global__ void __launch_bounds__(64 * NW) synth_kernel(SynthArgs a)
+{
+    synth_body<T, VEC, NF, NW, KC, VL, NG, MM>(a, blockIdx.x, gridDim.x);
+}
+
+// one grid over the syntheses of several shells (a batched step)
+template <typename T, bool VEC, int NF, int NW, int KC, bool VL, int NG = 0, int MM = 0>
+__global__ void __launch_bounds__(64 * NW) synth_batch_kernel(BatchRef<SynthArgs> b)
+{
+    const int sh = b.shell_of_block[blockIdx.x], o = b.block_off[sh];
+    synth_body<T, VEC, NF, NW, KC, VL, NG, MM>(b.args[sh], blockIdx.x - o, b.block_off[sh + 1] - o);
 }
 
 // ---------------------------------------------------------------- analysis
@@ -573,8 +589,8 @@ template <typename T, bool VEC, int NF> __global__ void __launch_bounds__(64 * N
 // 48 us: a wave's next-tile operand prefetch then spans fewer tiles; for FP64
 // it would exceed the register budget of a 512-thread block)
 template <typename T> constexpr int ana_waves() { return 4; }
-template <typename T, bool VEC, int NF, int KN, int MM = 0>
-__global__ void __launch_bounds__(64 * ana_waves<T>()) ana_res_kernel(AnaArgs a)
+template <typename T, bool VEC, int NF, int KN, int MM>
+__device__ __forceinline__ void ana_res_body(const AnaArgs &a, int blk, int nblk)
 {
     static_assert(!MM || sizeof(T) == 4, "the BF16x3 contraction takes FP32 operands");
     constexpr int ANA_WAVES = ana_waves<T>();
@@ -592,7 +608,7 @@ __global__ void __launch_bounds__(64 * ana_waves<T>()) ana_res_kernel(AnaArgs a)
     T *stg = sA + NT * KN * TMA;                 // [wave][16][TMP] when STAGE
     __shared__ double sinv[TM];                  // 1 / (l(l+1)) of the tile's rows (0 at l = 0)
     const int gx = (a.maxgroup + TM - 1) / TM, nmz = a.mmax + 1;
-    const int L = xcd_logical(blockIdx.x, gx * 2 * nmz);
+    const int L = xcd_logical(blk, nblk);
     const int bx = L % gx, bz = L / gx;
     const int m = paired_m(bz >> 1, nmz), p = bz & 1;
     const int K = a.moff[m + 1] - a.moff[m], ne = a.ne[m];
@@ -789,6 +805,20 @@ __global__ void __launch_bounds__(64 * ana_waves<T>()) ana_res_kernel(AnaArgs a)
     }
 }
 
+template <typename T, bool VEC, int NF, int KN, int MM = 0>
+__global__ void __launch_bounds__(64 * ana_waves<T>()) ana_res_kernel(AnaArgs a)
+{
+    ana_res_body<T, VEC, NF, KN, MM>(a, blockIdx.x, gridDim.x);
+}
+
+// one grid over the analyses of several shells (a batched step)
+template <typename T, bool VEC, int NF, int KN, int MM = 0>
+__global__ void __launch_bounds__(64 * ana_waves<T>()) ana_res_batch_kernel(BatchRef<AnaArgs> b)
+{
+    const int sh = b.shell_of_block[blockIdx.x], o = b.block_off[sh];
+    ana_res_body<T, VEC, NF, KN, MM>(b.args[sh], blockIdx.x - o, b.block_off[sh + 1] - o);
+}
+
 template <typename T, bool VEC, int NF, int KN> constexpr size_t ana_res_lds()
 {
     constexpr int NT = VEC ? 2 : 1;
@@ -842,6 +872,7 @@ template <typename T> int gd_launch_synth_vec(hipStream_t st, const SynthArgs &a
     // levels per 12-column group, 3 waves = 8 levels) or 8 (4 waves = 8 levels)
     if (a.ng == 1 && a.nf == 1) launch_synth<T, true, 1, 3, 16, 1>(st, a);
     else if (a.ng == 2 && a.nf == 1) launch_synth<T, true, 1, 4, 16, 2>(st, a);
+    else if (a.ng == 1 && a.nf == 2) launch_synth<T, true, 2, 5, 16, 1>(st, a); // 10 columns, 5 waves = 8 levels
     else if (a.ng != 0) GD_CHECK(false, GD_ERR_ARG, "synth_vec: nf=%d ng=%d", a.nf, a.ng);
     else if (a.nf == 5) launch_synth<T, true, 5, 5, 16>(st, a);
     else if (a.nf == 3) launch_synth<T, true, 3, 3, 16>(st, a); // 12 columns per level, 3 waves = 4 levels
@@ -1161,7 +1192,9 @@ __device__ __forceinline__ void fft_ct(C2<T> *buf, const C2<T> *tw, int nfft)
 // One block per (northern node i, level r).
 // NC > 0: the ring length is the compile-time constant NC (nlon); NC = 0 uses
 // the runtime plan.
-template <typename T, int MODE, int NC> __global__ void __launch_bounds__(256) ring_kernel(RingArgs a)
+// (i, r): the block's northern node and level (the launch's own grid, or its
+// share of a batched launch)
+template <typename T, int MODE, int NC> __device__ __forceinline__ void ring_body(const RingArgs &a, int i, int r)
 {
     extern __shared__ char smem_raw[];
     const int n = NC > 0 ? NC : a.nlon, nm = a.mmax + 1;
@@ -1177,12 +1210,6 @@ template <typename T, int MODE, int NC> __global__ void __launch_bounds__(256) r
     // for each XCD to take a contiguous range of (level, ring) and a line's
     // pieces meet in one L2.  The step modes read and write ring-contiguous
     // records and keep the plain order.
-    int i = blockIdx.x, r = blockIdx.y;
-    if (MODE == RING_SYNTH || MODE == RING_ANALYSIS) {
-        const int L = xcd_logical(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-        i = L % gridDim.x;
-        r = L / gridDim.x;
-    }
     const int tid = threadIdx.x;
     constexpr int nthr = RING_THREADS;
     const int is = a.nlat - 1 - i; // southern mirror
@@ -1393,6 +1420,9 @@ template <typename T, int MODE, int NC> __global__ void __launch_bounds__(256) r
     const T c = static_cast<const T *>(a.wq)[i];
     T *gb = static_cast<T *>(a.gb);
     const int NPn = NPC ? NPC : NP;
+    // products np1.. go to their own records in gb2 (the scalar analysis then
+    // reads only its own slots); np1 = 0: every product in gb
+    const int np1 = a.gb2 ? a.np1 : NPn;
     for (int e = tid; e < NPn * nm; e += nthr) {
         const int P = e % NPn, m = e / NPn;
         const C2<T> Z = buf[P * ld + sk(m)];
@@ -1405,12 +1435,33 @@ template <typename T, int MODE, int NC> __global__ void __launch_bounds__(256) r
             X = C2<T>{T(0.5) * (Z.x + Zc.x), T(0.5) * (Z.y - Zc.y)};
             Y = C2<T>{T(0.5) * (Z.y + Zc.y), T(0.5) * (Zc.x - Z.x)};
         }
-        T *d = gb + (((size_t)m * a.nlat2 + i) * a.nr + r) * a.nslot_g + 4 * P;
+        const size_t rec = ((size_t)m * a.nlat2 + i) * a.nr + r;
+        T *d = P < np1 ? gb + rec * a.nslot_g + 4 * P : static_cast<T *>(a.gb2) + rec * a.nslot_g2 + 4 * (P - np1);
         d[0] = (X.x + Y.x) * c;
         d[1] = (X.y + Y.y) * c;
         d[2] = (X.x - Y.x) * c;
         d[3] = (X.y - Y.y) * c;
     }
+}
+
+template <typename T, int MODE, int NC> __global__ void __launch_bounds__(256) ring_kernel(RingArgs a)
+{
+    int i = blockIdx.x, r = blockIdx.y;
+    if (MODE == RING_SYNTH || MODE == RING_ANALYSIS) {
+        const int L = xcd_logical(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+        i = L % gridDim.x;
+        r = L / gridDim.x;
+    }
+    ring_body<T, MODE, NC>(a, i, r);
+}
+
+// one grid over the ring passes of several shells (step modes): a shell's
+// blocks are its (node, level) pairs, node fastest
+template <typename T, int MODE, int NC> __global__ void __launch_bounds__(256) ring_batch_kernel(BatchRef<RingArgs> b)
+{
+    const int sh = b.shell_of_block[blockIdx.x], loc = blockIdx.x - b.block_off[sh];
+    const RingArgs &a = b.args[sh];
+    ring_body<T, MODE, NC>(a, loc % a.nlat2, loc / a.nlat2);
 }
 } // namespace
 

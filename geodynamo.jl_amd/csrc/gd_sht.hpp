@@ -53,6 +53,11 @@ struct RingArgs {
     const void *fb;       // T*
     void *gb;             // T*
     int nslot_f, nslot_g;
+    // optional second analysis input: products np1 .. (the scalar analysis's)
+    // in their own records GB2[m][i][level][nslot_g2]; gb2 = null keeps every
+    // product in GB
+    void *gb2;
+    int nslot_g2, np1;
     int nfield;           // real fields to synthesise (SYNTH/STEP) or analyse (ANALYSIS)
     int fn_off[16], fs_off[16]; // FB slot (re) of north / south spectra of each field
     const double *phys_in[2];

@@ -117,6 +117,31 @@ template <typename T> static int spectral_pre_into(gd_step *s, const double *sta
     return gd_check_launch("spectral_pre_kernel");
 }
 
+// The buoyancy of velocity.jl:534-537, F_r = (Pm/E)(Pm/Pr) Ra r T, into the
+// poloidal equation (radial_force = 1): the poloidal scalar with radial
+// component F_r (u_r = l(l+1) P / r^2, magnetic.jl:257-259) is
+// r^2 F_r / (l(l+1)), so nl(u_pol) += c r^3 T / (l(l+1)) for l >= 1 (l = 0 has
+// no poloidal flow).  Linear in T and spectral: no transform is needed.
+__global__ void __launch_bounds__(256) buoyancy_kernel(const double *__restrict__ st, const int *__restrict__ l_of_idx,
+                                                       const double *__restrict__ rtab, double c, int nlm, int nr,
+                                                       double *__restrict__ nl)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+    if (idx >= nlm) return;
+    const int l = l_of_idx[idx];
+    if (l == 0) return;
+    const size_t fs = (size_t)nlm * nr, o = (size_t)r * nlm + idx;
+    const double rr = rtab[r + 3 * nr];
+    const double f = c * rr * rr * rr / ((double)l * (double)(l + 1));
+    nl[2 * fs + o] += f * st[8 * fs + o];
+    nl[3 * fs + o] += f * st[9 * fs + o];
+}
+
+double gd_step_buoyancy_factor(const gd_step *s)
+{
+    return s->d.radial_force ? (s->d.Pm / s->d.E) * (s->d.Pm / s->d.Pr) * s->d.Ra : 0.0;
+}
+
 template <typename T> static int spectral_pre(gd_step *s)
 {
     GD_TRY(spectral_pre_into<T>(s, cur_state(s), s->synV, s->synS));
@@ -188,6 +213,9 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEven
     gd_sht_fill_ring(p, r, nr);
     r.fb = s->fb.p;
     r.gb = s->gb.p;
+    // the scalar analysis's products in records of their own, so that neither
+    // analysis reads the other's slots (cache lines hold whole records)
+    r.gb2 = s->gb2.p;
     r.ro = s->d.Pm / s->d.E;
     r.coriolis_index = s->d.coriolis_index;
     AnaArgs g{};
@@ -229,7 +257,9 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEven
         // -u.grad C (compute_composition_nonlinear!, compositional.jl:219-260)
         // as a fourth product; GB records of 12 slots, 16 with it
         r.nfield = s->comp ? 6 : 4;
-        r.nslot_g = s->comp ? 16 : 12;
+        r.nslot_g = 8;                    // the force (products 0, 1)
+        r.np1 = 2;
+        r.nslot_g2 = s->comp ? 8 : 4;     // -u.grad T (and -u.grad C)
         static const int fn[6] = {0, 2, 8, 10, 16, 18}, fsou[6] = {4, 6, 12, 14, 20, 22};
         for (int F = 0; F < r.nfield; ++F) {
             r.fn_off[F] = fn[F];
@@ -254,7 +284,9 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEven
         g.scale[1] = 0;
         GD_TRY(gd_launch_ana<T>(st, g, true, maxgroup));
         GD_MARK(5);
-        g.gslot0 = 8;
+        g.gb = s->gb2.p;
+        g.nslot_g = r.nslot_g2;
+        g.gslot0 = 0;
     } else {
         // vector fields u, omega, B, j (S,T) and grad T (S only)
         a.nslot = 60;
@@ -281,7 +313,9 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEven
         GD_TRY(gd_launch_synth_scal<T>(st, a));
         GD_MARK(3);
         r.nslot_f = 60;
-        r.nslot_g = 20;
+        r.nslot_g = 16;                   // the force and u x B (products 0..3)
+        r.np1 = 4;
+        r.nslot_g2 = 4;                   // -u.grad T
         r.nfield = 15;
         for (int F = 0; F < 10; ++F) {
             const int f = F / 2, c = F % 2;
@@ -295,7 +329,7 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEven
         GD_TRY(gd_launch_ring<T>(st, RING_STEP, r));
         GD_MARK(4);
         // force -> (pol, tor) of velocity nl (velocity.jl:331); u x B -> curl (magnetic.jl:437-440)
-        g.nslot_g = 20;
+        g.nslot_g = 16;
         g.nf = 2;
         g.gslot0 = 0;
         g.tab0 = gd_sht_tab(p, 1);
@@ -310,7 +344,9 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEven
         g.scale[3] = 2;
         GD_TRY(gd_launch_ana<T>(st, g, true, maxgroup));
         GD_MARK(5);
-        g.gslot0 = 16;
+        g.gb = s->gb2.p;
+        g.nslot_g = r.nslot_g2;
+        g.gslot0 = 0;
     }
     g.nf = 1;
     g.tab0 = gd_sht_tab(p, 0);
@@ -323,6 +359,11 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEven
         g.scale[1] = 1;
     }
     GD_TRY(gd_launch_ana<T>(side ? s->side : st, g, false, maxgroup));
+    if (s->d.radial_force) { // after the vector analysis wrote nl(u_pol), on its stream
+        hipLaunchKernelGGL(buoyancy_kernel, dim3((nlm + 255) / 256, nr), dim3(256), 0, st, cur_state(s),
+                           p->d_l_of_idx.as<int>(), s->rtab.as<double>(), gd_step_buoyancy_factor(s), nlm, nr, nl_out);
+        GD_TRY(gd_check_launch("buoyancy_kernel"));
+    }
     GD_MARK(6);
     return GD_OK;
 }
@@ -493,7 +534,7 @@ static int step_release(gd_step *s)
     gd_adj_work_release(s);
     DevBuf *bufs[] = {&s->state[0], &s->state[1], &s->nlbuf[0], &s->nlbuf[1], &s->Lt, &s->LUt, &s->ULt, &s->rdiag2, &s->Rint, &s->nuLap, &s->nus,
                       &s->D1, &s->D2, &s->rtab, &s->synV,
-                      &s->synS, &s->fb, &s->gb, &s->part, &s->eout, &s->scratch, &s->status, &s->zrow, &s->rdiag};
+                      &s->synS, &s->fb, &s->gb, &s->gb2, &s->part, &s->eout, &s->scratch, &s->status, &s->zrow, &s->rdiag};
     for (DevBuf *b : bufs) b->release();
     for (hipEvent_t *e : {&s->ev_fork, &s->ev_ring, &s->ev_join})
         if (*e) (void)hipEventDestroy(*e);
@@ -583,6 +624,8 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
         if (rc == GD_OK) rc = upload_vec(ctx, s->nuLap, nlap);
     }
     GD_CHECK(d->bc_rows >= 0 && d->bc_rows <= 2, GD_ERR_ARG, "gd_step_create: bc_rows=%d", d->bc_rows);
+    GD_CHECK(d->radial_force == 0 || d->radial_force == 1, GD_ERR_ARG, "gd_step_create: radial_force=%d",
+             d->radial_force);
     // the twisted column solve on the FP32-synthesis path with bw <= 3 (at
     // lmax=127, nr=65, bw=3: 68 vs 73 us; on the FP64 path 87 vs 83 us, so
     // FP64 keeps one lane per column); GD_CHAIN_TWIST = 0 keeps one lane here too
@@ -620,7 +663,8 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
     GD_TRY(s->synV.ensure(esz * s->fs * SYNV_COLS));
     GD_TRY(s->synS.ensure(esz * s->fs * 10));
     GD_TRY(s->fb.ensure(esz * (size_t)s->sht.nlat2 * nr * nm * 60));
-    GD_TRY(s->gb.ensure(esz * (size_t)nm * s->sht.nlat2 * nr * 20));
+    GD_TRY(s->gb.ensure(esz * (size_t)nm * s->sht.nlat2 * nr * 16));
+    GD_TRY(s->gb2.ensure(esz * (size_t)nm * s->sht.nlat2 * nr * 8));
     GD_TRY(s->part.ensure(sizeof(double) * 3 * 512));
     GD_TRY(s->eout.ensure(sizeof(double) * 4));
     GD_TRY(s->zrow.ensure(sizeof(double) * s->nlm));

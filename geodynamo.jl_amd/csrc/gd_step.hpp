@@ -15,6 +15,7 @@ struct gd_step {
     DevBuf nuLap, nus;             // [nu][W*nr] nu * Laplacian band; the three diffusivities
     DevBuf D1, D2, rtab;
     DevBuf synV, synS, fb, gb;
+    DevBuf gb2;                    // the scalar analysis's Fourier records (-u.grad T, -u.grad C)
     DevBuf part, eout, scratch;
     DevBuf status;
     DevBuf zrow;                   // nlm zeros
@@ -75,3 +76,5 @@ bool gd_step_tangential(const gd_step *s);
 void gd_step_drop_graphs(gd_step *s);
 // releases the adjoint's work buffers of the step (gd_adjoint.hip)
 void gd_adj_work_release(gd_step *s);
+// the buoyancy coefficient (Pm/E)(Pm/Pr) Ra of the radial forcing, 0 when it is off
+double gd_step_buoyancy_factor(const gd_step *s);

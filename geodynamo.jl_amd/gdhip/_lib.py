@@ -75,7 +75,8 @@ class StepDesc(ctypes.Structure):
                 ("lmax", ctypes.c_int), ("mmax", ctypes.c_int), ("nlat", ctypes.c_int),
                 ("nlon", ctypes.c_int), ("nr", ctypes.c_int), ("bw", ctypes.c_int),
                 ("prec", ctypes.c_int), ("fd_mode", ctypes.c_int), ("radials", ctypes.c_int),
-                ("coriolis_index", ctypes.c_int), ("bc_rows", ctypes.c_int), ("full_products", ctypes.c_int)]
+                ("coriolis_index", ctypes.c_int), ("bc_rows", ctypes.c_int), ("full_products", ctypes.c_int),
+                ("radial_force", ctypes.c_int)]
 
 
 _lib = None

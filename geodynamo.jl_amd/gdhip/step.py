@@ -17,11 +17,15 @@ FIELDS = ("u_tor", "u_pol", "B_tor", "B_pol", "T")
 class CNAB2Step:
     def __init__(self, lmax, nlat, nlon, nr, E=1e-3, Pm=5.0, Pr=1.0, Ra=100.0, rratio=0.35, dt=1e-5,
                  theta=0.5, bw=3, mmax=None, precision="fp64", literal_fd=False, radials=False,
-                 coriolis_index=0, bc_rows=1, full_products=False, device=0):
+                 coriolis_index=0, bc_rows=1, full_products=False, radial_force=False, device=0):
+        """radial_force: the buoyancy (velocity.jl:534-537) enters the poloidal
+        equation through u_r = l(l+1) P / r^2 (magnetic.jl:257-259); off by
+        default, the reference's data flow, where it reaches no equation."""
         mmax = lmax if mmax is None else mmax
         self.desc = StepDesc(E, Pm, Pr, Ra, rratio, dt, theta, lmax, mmax, nlat, nlon, nr, bw,
                              PRECISIONS[precision], GD_FD_LITERAL if literal_fd else GD_FD_TRANSPOSE,
-                             int(radials), int(coriolis_index), int(bc_rows), int(full_products))
+                             int(radials), int(coriolis_index), int(bc_rows), int(full_products),
+                             int(radial_force))
         self.lmax, self.mmax, self.nlat, self.nlon, self.nr = lmax, mmax, nlat, nlon, nr
         self.nlm = sum(min(l, mmax) + 1 for l in range(lmax + 1))
         self.device = device

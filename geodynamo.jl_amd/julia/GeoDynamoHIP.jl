@@ -269,6 +269,7 @@ struct GDStepDesc
     E::Cdouble; Pm::Cdouble; Pr::Cdouble; Ra::Cdouble; rratio::Cdouble; dt::Cdouble; theta::Cdouble
     lmax::Cint; mmax::Cint; nlat::Cint; nlon::Cint; nr::Cint; bw::Cint
     prec::Cint; fd_mode::Cint; radials::Cint; coriolis_index::Cint; bc_rows::Cint; full_products::Cint
+    radial_force::Cint
 end
 
 mutable struct HIPStep
@@ -277,9 +278,9 @@ end
 
 "The bench step of scripts/bench_timestepping.jl:6-18 on the GPU."
 function create_step(; lmax, nlat, nlon, nr, E=1e-3, Pm=5.0, Pr=1.0, Ra=100.0, rratio=0.35, dt=1e-5,
-                     theta=0.5, bw=3, precision::Symbol=:fp32)
+                     theta=0.5, bw=3, precision::Symbol=:fp32, radial_force::Bool=false)
     d = GDStepDesc(E, Pm, Pr, Ra, rratio, dt, theta, lmax, lmax, nlat, nlon, nr, bw, PREC[precision],
-                   GD_FD_TRANSPOSE, 0, 0, 1, 0)
+                   GD_FD_TRANSPOSE, 0, 0, 1, 0, radial_force ? 1 : 0)
     h = Ref{Ptr{Cvoid}}(C_NULL)
     check(ccall((:gd_step_create, libgdhip), Cint, (Ptr{Cvoid}, Ref{GDStepDesc}, Ref{Ptr{Cvoid}}),
                 context(), d, h))

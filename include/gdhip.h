@@ -183,6 +183,12 @@ typedef struct {
                            j x B and u x B and the u_r dT/dr term are identically zero, so only
                            u and grad T are transformed (same result: the dropped terms are
                            exact zeros); 1: transform every field and form every product */
+    int radial_force;   /* 0 (default): the reference's data flow, where the radial force
+                           (buoyancy on adv_r, velocity.jl:534-537) reaches no equation; 1: the
+                           buoyancy F_r = (Pm/E)(Pm/Pr) Ra r T enters the poloidal equation as the
+                           poloidal scalar whose radial component is F_r under the reference's
+                           definition u_r = l(l+1) P / r^2 (magnetic.jl:257-259):
+                           nl(u_pol) += r^2 F_r / (l(l+1)), l >= 1 */
 } gd_step_desc;
 
 int gd_step_create(gd_ctx *ctx, const gd_step_desc *desc, gd_step **out);

@@ -921,6 +921,19 @@ void gdo_nonlinear(gdo_model *M, const double *st_re, const double *st_im, doubl
             }
             /* temperature */
             level_scalar_analysis(c, Pr + 4 * ng, nl_re + 4 * fs + so, nl_im + 4 * fs + so, &w);
+            /* radial forcing: the buoyancy F_r = (Pm/E)(Pm/Pr) Ra r T as the poloidal
+             * scalar r^2 F_r / (l(l+1)) (u_r = l(l+1) P / r^2, magnetic.jl:257-259) */
+            if (M->p.radial_force) {
+                const double rr = M->r[(size_t)3 * N + r];
+                const double cb = (M->p.Pm / M->p.E) * (M->p.Pm / M->p.Pr) * M->p.Ra;
+                for (int q = 0; q < nlm; ++q) {
+                    const int l = M->l_of_lm[q];
+                    if (l == 0) continue;
+                    const double f = cb * rr * rr * rr / ((double)l * (l + 1));
+                    nl_re[1 * fs + so + q] += f * st_re[4 * fs + so + q];
+                    nl_im[1 * fs + so + q] += f * st_im[4 * fs + so + q];
+                }
+            }
         }
         free(a); free(F);
         ws_free(&w);

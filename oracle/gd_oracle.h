@@ -75,6 +75,10 @@ typedef struct {
     int coriolis_index;  /* 0: reference indexing theta_{min(j,nlat)}; 1: theta_i */
     int bc_rows;         /* Dirichlet rows (value 0) in S_l and the rhs: 1 all fields, 2 temperature
                             only, 0 none (the reference operator) */
+    int radial_force;    /* 1: the buoyancy (Pm/E)(Pm/Pr) Ra r T (velocity.jl:534-537) enters the
+                            poloidal equation as the poloidal scalar r^2 F_r / (l(l+1)) whose
+                            radial component is F_r (magnetic.jl:257-259); 0: it reaches no
+                            equation (the reference's data flow) */
 } gdo_params;
 
 typedef struct gdo_model gdo_model;

@@ -97,6 +97,8 @@ class NpAdjoint:
             v = self.Av @ np.concatenate([at, ap])
             out[0, k], out[1, k] = _zv(v[: 2 * nlm]), _zv(v[2 * nlm:])
             out[4, k] = _zv(self.As @ aT)
+        if self.m.radial_force:
+            out[1] += self.m.buoyancy_poloidal(st[4])
         return out
 
     def nonlinear_vjp(self, st, mu):
@@ -116,6 +118,9 @@ class NpAdjoint:
             lam[1, k] += _zv(x[2 * nlm:])
             y = self.Mv.T @ np.concatenate([gbt, gbp])
             lam[4, k] += _zv(y[2 * nlm:]) / r[k]
+        if self.m.radial_force:
+            # the forcing term is linear in T with a real per-(l, r) factor
+            lam[4] += self.m.buoyancy_poloidal(mu[1])
         return lam
 
     def step(self, st, nl_prev, boot):

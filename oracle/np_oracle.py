@@ -114,10 +114,13 @@ class NpModel:
     """Field order: u_tor, u_pol, B_tor, B_pol, T; state shape (5, nr, nlm) complex."""
 
     def __init__(self, lmax, nlat, nlon, nr, E=1e-3, Pm=5.0, Pr=1.0, Ra=100.0, rratio=0.35,
-                 dt=1e-5, theta=0.5, bw=3, radials=False, coriolis_index=0, bc_rows=1):
+                 dt=1e-5, theta=0.5, bw=3, radials=False, coriolis_index=0, bc_rows=1, radial_force=0):
         self.sht = NpSHT(lmax, lmax, nlat, nlon)
         self.nr, self.E, self.Pm, self.Pr, self.dt, self.th = nr, E, Pm, Pr, dt, theta
         self.radials, self.coriolis_index, self.bc_rows = radials, coriolis_index, bc_rows
+        # buoyancy coefficient of velocity.jl:534-537 when the radial forcing is on
+        self.radial_force = radial_force
+        self.cb = (Pm / E) * (Pm / Pr) * Ra if radial_force else 0.0
         self.r = radial_grid(nr, rratio)
         self.D1 = fd_dense(1, self.r, bw)
         self.D2 = fd_dense(2, self.r, bw)
@@ -179,7 +182,17 @@ class NpModel:
             nl[2, k] = ll * pol
             nl[3, k] = -ll * tor
             nl[4, k] = sht.scalar_analysis(aT)
+        if self.radial_force:
+            nl[1] += self.buoyancy_poloidal(T)
         return nl
+
+    def buoyancy_poloidal(self, T):
+        """The poloidal scalar whose radial component u_r = l(l+1) P / r^2
+        (magnetic.jl:257-259) is the buoyancy F_r = cb r T: r^2 F_r / (l(l+1)),
+        l >= 1 (l = 0 carries no poloidal flow)."""
+        ll = self.ll
+        inv = np.where(ll > 0, 1.0 / np.where(ll > 0, ll, 1.0), 0.0)
+        return self.cb * (self.r ** 3)[:, None] * T * inv[None, :]
 
     def step(self, st, nl_prev, step_index):
         st = st.copy()

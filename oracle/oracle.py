@@ -256,7 +256,7 @@ class _Params(ctypes.Structure):
                 ("lmax", ctypes.c_int), ("mmax", ctypes.c_int), ("nlat", ctypes.c_int),
                 ("nlon", ctypes.c_int), ("nr", ctypes.c_int), ("bw", ctypes.c_int),
                 ("fd_transpose", ctypes.c_int), ("radials", ctypes.c_int),
-                ("coriolis_index", ctypes.c_int), ("bc_rows", ctypes.c_int)]
+                ("coriolis_index", ctypes.c_int), ("bc_rows", ctypes.c_int), ("radial_force", ctypes.c_int)]
 
 
 class Model:
@@ -264,10 +264,10 @@ class Model:
 
     def __init__(self, lmax, nlat, nlon, nr, E=1e-3, Pm=5.0, Pr=1.0, Ra=100.0, rratio=0.35,
                  dt=1e-5, theta=0.5, bw=3, mmax=None, fd_transpose=True, radials=False,
-                 coriolis_index=0, bc_rows=1):
+                 coriolis_index=0, bc_rows=1, radial_force=0):
         mmax = lmax if mmax is None else mmax
         self.p = _Params(E, Pm, Pr, Ra, rratio, dt, theta, lmax, mmax, nlat, nlon, nr, bw,
-                         int(fd_transpose), int(radials), int(coriolis_index), int(bc_rows))
+                         int(fd_transpose), int(radials), int(coriolis_index), int(bc_rows), int(radial_force))
         self.lmax, self.mmax, self.nlat, self.nlon, self.nr = lmax, mmax, nlat, nlon, nr
         self.nlm = nlm(lmax, mmax)
         self._h = lib().gdo_model_create(ctypes.addressof(self.p))

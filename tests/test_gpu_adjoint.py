@@ -173,6 +173,10 @@ def test_run_gradient_matches_oracle_adjoint(oracle, kind):
     # the step is left at the end of the forward run
     assert g.step_count == 4
     assert np.abs(g.get_state_complex() - xs[-1]).max() <= 1e-10 * np.abs(xs[-1]).max()
+    # a second functional of the same forward run (every segment recomputed now)
+    J2 = adj.misfit(target, kind=kind, level=level)
+    grad2 = adj.backward()
+    assert J2 == J and np.array_equal(grad2, grad)
 
 
 def _c4_target(oracle, g):

@@ -178,3 +178,24 @@ def test_planted_mistakes_fail(oracle, adj):
         bad = cls(np_oracle.NpModel(**TINY, dt=1e-2))
         lo, _ = bad.step_vjp(x, True, lam, eta)
         assert abs(lhs - _dot(lo, v)) > 1e-8 * scale, cls.__name__
+
+
+def test_step_vjp_dot_product_with_radial_force(oracle):
+    """The adjoint restatement with the radial forcing against the C oracle's
+    step with it (dot-product test)."""
+    from oracle import np_adjoint, np_oracle
+
+    rng = np.random.default_rng(13)
+    nlm = oracle.nlm(TINY["lmax"], TINY["lmax"])
+    m = oracle.Model(**TINY, dt=1e-2, Ra=400.0, radial_force=1)
+    a = np_adjoint.NpAdjoint(np_oracle.NpModel(**TINY, dt=1e-2, Ra=400.0, radial_force=1))
+    sh = (TINY["lmax"], TINY["nr"], nlm)
+    x, h, v, w = (_rand(rng, *sh) for _ in range(4))
+    lam, eta = _rand(rng, *sh), _rand(rng, *sh)
+    xp, nlp, _ = m.step(x + v, h + w, 1)
+    xm, nlm_, _ = m.step(x - v, h - w, 1)
+    lhs = _dot(lam, 0.5 * (xp - xm)) + _dot(eta, 0.5 * (nlp - nlm_))
+    lo, eo = a.step_vjp(x, False, lam, eta)
+    rhs = _dot(lo, v) + _dot(eo, w)
+    scale = np.abs(lam).sum() * np.abs(xp - xm).max() + np.abs(eta).sum() * np.abs(nlp - nlm_).max()
+    assert abs(lhs - rhs) <= 1e-12 * scale

@@ -103,3 +103,56 @@ def test_adopted_reading_is_stable_over_100_steps(oracle):
 def test_seeded_normals_statistics(oracle):
     g = oracle.seeded_normals(99, 200000)
     assert abs(g.mean()) < 0.01 and abs(g.var() - 1) < 0.01
+
+
+def test_radial_force_matches_numpy_restatement(oracle):
+    """With the radial forcing on, the C oracle's nonlinear terms equal the
+    numpy restatement's (the u_pol term differs by the buoyancy's poloidal
+    scalar, the rest are unchanged)."""
+    from oracle import np_oracle
+
+    st = oracle.seeded_state(17, TINY["lmax"], TINY["lmax"], TINY["nr"], amp=1.0, zero_boundaries=False)
+    on = oracle.Model(**TINY, Ra=300.0, radial_force=1).nonlinear(st)
+    off = oracle.Model(**TINY, Ra=300.0).nonlinear(st)
+    ref = np_oracle.NpModel(**TINY, Ra=300.0, radial_force=1).nonlinear(st)
+    assert _relf(on, ref) < 1e-12
+    assert np.abs(on[1] - off[1]).max() > 1e-3 * np.abs(off[1]).max()
+    for f in (0, 2, 3, 4):
+        assert np.array_equal(on[f], off[f])
+
+
+def test_radial_force_has_the_buoyancy_as_radial_component(oracle):
+    """The poloidal increment P_F the forcing adds to nl(u_pol) has, under
+    u_r = l(l+1) P / r^2 (magnetic.jl:257-259), the radial component of the
+    buoyancy (Pm/E)(Pm/Pr) Ra r T (velocity.jl:534-537): synthesised on the
+    grid at every level, l(l+1) P_F / r^2 equals the synthesised buoyancy
+    without its l = 0 part (which has no poloidal flow)."""
+    L, nlat, nlon, nr = TINY["lmax"], TINY["nlat"], TINY["nlon"], TINY["nr"]
+    E, Pm, Pr, Ra = 1e-3, 5.0, 1.0, 250.0
+    st = oracle.seeded_state(18, L, L, nr, amp=1.0, zero_boundaries=False)
+    dP = (oracle.Model(**TINY, E=E, Pm=Pm, Pr=Pr, Ra=Ra, radial_force=1).nonlinear(st)[1]
+          - oracle.Model(**TINY, E=E, Pm=Pm, Pr=Pr, Ra=Ra).nonlinear(st)[1])
+    sht = oracle.SHT(L, L, nlat, nlon)
+    r = oracle.radial_grid(nr, 0.35)[3]
+    l = np.array([ll for ll in range(L + 1) for m in range(ll + 1)], dtype=float)
+    Fr_spec = (Pm / E) * (Pm / Pr) * Ra * r[:, None] * st[4]
+    Fr_spec[:, l == 0] = 0.0
+    ur_spec = l * (l + 1) * dP / (r ** 2)[:, None]
+    a = sht.scalar_synth(ur_spec)
+    b = sht.scalar_synth(Fr_spec)
+    assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+
+
+def test_radial_force_makes_ra_matter(oracle):
+    """Ra enters only through the buoyancy: without the forcing two Rayleigh
+    numbers give the same state; with it they differ."""
+    st = oracle.seeded_state(19, TINY["lmax"], TINY["lmax"], TINY["nr"], amp=1e-3)
+    out = {}
+    for rf in (0, 1):
+        for Ra in (100.0, 500.0):
+            m = oracle.Model(**TINY, Ra=Ra, radial_force=rf)
+            s, nl, nlp = m.step(st, None, 0)
+            s, nl, nlp = m.step(s, nlp, 1)
+            out[rf, Ra] = s
+    assert np.array_equal(out[0, 100.0], out[0, 500.0])
+    assert np.abs(out[1, 100.0][1] - out[1, 500.0][1]).max() > 1e-6 * np.abs(out[1, 500.0][1]).max()
