@@ -432,55 +432,92 @@ def adjoint_c4(nsteps=20, every=5, reps=3):
     return out
 
 
-def batch_c3(nshell, steps, nstreams):
-    """Uneven batch: nshell independent shells, lmax drawn from {31, 47, 63, 95,
-    127}, nr from {33, 49, 65, 97}, grid 3(L+1)/2 x 3(L+1), radius ratio
-    U[0.2, 0.5] (numpy seed 3), FP64 steps, (l+1)^-2 states of amplitude
-    1e-3.  Aggregate shell-steps/s of the batch graph (every shell's step in
-    one graph, shells spread over nstreams streams) against the same shells
-    advanced one after another by their own step graphs."""
+def c3_shells(nshell, seed=3):
+    """The uneven-batch workload: lmax drawn from {31, 47, 63, 95, 127}, nr
+    from {33, 49, 65, 97}, grid 3(L+1)/2 x 3(L+1), radius ratio U[0.2, 0.5]
+    (numpy seed 3)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    shells = []
+    for _ in range(nshell):
+        L = int(rng.choice([31, 47, 63, 95, 127]))
+        nr = int(rng.choice([33, 49, 65, 97]))
+        shells.append((L, nr, 3 * (L + 1) // 2, 3 * (L + 1), float(rng.uniform(0.2, 0.5))))
+    return shells
+
+
+def batch_c3_prec(shells, steps, nstreams, prec, with_streams=True):
+    """Aggregate shell-steps/s of the shells advanced (a) one after another by
+    their own step graphs, (b) by one batch graph of fused launches (one grid
+    per kernel form and stage over the shells of each lane; synthesis tiles
+    from the tuning table, lane count from the lane autotuner), (c)
+    with_streams: by one batch graph of per-shell branches over nstreams
+    streams; each timed over `steps` steps after the bootstrap and both
+    captures.  States of (b) and (c) are compared bit for bit with (a)."""
     import numpy as np
 
     import gdhip
 
-    rng = np.random.default_rng(3)
-    shells = []
-    for k in range(nshell):
-        L = int(rng.choice([31, 47, 63, 95, 127]))
-        nr = int(rng.choice([33, 49, 65, 97]))
-        shells.append((L, nr, 3 * (L + 1) // 2, 3 * (L + 1), float(rng.uniform(0.2, 0.5))))
-    groups = {"batch": [], "sequential": []}
+    seq, bat = [], []
     for k, (L, nr, nlat, nlon, rr) in enumerate(shells):
         st = gdhip.seeded_state(500 + k, L, nr, amp=1e-3)
-        for g in groups.values():
-            s = gdhip.CNAB2Step(L, nlat, nlon, nr, rratio=rr, Ra=500.0, dt=1e-5, precision="fp64")
+        for g in (seq, bat):
+            s = gdhip.CNAB2Step(L, nlat, nlon, nr, rratio=rr, Ra=500.0, dt=1e-5, precision=prec)
             s.set_state_complex(st)
             g.append(s)
-    batch = gdhip.StepBatch(groups["batch"], nstreams=nstreams)
-    batch.run(3)  # bootstrap, graph capture of both parities
-    for s in groups["sequential"]:
-        s.run(3, graph=True)
-    batch.synchronize()
-    t0 = time.perf_counter()
-    batch.run(steps)
-    batch.synchronize()
-    tb = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        for s in groups["sequential"]:
-            s.run(1, graph=True)
-    groups["sequential"][0].synchronize()
-    ts = time.perf_counter() - t0
-    same = all(np.array_equal(a.get_state_complex(), b.get_state_complex())
-               for a, b in zip(groups["batch"], groups["sequential"]))
-    batch.close()
-    return {"workload": "%d uneven shells, lmax {31,47,63,95,127}, nr {33,49,65,97}, 3(L+1)/2 x 3(L+1), "
-                        "FP64 steps" % nshell,
-            "shells_lmax_nr": [[L, nr] for L, nr, _, _, _ in shells],
-            "batch_shell_steps_per_s": round(nshell * steps / tb, 1),
-            "sequential_shell_steps_per_s": round(nshell * steps / ts, 1),
-            "batch_over_sequential": round(ts / tb, 3), "streams": nstreams, "steps": steps,
-            "states_equal_bitwise": same}
+
+    def timed(fn):
+        seq[0].synchronize()
+        t0 = time.perf_counter()
+        fn()
+        seq[0].synchronize()
+        return time.perf_counter() - t0
+
+    def seq_run(k):
+        for _ in range(k):
+            for s in seq:
+                s.run(1, graph=True)
+
+    def same():
+        return all(np.array_equal(a.get_state_complex(), b.get_state_complex()) for a, b in zip(bat, seq))
+
+    n = len(shells)
+    fused = gdhip.StepBatch(bat, nstreams=nstreams, mode="fused")
+    lanes = fused.tune_lanes(3)  # real steps of the batch
+    fused.run(2)  # captures with the chosen lanes
+    seq_run(bat[0].step_count - seq[0].step_count)
+    tf = timed(lambda: fused.run(steps))
+    ts = timed(lambda: seq_run(steps))
+    out = {"sequential_shell_steps_per_s": round(n * steps / ts, 1),
+           "fused_shell_steps_per_s": round(n * steps / tf, 1),
+           "fused_over_sequential": round(ts / tf, 3), "fused_equal_bitwise": same(), "tiles": fused.tiles(),
+           "lanes": lanes, "lanes_ms_per_step": [round(v, 4) for v in fused.lanes_ms]}
+    fused.close()
+    if with_streams:
+        streams = gdhip.StepBatch(bat, nstreams=nstreams, mode="streams")
+        streams.run(2)  # both captures
+        seq_run(2)
+        tb = timed(lambda: streams.run(steps))
+        seq_run(steps)
+        out.update({"streams_shell_steps_per_s": round(n * steps / tb, 1),
+                    "streams_over_sequential": round(ts / tb, 3), "fused_over_streams": round(tb / tf, 3),
+                    "streams_equal_bitwise": same()})
+        streams.close()
+    return out
+
+
+def batch_c3(nshell, steps, nstreams, precs=("fp64", "fp32")):
+    """Uneven batch: nshell independent shells of the workload c3_shells
+    describes, (l+1)^-2 states of amplitude 1e-3, in FP64 and in FP32 (the
+    mixed step).  The stream-branch form of FP32 shells stays off the GPU (a
+    host fault in an earlier form of its capture): fused and sequential only."""
+    shells = c3_shells(nshell)
+    out = {"workload": "%d uneven shells, lmax {31,47,63,95,127}, nr {33,49,65,97}, 3(L+1)/2 x 3(L+1)" % nshell,
+           "shells_lmax_nr": [[L, nr] for L, nr, _, _, _ in shells], "streams": nstreams, "steps": steps}
+    for prec in precs:
+        out[prec] = batch_c3_prec(shells, steps, nstreams, prec, with_streams=prec == "fp64")
+    return out
 
 
 def restart_io(step, reps):

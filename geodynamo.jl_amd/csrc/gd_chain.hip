@@ -262,6 +262,7 @@ template <int BW, int D, typename TS, bool DIRECT, typename IX>
 __global__ void __launch_bounds__(64) cnab2_chain_batch_kernel(BatchRef<ChainArgs> b)
 {
     const int sh = b.shell_of_block[blockIdx.x];
+    if (sh == BATCH_PAD) return;
     const ChainArgs &c = b.args[sh];
     int bx, by, bz;
     chain_decode(c, blockIdx.x - b.block_off[sh], bx, by, bz);
@@ -485,6 +486,7 @@ template <int BW, int D, typename TS>
 __global__ void __launch_bounds__(128, 3) cnab2_twist_batch_kernel(BatchRef<ChainArgs> b)
 {
     const int sh = b.shell_of_block[blockIdx.x];
+    if (sh == BATCH_PAD) return;
     const ChainArgs &c = b.args[sh];
     int bx, by, bz;
     chain_decode(c, blockIdx.x - b.block_off[sh], bx, by, bz);
@@ -513,6 +515,61 @@ int gd_cnab2_chain_supported(int n, int bw)
            sizeof(double) * (size_t)((((2 * bw + 1) + 2) & ~1) + 2 * (2 * bw + 1) + 2) * n <= 64 * 1024;
 }
 
+void gd_cnab2_chain_form(const ChainArgs &c, ChainForm *f)
+{
+    f->lds_lane = sizeof(double) * (size_t)((((2 * c.bw + 1) + 2) & ~1) + 2 * (2 * c.bw + 1) + 2) * c.n;
+    f->direct = c.synV && c.syn_tang;
+    f->lds_twist = sizeof(double) * ((size_t)(2 * (((2 * c.bw + 1) + 2) & ~1) + 1) * c.n + 2 * (size_t)c.bw * 64);
+    // row, synthesis-column and table offsets from the per-column base pointers
+    // fit 32 bits (fewer scalar instructions per level); the twisted kernel needs it
+    f->off32 = !c.off64 && (size_t)SYNV_COLS * c.nlm * c.n < ((size_t)1 << 32) &&
+               (size_t)(2 * c.bw + 1) * c.n * c.nl < ((size_t)1 << 32);
+    f->twist = f->direct && c.ULt && c.rdiag2 && c.Rint && c.bw <= 3 && c.n >= 2 * c.bw + 2 &&
+               f->lds_twist <= 64 * 1024 && f->off32;
+}
+
+int gd_cnab2_chain_blocks(const ChainArgs &c)
+{
+    const int nfield = c.nf ? c.nf : (c.c_in ? 6 : 5);
+    return ((c.nlm + 63) / 64) * nfield * 2;
+}
+
+// several shells' column solves in one grid, every shell in form `fm` (as
+// gd_cnab2_chain_form gives it for each of them); lds: the largest shell's
+int gd_launch_cnab2_chain_batch(hipStream_t st, const BatchRef<ChainArgs> &b, int nblocks, int bw, const ChainForm &fm,
+                                size_t lds, bool syn_fp64)
+{
+    if (nblocks <= 0) return GD_OK;
+    GD_CHECK(lds <= 64 * 1024, GD_ERR_ARG, "cnab2_chain batch: %zu B of LDS", lds);
+    if (fm.twist) {
+#define GD_TWISTB(B)                                                                                                   \
+    if (bw == B) {                                                                                                     \
+        if (syn_fp64) hipLaunchKernelGGL((cnab2_twist_batch_kernel<B, 4, double>), dim3(nblocks), dim3(128), lds, st, b); \
+        else hipLaunchKernelGGL((cnab2_twist_batch_kernel<B, 4, float>), dim3(nblocks), dim3(128), lds, st, b);        \
+        return gd_check_launch("cnab2_twist_batch_kernel");                                                            \
+    }
+        GD_TWISTB(1) GD_TWISTB(2) GD_TWISTB(3)
+#undef GD_TWISTB
+        GD_CHECK(false, GD_ERR_ARG, "cnab2_chain batch: twisted form with bw=%d", bw);
+    }
+    GD_CHECK(fm.direct, GD_ERR_ARG, "cnab2_chain batch: the tangential (direct) form only");
+#define GD_CHAINB_IX(B, IX)                                                                                            \
+    {                                                                                                                  \
+        if (syn_fp64) hipLaunchKernelGGL((cnab2_chain_batch_kernel<B, 4, double, true, IX>), dim3(nblocks), dim3(64), lds, st, b); \
+        else hipLaunchKernelGGL((cnab2_chain_batch_kernel<B, 4, float, true, IX>), dim3(nblocks), dim3(64), lds, st, b);   \
+    }
+#define GD_CHAINB(B)                                                                                                   \
+    if (bw == B) {                                                                                                     \
+        if (fm.off32) GD_CHAINB_IX(B, unsigned) else GD_CHAINB_IX(B, size_t)                                           \
+        return gd_check_launch("cnab2_chain_batch_kernel");                                                            \
+    }
+    GD_CHAINB(1) GD_CHAINB(2) GD_CHAINB(3) GD_CHAINB(4) GD_CHAINB(5) GD_CHAINB(6) GD_CHAINB(7)
+#undef GD_CHAINB
+#undef GD_CHAINB_IX
+    gd_set_error("cnab2_chain batch: no kernel for bw=%d", bw);
+    return GD_ERR_ARG;
+}
+
 int gd_launch_cnab2_chain(hipStream_t st, const ChainArgs &c, bool syn_fp64)
 {
     GD_CHECK(c.st_in != c.st_out, GD_ERR_ARG, "cnab2_chain: in and out state must differ");
@@ -526,15 +583,12 @@ int gd_launch_cnab2_chain(hipStream_t st, const ChainArgs &c, bool syn_fp64)
     GD_CHECK(!c.c_in || (c.c_out && c.c_nl && c.c_np && (!c.synV || c.syn_tang)), GD_ERR_ARG,
              "cnab2_chain: incomplete composition buffers");
     dim3 grid((c.nlm + 63) / 64, nfield, 2);
-    const size_t lds = sizeof(double) * (size_t)((((2 * c.bw + 1) + 2) & ~1) + 2 * (2 * c.bw + 1) + 2) * c.n;
+    ChainForm fm;
+    gd_cnab2_chain_form(c, &fm);
+    const size_t lds = fm.lds_lane, tlds = fm.lds_twist;
     GD_CHECK(lds <= 64 * 1024, GD_ERR_ARG, "cnab2_chain: n=%d bw=%d needs %zu B of LDS", c.n, c.bw, lds);
-    const bool direct = c.synV && c.syn_tang;
-    const size_t tlds = sizeof(double) * ((size_t)(2 * (((2 * c.bw + 1) + 2) & ~1) + 1) * c.n + 2 * (size_t)c.bw * 64);
-    // row, synthesis-column and table offsets from the per-column base pointers
-    // fit 32 bits (fewer scalar instructions per level); the twisted kernel needs it
-    const bool off32 = !c.off64 && (size_t)SYNV_COLS * c.nlm * c.n < ((size_t)1 << 32) &&
-                       (size_t)(2 * c.bw + 1) * c.n * c.nl < ((size_t)1 << 32);
-    if (direct && c.ULt && c.rdiag2 && c.Rint && c.bw <= 3 && c.n >= 2 * c.bw + 2 && tlds <= 64 * 1024 && off32) {
+    const bool direct = fm.direct, off32 = fm.off32;
+    if (fm.twist) {
         // twisted form: 64 columns per block, one wave per half
         const dim3 tgrid((c.nlm + 63) / 64, grid.y, 2);
 #define GD_TWIST(B)                                                                                              \

@@ -144,12 +144,16 @@ struct FftPlan {
 
 // A batched launch: one grid over the blocks that several shells' launches of
 // one kernel form (the same compile-time form for all); block b belongs to
-// shell shell_of_block[b] and is that shell's block b - block_off[shell],
-// and args[shell] are that shell's arguments (device arrays).
+// shell shell_of_block[b] and is that shell's block b - block_off[shell] of
+// block_cnt[shell], and args[shell] are that shell's arguments (device
+// arrays).  Each shell's run of blocks starts at a multiple of 8 (the XCD
+// count, so a shell's block i runs on XCD i % 8 as in its own launch); the
+// padding blocks between runs carry shell BATCH_PAD and exit at once.
+constexpr unsigned short BATCH_PAD = 0xFFFF;
 template <class A> struct BatchRef {
     const A *args;
     const unsigned short *shell_of_block;
-    const int *block_off; // nshell + 1 entries
+    const int *block_off, *block_cnt;
 };
 
 // launch helper: returns GD_ERR_HIP on launch failure

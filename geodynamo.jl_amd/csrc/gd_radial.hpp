@@ -62,6 +62,15 @@ int gd_chain_twist_tables(const double *S, const double *LU, int n, int bw, int 
 // rdiag[(k*n + i)*nl + l] = 1 / LU_t[k][(i*W + bw)][l] for nk diffusivities
 int gd_launch_recip_diag(hipStream_t st, const double *LUt, int nk, int n, int bw, int nl, double *rdiag);
 int gd_launch_cnab2_chain(hipStream_t st, const ChainArgs &c, bool syn_fp64);
+// the kernel form gd_launch_cnab2_chain picks for c, and its LDS bytes
+struct ChainForm {
+    bool twist, off32, direct;
+    size_t lds_lane, lds_twist;
+};
+void gd_cnab2_chain_form(const ChainArgs &c, ChainForm *f);
+int gd_cnab2_chain_blocks(const ChainArgs &c);
+int gd_launch_cnab2_chain_batch(hipStream_t st, const BatchRef<ChainArgs> &b, int nblocks, int bw, const ChainForm &fm,
+                                size_t lds, bool syn_fp64);
 int gd_launch_energies(hipStream_t st, const double *state, const int *l_of_idx, int nlm, int n, double *part,
                        int nblk, double *out3);
 int gd_launch_sqdiff(hipStream_t st, long n, const double *ar, const double *ai, const double *br, const double *bi,

@@ -409,8 +409,9 @@ __global__ void __launch_bounds__(64 * NW) synth_kernel(SynthArgs a)
 template <typename T, bool VEC, int NF, int NW, int KC, bool VL, int NG = 0, int MM = 0>
 __global__ void __launch_bounds__(64 * NW) synth_batch_kernel(BatchRef<SynthArgs> b)
 {
-    const int sh = b.shell_of_block[blockIdx.x], o = b.block_off[sh];
-    synth_body<T, VEC, NF, NW, KC, VL, NG, MM>(b.args[sh], blockIdx.x - o, b.block_off[sh + 1] - o);
+    const int sh = b.shell_of_block[blockIdx.x];
+    if (sh == BATCH_PAD) return;
+    synth_body<T, VEC, NF, NW, KC, VL, NG, MM>(b.args[sh], blockIdx.x - b.block_off[sh], b.block_cnt[sh]);
 }
 
 // ---------------------------------------------------------------- analysis
@@ -815,8 +816,9 @@ __global__ void __launch_bounds__(64 * ana_waves<T>()) ana_res_kernel(AnaArgs a)
 template <typename T, bool VEC, int NF, int KN, int MM = 0>
 __global__ void __launch_bounds__(64 * ana_waves<T>()) ana_res_batch_kernel(BatchRef<AnaArgs> b)
 {
-    const int sh = b.shell_of_block[blockIdx.x], o = b.block_off[sh];
-    ana_res_body<T, VEC, NF, KN, MM>(b.args[sh], blockIdx.x - o, b.block_off[sh + 1] - o);
+    const int sh = b.shell_of_block[blockIdx.x];
+    if (sh == BATCH_PAD) return;
+    ana_res_body<T, VEC, NF, KN, MM>(b.args[sh], blockIdx.x - b.block_off[sh], b.block_cnt[sh]);
 }
 
 template <typename T, bool VEC, int NF, int KN> constexpr size_t ana_res_lds()
@@ -960,6 +962,132 @@ template <typename T> int gd_launch_ana(hipStream_t st, const AnaArgs &a, bool v
     b.maxgroup = maxgroup;
     return launch_ana<T>(st, b, vec, maxgroup);
 }
+
+// ---------------------------------------------------------------- batched step launches
+// The tangential step's synthesis (u as a full field, grad T as a gradient
+// field) in one grid over several shells, with a tile chosen per group:
+//   tile 0: 3 waves (8 levels), K chunks of 16 degrees (the step's own form)
+//   tile 1: 3 waves, K chunks of 32 degrees
+//   tile 2: 6 waves (16 levels), K chunks of 16 (FP32 only: the FP64 output
+//           tile would exceed 64 KB of LDS)
+static int synth_tile_nw(int tile) { return tile == 2 ? 6 : 3; }
+static int synth_tile_kc(int tile) { return tile == 1 ? 32 : 16; }
+
+int gd_synth_tang_blocks(const SynthArgs &a, int tile)
+{
+    const int nw = synth_tile_nw(tile), lt = 16 * nw / 6; // 6 columns per level, two levels per 12-column group
+    return ((a.nlat2 + TM - 1) / TM) * ((a.nr + lt - 1) / lt) * (a.mmax + 1);
+}
+
+bool gd_synth_tile_ok(int tile, bool fp64) { return tile == 0 || tile == 1 || (tile == 2 && !fp64); }
+
+template <typename T, int NW, int KC, int MM>
+static void synth_batch_launch(hipStream_t st, const BatchRef<SynthArgs> &b, int nblocks)
+{
+    constexpr size_t lds = synth_lds_bytes<T, true, 1, NW, KC, 1>();
+    hipLaunchKernelGGL((synth_batch_kernel<T, true, 1, NW, KC, true, 1, MM>), dim3(nblocks), dim3(64 * NW), lds, st, b);
+}
+
+template <typename T>
+int gd_launch_synth_tang_batch(hipStream_t st, const BatchRef<SynthArgs> &b, int nblocks, int tile, int mm)
+{
+    GD_CHECK(gd_synth_tile_ok(tile, sizeof(T) == 8), GD_ERR_ARG, "synth batch: tile %d", tile);
+    GD_CHECK(!mm || sizeof(T) == 4, GD_ERR_ARG, "synth batch: the BF16x3 contraction takes FP32 operands");
+    if (nblocks <= 0) return GD_OK;
+#define GD_SB(NW, KC)                                                                   \
+    do {                                                                                \
+        if constexpr (sizeof(T) == 4) {                                                 \
+            if (mm) synth_batch_launch<T, NW, KC, 1>(st, b, nblocks);                   \
+            else synth_batch_launch<T, NW, KC, 0>(st, b, nblocks);                      \
+        } else {                                                                        \
+            synth_batch_launch<T, NW, KC, 0>(st, b, nblocks);                           \
+        }                                                                               \
+    } while (0)
+    if (tile == 0) GD_SB(3, 16);
+    else if (tile == 1) GD_SB(3, 32);
+    else if constexpr (sizeof(T) == 4) GD_SB(6, 16);
+#undef GD_SB
+    return gd_check_launch("synth_batch_kernel");
+}
+template int gd_launch_synth_tang_batch<float>(hipStream_t, const BatchRef<SynthArgs> &, int, int, int);
+template int gd_launch_synth_tang_batch<double>(hipStream_t, const BatchRef<SynthArgs> &, int, int, int);
+
+// one shell's synthesis in tile form `tile` (the autotuner times these alone)
+template <typename T> int gd_launch_synth_tang_tile(hipStream_t st, const SynthArgs &a, int tile)
+{
+    GD_CHECK(gd_synth_tile_ok(tile, sizeof(T) == 8), GD_ERR_ARG, "synth: tile %d", tile);
+    GD_CHECK(a.nf == 1 && a.ng == 1, GD_ERR_ARG, "synth tile: the tangential form (nf=1, ng=1)");
+    if (tile == 0) launch_synth<T, true, 1, 3, 16, 1>(st, a);
+    else if (tile == 1) launch_synth<T, true, 1, 3, 32, 1>(st, a);
+    else if constexpr (sizeof(T) == 4) launch_synth<T, true, 1, 6, 16, 1>(st, a);
+    return gd_check_launch("synth_kernel<tile>");
+}
+template int gd_launch_synth_tang_tile<float>(hipStream_t, const SynthArgs &, int);
+template int gd_launch_synth_tang_tile<double>(hipStream_t, const SynthArgs &, int);
+
+// a shell's tangential synthesis can join a batched launch: the checks of
+// its own launch, and the 16-byte table form (VL) that the batched kernel has
+int gd_synth_tang_batchable(const SynthArgs &a)
+{
+    GD_TRY(synth_check(a, "synth batch"));
+    GD_CHECK(a.nf == 1 && a.ng == 1 && a.slot0 + 16 <= a.nslot, GD_ERR_ARG, "synth batch: nf=%d ng=%d slots %d/%d",
+             a.nf, a.ng, a.slot0, a.nslot);
+    GD_CHECK(a.nlat2 % 4 == 0 && (uintptr_t)a.tab0 % 16 == 0 && (uintptr_t)a.tab1 % 16 == 0, GD_ERR_ARG,
+             "synth batch: nlat/2=%d needs 16-byte table rows", a.nlat2);
+    return GD_OK;
+}
+
+// node tile of the table-resident analysis for nlat/2 (0: not in one chunk)
+int gd_ana_kn(int nlat2, bool fp64)
+{
+    if (nlat2 <= 32) return 32;
+    if (nlat2 <= 48) return 48;
+    if (nlat2 <= 96) return 96;
+    if (!fp64 && nlat2 <= 128) return 128;
+    return 0;
+}
+
+int gd_ana_blocks(const AnaArgs &a) { return ((a.maxgroup + TM - 1) / TM) * 2 * (a.mmax + 1); }
+
+template <typename T, bool VEC, int KN, int MM>
+static void ana_batch_launch(hipStream_t st, const BatchRef<AnaArgs> &b, int nblocks)
+{
+    constexpr size_t lds = ana_res_lds<T, VEC, 1, KN>();
+    hipLaunchKernelGGL((ana_res_batch_kernel<T, VEC, 1, KN, MM>), dim3(nblocks), dim3(64 * ana_waves<T>()), lds, st, b);
+}
+
+template <typename T, bool VEC, int KN> static void ana_batch_mm(hipStream_t st, const BatchRef<AnaArgs> &b, int nblocks, int mm)
+{
+    if constexpr (sizeof(T) == 4) {
+        if (mm) {
+            ana_batch_launch<T, VEC, KN, 1>(st, b, nblocks);
+            return;
+        }
+    }
+    ana_batch_launch<T, VEC, KN, 0>(st, b, nblocks);
+}
+
+template <typename T>
+int gd_launch_ana_batch(hipStream_t st, const BatchRef<AnaArgs> &b, int nblocks, bool vec, int kn, int mm)
+{
+    GD_CHECK(!mm || sizeof(T) == 4, GD_ERR_ARG, "ana batch: the BF16x3 contraction takes FP32 operands");
+    if (nblocks <= 0) return GD_OK;
+    switch (kn) {
+    case 32: vec ? ana_batch_mm<T, true, 32>(st, b, nblocks, mm) : ana_batch_mm<T, false, 32>(st, b, nblocks, mm); break;
+    case 48: vec ? ana_batch_mm<T, true, 48>(st, b, nblocks, mm) : ana_batch_mm<T, false, 48>(st, b, nblocks, mm); break;
+    case 96: vec ? ana_batch_mm<T, true, 96>(st, b, nblocks, mm) : ana_batch_mm<T, false, 96>(st, b, nblocks, mm); break;
+    case 128:
+        if constexpr (sizeof(T) == 4) {
+            vec ? ana_batch_mm<T, true, 128>(st, b, nblocks, mm) : ana_batch_mm<T, false, 128>(st, b, nblocks, mm);
+            break;
+        }
+        [[fallthrough]];
+    default: GD_CHECK(false, GD_ERR_ARG, "ana batch: node tile %d", kn);
+    }
+    return gd_check_launch("ana_res_batch_kernel");
+}
+template int gd_launch_ana_batch<float>(hipStream_t, const BatchRef<AnaArgs> &, int, bool, int, int);
+template int gd_launch_ana_batch<double>(hipStream_t, const BatchRef<AnaArgs> &, int, bool, int, int);
 
 // ====================================================================== FFT ring kernel
 namespace {
@@ -1459,7 +1587,9 @@ template <typename T, int MODE, int NC> __global__ void __launch_bounds__(256) r
 // blocks are its (node, level) pairs, node fastest
 template <typename T, int MODE, int NC> __global__ void __launch_bounds__(256) ring_batch_kernel(BatchRef<RingArgs> b)
 {
-    const int sh = b.shell_of_block[blockIdx.x], loc = blockIdx.x - b.block_off[sh];
+    const int sh = b.shell_of_block[blockIdx.x];
+    if (sh == BATCH_PAD) return;
+    const int loc = blockIdx.x - b.block_off[sh];
     const RingArgs &a = b.args[sh];
     ring_body<T, MODE, NC>(a, loc % a.nlat2, loc / a.nlat2);
 }
@@ -1484,7 +1614,7 @@ template <typename T, int MODE> static void ring_dispatch(hipStream_t st, dim3 g
     }
 }
 
-template <typename T> int gd_launch_ring(hipStream_t st, int mode, const RingArgs &a)
+static int ring_check(int mode, const RingArgs &a, int *nf_out)
 {
     const int nthr = RING_THREADS;
     int nf = (mode == RING_STEP) ? 15 : (mode == RING_STEP_TANG) ? 4 : (mode == RING_STEP_TANG_C) ? 6
@@ -1497,6 +1627,24 @@ template <typename T> int gd_launch_ring(hipStream_t st, int mode, const RingArg
         long per = ((long)nf * (a.nlon / R) + nthr - 1) / nthr;
         GD_CHECK(per * R <= MAXC, GD_ERR_ARG, "ring: nlon=%d too large for %d fields", a.nlon, nf);
     }
+    *nf_out = nf;
+    return GD_OK;
+}
+
+// a shell's tangential ring pass can join a batched launch (its own launch's checks)
+int gd_ring_tang_batchable(const RingArgs &a, bool fp64)
+{
+    int nf;
+    GD_TRY(ring_check(RING_STEP_TANG, a, &nf));
+    GD_CHECK((fp64 ? 16 : 8) * ((size_t)ring_ld(a.nlon) * nf + a.nlon) <= 160 * 1024, GD_ERR_ARG,
+             "ring batch: nlon=%d needs too much LDS", a.nlon);
+    return GD_OK;
+}
+
+template <typename T> int gd_launch_ring(hipStream_t st, int mode, const RingArgs &a)
+{
+    int nf;
+    GD_TRY(ring_check(mode, a, &nf));
     size_t lds = sizeof(C2<T>) * ((size_t)ring_ld(a.nlon) * nf + a.nlon);
     GD_CHECK(lds <= 160 * 1024, GD_ERR_ARG, "ring: LDS %zu too large", lds);
     dim3 grid(a.nlat2, a.nr);
@@ -1510,6 +1658,34 @@ template <typename T> int gd_launch_ring(hipStream_t st, int mode, const RingArg
     }
     return gd_check_launch("ring_kernel");
 }
+
+// The tangential step's ring pass in one grid over several shells of one
+// ring length (the compile-time FFT plan of nlon where there is one)
+template <typename T, int NC>
+static void ring_batch_launch(hipStream_t st, const BatchRef<RingArgs> &b, int nblocks, size_t lds)
+{
+    hipLaunchKernelGGL((ring_batch_kernel<T, RING_STEP_TANG, NC>), dim3(nblocks), dim3(RING_THREADS), lds, st, b);
+}
+
+template <typename T> int gd_launch_ring_tang_batch(hipStream_t st, const BatchRef<RingArgs> &b, int nblocks, int nlon)
+{
+    if (nblocks <= 0) return GD_OK;
+    const size_t lds = sizeof(C2<T>) * ((size_t)ring_ld(nlon) * 4 + nlon);
+    GD_CHECK(lds <= 160 * 1024, GD_ERR_ARG, "ring batch: LDS %zu too large", lds);
+    switch (nlon) {
+    case 48: ring_batch_launch<T, 48>(st, b, nblocks, lds); break;
+    case 96: ring_batch_launch<T, 96>(st, b, nblocks, lds); break;
+    case 144: ring_batch_launch<T, 144>(st, b, nblocks, lds); break;
+    case 192: ring_batch_launch<T, 192>(st, b, nblocks, lds); break;
+    case 288: ring_batch_launch<T, 288>(st, b, nblocks, lds); break;
+    case 384: ring_batch_launch<T, 384>(st, b, nblocks, lds); break;
+    case 768: ring_batch_launch<T, 768>(st, b, nblocks, lds); break;
+    default: ring_batch_launch<T, 0>(st, b, nblocks, lds); break;
+    }
+    return gd_check_launch("ring_batch_kernel");
+}
+template int gd_launch_ring_tang_batch<float>(hipStream_t, const BatchRef<RingArgs> &, int, int);
+template int gd_launch_ring_tang_batch<double>(hipStream_t, const BatchRef<RingArgs> &, int, int);
 
 // ====================================================================== permutations
 template <typename T>

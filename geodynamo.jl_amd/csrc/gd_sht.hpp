@@ -44,7 +44,7 @@ struct AnaArgs {
     double *out[4];
     int scale[4];         // 0: /l(l+1) (0 at l=0), 1: x1, 2: x(-1)
     // table-resident analysis in node chunks (set by its launcher): nodes
-    // k0 .. k0 + nk - 1 (nk = 0: all), accum = 1 adds to out instead of storing
+    // k0 .. k0 + nk - 1, accum = 1 adds to out instead of storing
     int k0, nk, accum;
     int mm;               // 1: BF16x3 split-precision contraction (FP32 operands), 0: native MFMA
 };
@@ -95,3 +95,22 @@ int gd_launch_pack(hipStream_t st, const double *re, const double *im, const int
                    int cin, int c0, T *out);
 int gd_launch_unpack(hipStream_t st, const double *src, const int *idx_of_lmaj, int nlm, int nr, double *dst);
 int gd_launch_perm(hipStream_t st, const double *src, const int *map, int nlm, int nr, double *dst);
+
+// ---- batched step launches (gd_sht.hip): one grid over several shells'
+// launches of the tangential step's kernels, all in one compile-time form.
+// Block counts of one shell: gd_synth_tang_blocks (tile 0, 1, or 2 = FP32
+// only), nlat/2 * nr for the ring, gd_ana_blocks (args with maxgroup set) for
+// the analyses, whose node tile gd_ana_kn gives (0: several node chunks, not
+// batched).
+int gd_synth_tang_blocks(const SynthArgs &a, int tile);
+bool gd_synth_tile_ok(int tile, bool fp64);
+template <typename T>
+int gd_launch_synth_tang_batch(hipStream_t st, const BatchRef<SynthArgs> &b, int nblocks, int tile, int mm);
+template <typename T> int gd_launch_synth_tang_tile(hipStream_t st, const SynthArgs &a, int tile);
+template <typename T> int gd_launch_ring_tang_batch(hipStream_t st, const BatchRef<RingArgs> &b, int nblocks, int nlon);
+int gd_synth_tang_batchable(const SynthArgs &a);
+int gd_ring_tang_batchable(const RingArgs &a, bool fp64);
+int gd_ana_kn(int nlat2, bool fp64);
+int gd_ana_blocks(const AnaArgs &a);
+template <typename T>
+int gd_launch_ana_batch(hipStream_t st, const BatchRef<AnaArgs> &b, int nblocks, bool vec, int kn, int mm);

@@ -186,16 +186,14 @@ static int ensure_syn(gd_step *s)
     return GD_OK;
 }
 
-template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEvent_t *ev, bool side)
+// The transform arguments common to both step forms: the synthesis a (the
+// vector columns synV into FB), the ring pass r (FB into GB, the scalar
+// analysis's products into GB2) and the analysis g (GB into nl_out).
+static void base_args(gd_step *s, SynthArgs &a, RingArgs &r, AnaArgs &g)
 {
-    hipStream_t st = s->ctx->stream;
-    GD_MARK(0);
     gd_sht_plan *p = &s->sht;
     const int nr = s->d.nr, nlm = s->nlm;
-    if (!s->syn_ok) GD_TRY(spectral_pre<T>(s));
-    GD_MARK(1);
-
-    SynthArgs a{};
+    a = SynthArgs{};
     a.mm = p->prec == GD_PREC_BF16X3;
     a.moff = p->d_moff.as<int>();
     a.ne = p->d_ne.as<int>();
@@ -209,7 +207,7 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEven
     a.tab0 = gd_sht_tab(p, 1);
     a.tab1 = gd_sht_tab(p, 2);
     a.slot0 = 0;
-    RingArgs r{};
+    r = RingArgs{};
     gd_sht_fill_ring(p, r, nr);
     r.fb = s->fb.p;
     r.gb = s->gb.p;
@@ -218,7 +216,7 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEven
     r.gb2 = s->gb2.p;
     r.ro = s->d.Pm / s->d.E;
     r.coriolis_index = s->d.coriolis_index;
-    AnaArgs g{};
+    g = AnaArgs{};
     g.mm = p->prec == GD_PREC_BF16X3;
     g.gb = s->gb.p;
     g.moff = p->d_moff.as<int>();
@@ -228,66 +226,125 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEven
     g.nr = nr;
     g.nlm = nlm;
     g.mmax = p->mmax;
+    g.maxgroup = (p->lmax + 2) / 2;
+}
+
+// The tangential step's arguments: u (S = u_pol, T = u_tor) and grad T
+// (S = T/r) only; FB records of 16 slots, 24 with the composition's gradient
+// (S = C/r) in slots 16..23.  gv: the vector analysis of the force, gs: the
+// scalar analysis of -u.grad T (and -u.grad C), into nl_out; *mode: the ring
+// pass's form.
+static void tang_args(gd_step *s, double *nl_out, SynthArgs &a, RingArgs &r, AnaArgs &gv, AnaArgs &gs, int *mode)
+{
+    gd_sht_plan *p = &s->sht;
+    const size_t fs = s->fs;
+    AnaArgs g;
+    base_args(s, a, r, g);
+    const int nslot = s->comp ? 24 : 16;
+    a.nslot = nslot;
+    if (s->synth_grad) {
+        // u as a full field, grad T (and grad C) as gradient fields: their
+        // zero toroidal columns are not contracted
+        a.nf = 1;
+        a.ng = s->comp ? 2 : 1;
+        static const int gmap[8] = {0, 1, 2, 3, 16, 17, SYNV_COLS - 2, SYNV_COLS - 1};
+        for (int k = 0; k < 20; ++k) a.colmap[k] = k < 4 + 2 * a.ng ? gmap[k] : -1;
+    } else {
+        a.nf = s->comp ? 3 : 2;
+        static const int vmap[12] = {0, 1, 2, 3, 16, 17, -1, -1, SYNV_COLS - 2, SYNV_COLS - 1, -1, -1};
+        for (int k = 0; k < 20; ++k) a.colmap[k] = k < 4 * a.nf ? vmap[k] : -1;
+    }
+    r.nslot_f = nslot;
+    // with composition the ring also takes grad C (fields 4, 5) and forms
+    // -u.grad C (compute_composition_nonlinear!, compositional.jl:219-260)
+    // as a fourth product; GB records of 12 slots, 16 with it
+    r.nfield = s->comp ? 6 : 4;
+    r.nslot_g = 8;                    // the force (products 0, 1)
+    r.np1 = 2;
+    r.nslot_g2 = s->comp ? 8 : 4;     // -u.grad T (and -u.grad C)
+    static const int fn[6] = {0, 2, 8, 10, 16, 18}, fsou[6] = {4, 6, 12, 14, 20, 22};
+    for (int F = 0; F < r.nfield; ++F) {
+        r.fn_off[F] = fn[F];
+        r.fs_off[F] = fsou[F];
+    }
+    *mode = s->comp ? RING_STEP_TANG_C : RING_STEP_TANG;
+    // force -> (pol, tor) of velocity nl (velocity.jl:331); the magnetic nl
+    // is u x B = 0 here and its planes stay zero
+    gv = g;
+    gv.nslot_g = r.nslot_g;
+    gv.nf = 1;
+    gv.gslot0 = 0;
+    gv.tab0 = gd_sht_tab(p, 1);
+    gv.tab1 = gd_sht_tab(p, 2);
+    gv.out[0] = nl_out + 2 * fs; // S(adv) -> u_pol
+    gv.out[1] = nl_out + 0 * fs; // T(adv) -> u_tor
+    gv.scale[0] = 0;
+    gv.scale[1] = 0;
+    gs = g;
+    gs.gb = s->gb2.p;
+    gs.nslot_g = r.nslot_g2;
+    gs.gslot0 = 0;
+    gs.nf = 1;
+    gs.tab0 = gd_sht_tab(p, 0);
+    gs.tab1 = nullptr;
+    gs.out[0] = nl_out + 8 * fs; // -u.grad T -> T
+    gs.scale[0] = 1;
+    if (s->comp) { // and -u.grad C -> the composition's nl (GB slots 12..15)
+        gs.nf = 2;
+        gs.out[1] = s->cnl[s->step & 1].as<double>() + 8 * fs;
+        gs.scale[1] = 1;
+    }
+}
+
+template <typename T> static int radial_force(gd_step *s, hipStream_t st, double *nl_out)
+{
+    if (!s->d.radial_force) return GD_OK;
+    // after the vector analysis wrote nl(u_pol), on its stream
+    hipLaunchKernelGGL(buoyancy_kernel, dim3((s->nlm + 255) / 256, s->d.nr), dim3(256), 0, st, cur_state(s),
+                       s->sht.d_l_of_idx.as<int>(), s->rtab.as<double>(), gd_step_buoyancy_factor(s), s->nlm,
+                       s->d.nr, nl_out);
+    return gd_check_launch("buoyancy_kernel");
+}
+
+template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEvent_t *ev, bool side)
+{
+    hipStream_t st = s->ctx->stream;
+    GD_MARK(0);
+    gd_sht_plan *p = &s->sht;
+    const int nr = s->d.nr;
+    if (!s->syn_ok) GD_TRY(spectral_pre<T>(s));
+    GD_MARK(1);
     const size_t fs = s->fs;
     const int maxgroup = (p->lmax + 2) / 2;
 
     if (tangential(s)) {
-        // u (S = u_pol, T = u_tor) and grad T (S = T/r) only; FB records of 16
-        // slots, 24 with the composition's gradient (S = C/r) in slots 16..23
-        const int nslot = s->comp ? 24 : 16;
-        a.nslot = nslot;
-        if (s->synth_grad) {
-            // u as a full field, grad T (and grad C) as gradient fields: their
-            // zero toroidal columns are not contracted
-            a.nf = 1;
-            a.ng = s->comp ? 2 : 1;
-            static const int gmap[8] = {0, 1, 2, 3, 16, 17, SYNV_COLS - 2, SYNV_COLS - 1};
-            for (int k = 0; k < 20; ++k) a.colmap[k] = k < 4 + 2 * a.ng ? gmap[k] : -1;
-        } else {
-            a.nf = s->comp ? 3 : 2;
-            static const int vmap[12] = {0, 1, 2, 3, 16, 17, -1, -1, SYNV_COLS - 2, SYNV_COLS - 1, -1, -1};
-            for (int k = 0; k < 20; ++k) a.colmap[k] = k < 4 * a.nf ? vmap[k] : -1;
-        }
         if (s->comp && !s->csyn_ok) GD_TRY(spectral_pre_comp<T>(s));
+        SynthArgs a;
+        RingArgs r;
+        AnaArgs gv, gs;
+        int mode;
+        tang_args(s, nl_out, a, r, gv, gs, &mode);
         GD_TRY(gd_launch_synth_vec<T>(st, a));
         GD_MARK(2);
         GD_MARK(3);
-        r.nslot_f = nslot;
-        // with composition the ring also takes grad C (fields 4, 5) and forms
-        // -u.grad C (compute_composition_nonlinear!, compositional.jl:219-260)
-        // as a fourth product; GB records of 12 slots, 16 with it
-        r.nfield = s->comp ? 6 : 4;
-        r.nslot_g = 8;                    // the force (products 0, 1)
-        r.np1 = 2;
-        r.nslot_g2 = s->comp ? 8 : 4;     // -u.grad T (and -u.grad C)
-        static const int fn[6] = {0, 2, 8, 10, 16, 18}, fsou[6] = {4, 6, 12, 14, 20, 22};
-        for (int F = 0; F < r.nfield; ++F) {
-            r.fn_off[F] = fn[F];
-            r.fs_off[F] = fsou[F];
-        }
-        GD_TRY(gd_launch_ring<T>(st, s->comp ? RING_STEP_TANG_C : RING_STEP_TANG, r));
+        GD_TRY(gd_launch_ring<T>(st, mode, r));
         GD_MARK(4);
         if (side) {
             GD_HIP(hipEventRecord(s->ev_ring, st));
             GD_HIP(hipStreamWaitEvent(s->side, s->ev_ring, 0));
         }
-        // force -> (pol, tor) of velocity nl (velocity.jl:331); the magnetic nl
-        // is u x B = 0 here and its planes stay zero
-        g.nslot_g = r.nslot_g;
-        g.nf = 1;
-        g.gslot0 = 0;
-        g.tab0 = gd_sht_tab(p, 1);
-        g.tab1 = gd_sht_tab(p, 2);
-        g.out[0] = nl_out + 2 * fs; // S(adv) -> u_pol
-        g.out[1] = nl_out + 0 * fs; // T(adv) -> u_tor
-        g.scale[0] = 0;
-        g.scale[1] = 0;
-        GD_TRY(gd_launch_ana<T>(st, g, true, maxgroup));
+        GD_TRY(gd_launch_ana<T>(st, gv, true, maxgroup));
         GD_MARK(5);
-        g.gb = s->gb2.p;
-        g.nslot_g = r.nslot_g2;
-        g.gslot0 = 0;
-    } else {
+        GD_TRY(gd_launch_ana<T>(side ? s->side : st, gs, false, maxgroup));
+        GD_TRY(radial_force<T>(s, st, nl_out));
+        GD_MARK(6);
+        return GD_OK;
+    }
+    SynthArgs a;
+    RingArgs r;
+    AnaArgs g;
+    base_args(s, a, r, g);
+    {
         // vector fields u, omega, B, j (S,T) and grad T (S only)
         a.nslot = 60;
         a.nf = 5;
@@ -353,17 +410,8 @@ template <typename T> static int nonlinear_T(gd_step *s, double *nl_out, hipEven
     g.tab1 = nullptr;
     g.out[0] = nl_out + 8 * fs; // -u.grad T -> T
     g.scale[0] = 1;
-    if (s->comp && tangential(s)) { // and -u.grad C -> the composition's nl (GB slots 12..15)
-        g.nf = 2;
-        g.out[1] = s->cnl[s->step & 1].as<double>() + 8 * fs;
-        g.scale[1] = 1;
-    }
     GD_TRY(gd_launch_ana<T>(side ? s->side : st, g, false, maxgroup));
-    if (s->d.radial_force) { // after the vector analysis wrote nl(u_pol), on its stream
-        hipLaunchKernelGGL(buoyancy_kernel, dim3((nlm + 255) / 256, nr), dim3(256), 0, st, cur_state(s),
-                           p->d_l_of_idx.as<int>(), s->rtab.as<double>(), gd_step_buoyancy_factor(s), nlm, nr, nl_out);
-        GD_TRY(gd_check_launch("buoyancy_kernel"));
-    }
+    GD_TRY(radial_force<T>(s, st, nl_out));
     GD_MARK(6);
     return GD_OK;
 }
@@ -412,6 +460,54 @@ static int composition_bc(gd_step *s, int parity, hipStream_t st)
 //      scalar analysis there, the velocity solves on the main stream.  Written
 //      buffers are disjoint between the streams (state fields, nl planes,
 //      synthesis columns of u and of T/r).
+// the CNAB2 column solves' arguments of a step of the given parity (the chain form)
+static void chain_args(gd_step *s, int parity, bool boot, ChainArgs &c)
+{
+    double *cur = s->nlbuf[parity].as<double>();
+    double *prev = boot ? cur : s->nlbuf[parity ^ 1].as<double>();
+    c = ChainArgs{};
+    c.nuLap = s->nuLap.as<double>();
+    c.rinv2 = s->rtab.as<double>() + s->d.nr;
+    c.nus = s->nus.as<double>();
+    c.LUt = s->LUt.as<double>();
+    c.l_of_idx = s->sht.d_l_of_idx.as<int>();
+    c.nl = s->nl;
+    c.n = s->d.nr;
+    c.bw = s->d.bw;
+    c.nlm = s->nlm;
+    c.dt = s->d.dt;
+    c.theta = s->d.theta;
+    c.bc_mask = bc_mask(s);
+    c.st_in = s->state[parity].as<double>();
+    c.st_out = s->state[parity ^ 1].as<double>();
+    c.nl_arr = cur;
+    c.np_arr = prev;
+    // the magnetic nonlinear terms vanish identically in the tangential step
+    c.nl_zero_mask = tangential(s) ? 0x0c : 0;
+    c.zero_row = s->zrow.as<double>();
+    c.rdiag = s->rdiag.as<double>();
+    c.D1 = s->D1.as<double>();
+    c.D2 = s->D2.as<double>();
+    c.rtab = s->rtab.as<double>();
+    c.radials = s->d.radials;
+    c.synV = s->synV.p;
+    c.synS = s->synS.p;
+    c.syn_tang = tangential(s) ? 1 : 0;
+    c.off64 = s->chain_off64;
+    if (s->twist) {
+        c.ULt = s->ULt.as<double>();
+        c.rdiag2 = s->rdiag2.as<double>();
+        c.Rint = s->Rint.as<double>();
+    }
+    if (s->comp) { // the composition as column group 5 of the same launch
+        c.c_in = s->cstate[parity].as<double>();
+        c.c_out = s->cstate[parity ^ 1].as<double>();
+        c.c_nl = s->cnl[parity].as<double>();
+        c.c_np = boot ? c.c_nl : s->cnl[parity ^ 1].as<double>();
+        c.c_bcr = s->d.bc_rows == 1;
+    }
+}
+
 static int enqueue_step(gd_step *s, int parity, bool boot, hipEvent_t *ev = nullptr)
 {
     double *cur = s->nlbuf[parity].as<double>();
@@ -420,48 +516,7 @@ static int enqueue_step(gd_step *s, int parity, bool boot, hipEvent_t *ev = null
     const bool side_ok = s->chain && tangential(s) && !ev && s->side;
     const bool split = side_ok && s->overlap == 1 && !s->tbc, ana_side = side_ok && s->overlap >= 1;
     ChainArgs c{};
-    if (s->chain) {
-        c.nuLap = s->nuLap.as<double>();
-        c.rinv2 = s->rtab.as<double>() + s->d.nr;
-        c.nus = s->nus.as<double>();
-        c.LUt = s->LUt.as<double>();
-        c.l_of_idx = s->sht.d_l_of_idx.as<int>();
-        c.nl = s->nl;
-        c.n = s->d.nr;
-        c.bw = s->d.bw;
-        c.nlm = s->nlm;
-        c.dt = s->d.dt;
-        c.theta = s->d.theta;
-        c.bc_mask = bc_mask(s);
-        c.st_in = s->state[parity].as<double>();
-        c.st_out = s->state[parity ^ 1].as<double>();
-        c.nl_arr = cur;
-        c.np_arr = prev;
-        // the magnetic nonlinear terms vanish identically in the tangential step
-        c.nl_zero_mask = tangential(s) ? 0x0c : 0;
-        c.zero_row = s->zrow.as<double>();
-        c.rdiag = s->rdiag.as<double>();
-        c.D1 = s->D1.as<double>();
-        c.D2 = s->D2.as<double>();
-        c.rtab = s->rtab.as<double>();
-        c.radials = s->d.radials;
-        c.synV = s->synV.p;
-        c.synS = s->synS.p;
-        c.syn_tang = tangential(s) ? 1 : 0;
-        c.off64 = s->chain_off64;
-        if (s->twist) {
-            c.ULt = s->ULt.as<double>();
-            c.rdiag2 = s->rdiag2.as<double>();
-            c.Rint = s->Rint.as<double>();
-        }
-        if (s->comp) { // the composition as column group 5 of the same launch
-            c.c_in = s->cstate[parity].as<double>();
-            c.c_out = s->cstate[parity ^ 1].as<double>();
-            c.c_nl = s->cnl[parity].as<double>();
-            c.c_np = boot ? c.c_nl : s->cnl[parity ^ 1].as<double>();
-            c.c_bcr = s->d.bc_rows == 1;
-        }
-    }
+    if (s->chain) chain_args(s, parity, boot, c);
     const bool fp64 = s->d.prec == GD_PREC_FP64;
     if (split) {
         GD_HIP(hipEventRecord(s->ev_fork, st));
@@ -1183,6 +1238,33 @@ extern "C" int gd_step_profile(gd_step *s, int nsteps, double *out)
 // batch's streams (forked from and joined to the context stream), so small
 // shells run side by side instead of one after another.  No counterpart in the
 // reference (its shells are separate runs).
+//
+// Fused mode (GD_BATCH_FUSED): the graph of a parity holds one launch per
+// kernel form instead of one branch per shell.  The tangential step's five
+// stages -- synthesis, ring pass, vector analysis, scalar analysis, CNAB2
+// column solves -- each run as one grid over the blocks of every shell that
+// shares the stage's compile-time form (synthesis tile, ring length, node
+// tile, band width), the shells of a grid in decreasing order of cost so the
+// largest start first, every shell's blocks in the order of its own launch
+// (so each output element gets the same operations as in the shell's own
+// step: the results are the same bits).  A shell whose step has no fused form
+// (composition, the full products, ...) runs its own step in the same graph.
+// With several lanes the fused shells are split over that many of the batch
+// streams (balanced by cost, largest first onto the least loaded lane), each
+// lane running its own fused stages, so one lane's stage tail overlaps the
+// others' work.
+enum { BK_SYNTH = 0, BK_RING, BK_ANAV, BK_ANAS, BK_CHAIN, BK_NKIND };
+
+struct BatchGroup {
+    int lane = 0;
+    int kind = 0, fp64 = 0, mm = 0, key = 0; // key: synthesis tile, ring length, node tile or band width
+    ChainForm fm{};
+    size_t lds = 0;
+    int nblocks = 0;
+    size_t off_args = 0, off_sob = 0, off_boff = 0, off_bcnt = 0; // in the parity's table
+    std::vector<int> shells;                                      // batch indices, by decreasing cost
+};
+
 struct gd_batch {
     gd_ctx *ctx = nullptr;
     std::vector<gd_step *> steps;
@@ -1195,6 +1277,12 @@ struct gd_batch {
     // captured work changed since (conditions set or dropped, composition
     // enabled, ...) makes the batch recapture before its next replay
     std::vector<unsigned> gens[2];
+    int mode = GD_BATCH_STREAMS;
+    int lanes = 1;                         // fused mode: streams the fused shells are split over
+    std::vector<int> tiles;                // synthesis tile of each shell (fused mode)
+    std::vector<BatchGroup> groups[2];     // fused launches of each parity
+    std::vector<int> solo[2];              // shells that run their own step
+    DevBuf table[2];                       // the groups' argument and block tables
 };
 
 static void batch_drop_graphs(gd_batch *b)
@@ -1205,6 +1293,10 @@ static void batch_drop_graphs(gd_batch *b)
         b->gexec[k] = nullptr;
         b->graph[k] = nullptr;
         b->gens[k].clear();
+        // the fused tables are rebuilt with the graphs (they hold the
+        // members' buffer addresses)
+        b->groups[k].clear();
+        b->solo[k].clear();
     }
 }
 
@@ -1219,6 +1311,8 @@ static bool batch_stale(const gd_batch *b, int par)
 static void batch_release(gd_batch *b)
 {
     batch_drop_graphs(b);
+    b->table[0].release();
+    b->table[1].release();
     for (hipEvent_t e : b->joins)
         if (e) (void)hipEventDestroy(e);
     if (b->fork) (void)hipEventDestroy(b->fork);
@@ -1230,13 +1324,17 @@ extern "C" int gd_batch_create(gd_ctx *ctx, int n, gd_step *const *steps, int ns
 {
     GD_CHECK(ctx && steps && out && n >= 1 && nstreams >= 1 && nstreams <= 64, GD_ERR_ARG,
              "gd_batch_create: bad arguments");
-    for (int i = 0; i < n; ++i)
+    for (int i = 0; i < n; ++i) {
         GD_CHECK(steps[i] && steps[i]->ctx == ctx, GD_ERR_ARG, "gd_batch_create: step %d is null or on another context",
                  i);
+        for (int j = 0; j < i; ++j)
+            GD_CHECK(steps[j] != steps[i], GD_ERR_ARG, "gd_batch_create: step %d appears twice (also %d)", i, j);
+    }
     GD_HIP(hipSetDevice(ctx->device));
     gd_batch *b = new gd_batch();
     b->ctx = ctx;
     b->steps.assign(steps, steps + n);
+    b->tiles.assign(n, 0);
     hipError_t e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
     for (int k = 0; k < nstreams && e == hipSuccess; ++k) {
         hipStream_t s = nullptr;
@@ -1263,6 +1361,268 @@ extern "C" int gd_batch_destroy(gd_batch *b)
     (void)hipStreamSynchronize(b->ctx->stream);
     batch_release(b);
     delete b;
+    return GD_OK;
+}
+
+// ---- fused mode: the launch groups of one parity
+struct ShellArgs {
+    SynthArgs a;
+    RingArgs r;
+    AnaArgs gv, gs;
+    ChainArgs c;
+    ChainForm fm;
+    bool fp64;
+    int kn;
+};
+
+// the shell's fused-form arguments for a step of the given parity; false when
+// its step has no fused form
+static bool fused_args(gd_step *s, int parity, ShellArgs &x)
+{
+    if (!tangential(s) || !s->chain || s->comp || !s->synth_grad) return false;
+    int mode;
+    tang_args(s, s->nlbuf[parity].as<double>(), x.a, x.r, x.gv, x.gs, &mode);
+    chain_args(s, parity, false, x.c);
+    gd_cnab2_chain_form(x.c, &x.fm);
+    x.fp64 = s->d.prec == GD_PREC_FP64;
+    x.kn = gd_ana_kn(x.a.nlat2, x.fp64);
+    if (mode != RING_STEP_TANG || x.kn == 0 || !x.fm.direct) return false;
+    // the analyses in one node chunk of all nlat/2 nodes (the grid its own
+    // launch has when nlat/2 <= the node tile)
+    for (AnaArgs *g : {&x.gv, &x.gs}) {
+        g->k0 = 0;
+        g->nk = x.a.nlat2;
+        g->accum = 0;
+    }
+    if (x.fm.lds_lane > 64 * 1024 || (x.fm.twist && x.fm.lds_twist > 64 * 1024)) return false;
+    if (gd_synth_tang_batchable(x.a) != GD_OK || gd_ring_tang_batchable(x.r, x.fp64) != GD_OK) return false;
+    return true;
+}
+
+static size_t align16(size_t v) { return (v + 15) & ~(size_t)15; }
+
+static int fused_build(gd_batch *b, int parity)
+{
+    const int n = (int)b->steps.size();
+    std::vector<ShellArgs> xs(n);
+    std::vector<char> ok(n, 0);
+    auto &groups = b->groups[parity];
+    auto &solo = b->solo[parity];
+    groups.clear();
+    solo.clear();
+    for (int i = 0; i < n; ++i) {
+        ok[i] = fused_args(b->steps[i], parity, xs[i]);
+        if (!ok[i]) solo.push_back(i);
+    }
+    // group key of each stage; costs for the order within a group
+    // lanes: largest shells first onto the least loaded lane (cost: the
+    // shell's Legendre work, which the transform stages scale with)
+    std::vector<int> lane_of(n, 0);
+    {
+        std::vector<int> ord;
+        for (int i = 0; i < n; ++i)
+            if (ok[i]) ord.push_back(i);
+        auto work = [&](int i) { return (double)xs[i].a.nlm * xs[i].a.nlat2 * xs[i].a.nr; };
+        std::stable_sort(ord.begin(), ord.end(), [&](int p, int q) { return work(p) > work(q); });
+        std::vector<double> load(b->lanes, 0.0);
+        for (int i : ord) {
+            const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            lane_of[i] = l;
+            load[l] += work(i);
+        }
+    }
+    int cur_lane = 0;
+    auto find = [&](int kind, int fp64, int mm, int key, const ChainForm *fm) -> BatchGroup & {
+        for (auto &g : groups)
+            if (g.lane == cur_lane && g.kind == kind && g.fp64 == fp64 && g.mm == mm && g.key == key &&
+                (!fm || (g.fm.twist == fm->twist && g.fm.off32 == fm->off32)))
+                return g;
+        groups.emplace_back();
+        BatchGroup &g = groups.back();
+        g.lane = cur_lane;
+        g.kind = kind;
+        g.fp64 = fp64;
+        g.mm = mm;
+        g.key = key;
+        if (fm) g.fm = *fm;
+        return g;
+    };
+    for (int i = 0; i < n; ++i) {
+        if (!ok[i]) continue;
+        const ShellArgs &x = xs[i];
+        const int t = b->tiles[i];
+        cur_lane = lane_of[i];
+        GD_CHECK(gd_synth_tile_ok(t, x.fp64), GD_ERR_ARG, "gd_batch: shell %d: synthesis tile %d not available", i, t);
+        find(BK_SYNTH, x.fp64, x.a.mm, t, nullptr).shells.push_back(i);
+        find(BK_RING, x.fp64, 0, x.r.nlon, nullptr).shells.push_back(i);
+        find(BK_ANAV, x.fp64, x.gv.mm, x.kn, nullptr).shells.push_back(i);
+        find(BK_ANAS, x.fp64, x.gs.mm, x.kn, nullptr).shells.push_back(i);
+        BatchGroup &gc = find(BK_CHAIN, x.fp64, 0, x.c.bw, &x.fm);
+        gc.shells.push_back(i);
+        gc.lds = std::max(gc.lds, x.fm.twist ? x.fm.lds_twist : x.fm.lds_lane);
+    }
+    auto blocks = [&](const BatchGroup &g, int i) -> int {
+        const ShellArgs &x = xs[i];
+        switch (g.kind) {
+        case BK_SYNTH: return gd_synth_tang_blocks(x.a, g.key);
+        case BK_RING: return x.r.nlat2 * x.r.nr;
+        case BK_ANAV: return gd_ana_blocks(x.gv);
+        case BK_ANAS: return gd_ana_blocks(x.gs);
+        default: return gd_cnab2_chain_blocks(x.c);
+        }
+    };
+    auto cost = [&](const BatchGroup &g, int i) -> double {
+        const ShellArgs &x = xs[i];
+        if (g.kind == BK_RING) return (double)x.r.nlat2 * x.r.nr * x.r.nlon * std::log2((double)x.r.nlon);
+        if (g.kind == BK_CHAIN) return (double)x.c.nlm * x.c.n;
+        return (double)x.a.nlm * x.a.nlat2 * x.a.nr;
+    };
+    // stage order, then the table: per group the args array, then the block
+    // tables; each shell's blocks start at a multiple of 8
+    std::stable_sort(groups.begin(), groups.end(), [](const BatchGroup &p, const BatchGroup &q) {
+        return p.lane != q.lane ? p.lane < q.lane : p.kind < q.kind;
+    });
+    std::vector<char> blob;
+    auto put = [&](const void *src, size_t bytes) -> size_t {
+        const size_t o = align16(blob.size());
+        blob.resize(o + bytes);
+        std::memcpy(blob.data() + o, src, bytes);
+        return o;
+    };
+    for (auto &g : groups) {
+        std::stable_sort(g.shells.begin(), g.shells.end(), [&](int p, int q) { return cost(g, p) > cost(g, q); });
+        const int ng = (int)g.shells.size();
+        GD_CHECK(ng < (int)BATCH_PAD, GD_ERR_ARG, "gd_batch: %d shells in one launch", ng);
+        std::vector<int> boff(ng), bcnt(ng);
+        std::vector<unsigned short> sob;
+        for (int k = 0; k < ng; ++k) {
+            bcnt[k] = blocks(g, g.shells[k]);
+            while (sob.size() % 8) sob.push_back(BATCH_PAD);
+            boff[k] = (int)sob.size();
+            sob.insert(sob.end(), (size_t)bcnt[k], (unsigned short)k);
+        }
+        GD_CHECK(sob.size() < (size_t)1 << 31, GD_ERR_ARG, "gd_batch: %zu blocks in one launch", sob.size());
+        g.nblocks = (int)sob.size();
+        switch (g.kind) {
+        case BK_SYNTH: {
+            std::vector<SynthArgs> v;
+            for (int i : g.shells) v.push_back(xs[i].a);
+            g.off_args = put(v.data(), sizeof(SynthArgs) * v.size());
+            break;
+        }
+        case BK_RING: {
+            std::vector<RingArgs> v;
+            for (int i : g.shells) v.push_back(xs[i].r);
+            g.off_args = put(v.data(), sizeof(RingArgs) * v.size());
+            break;
+        }
+        case BK_ANAV:
+        case BK_ANAS: {
+            std::vector<AnaArgs> v;
+            for (int i : g.shells) v.push_back(g.kind == BK_ANAV ? xs[i].gv : xs[i].gs);
+            g.off_args = put(v.data(), sizeof(AnaArgs) * v.size());
+            break;
+        }
+        default: {
+            std::vector<ChainArgs> v;
+            for (int i : g.shells) v.push_back(xs[i].c);
+            g.off_args = put(v.data(), sizeof(ChainArgs) * v.size());
+            break;
+        }
+        }
+        g.off_sob = put(sob.data(), sizeof(unsigned short) * sob.size());
+        g.off_boff = put(boff.data(), sizeof(int) * ng);
+        g.off_bcnt = put(bcnt.data(), sizeof(int) * ng);
+    }
+    if (blob.empty()) return GD_OK;
+    GD_TRY(b->table[parity].ensure(blob.size()));
+    GD_HIP(hipMemcpy(b->table[parity].p, blob.data(), blob.size(), hipMemcpyHostToDevice));
+    return GD_OK;
+}
+
+template <class A> static BatchRef<A> group_ref(const gd_batch *b, int parity, const BatchGroup &g)
+{
+    const char *base = static_cast<const char *>(b->table[parity].p);
+    return BatchRef<A>{reinterpret_cast<const A *>(base + g.off_args),
+                       reinterpret_cast<const unsigned short *>(base + g.off_sob),
+                       reinterpret_cast<const int *>(base + g.off_boff),
+                       reinterpret_cast<const int *>(base + g.off_bcnt)};
+}
+
+template <typename T> static int launch_group(gd_batch *b, int parity, const BatchGroup &g, hipStream_t st)
+{
+    switch (g.kind) {
+    case BK_SYNTH: return gd_launch_synth_tang_batch<T>(st, group_ref<SynthArgs>(b, parity, g), g.nblocks, g.key, g.mm);
+    case BK_RING: return gd_launch_ring_tang_batch<T>(st, group_ref<RingArgs>(b, parity, g), g.nblocks, g.key);
+    case BK_ANAV:
+    case BK_ANAS:
+        return gd_launch_ana_batch<T>(st, group_ref<AnaArgs>(b, parity, g), g.nblocks, g.kind == BK_ANAV, g.key, g.mm);
+    default:
+        return gd_launch_cnab2_chain_batch(st, group_ref<ChainArgs>(b, parity, g), g.nblocks, g.key, g.fm, g.lds,
+                                           g.fp64 != 0);
+    }
+}
+
+// every shell's step of the given parity: the shells without a fused form
+// first (their own steps, on the context stream), then each lane's fused
+// stages (one lane: on the context stream; more: on batch streams forked
+// from and joined to it)
+static int fused_lane(gd_batch *b, int parity, int lane, hipStream_t st)
+{
+    bool extras = false;
+    for (const BatchGroup &g : b->groups[parity]) {
+        if (g.lane != lane) continue;
+        if (g.kind == BK_CHAIN && !extras) {
+            // per-shell work between the analyses and the solves: the radial
+            // forcing (after nl(u_pol) is written) and the temperature
+            // conditions on the state the solve reads
+            extras = true;
+            for (const BatchGroup &h : b->groups[parity]) {
+                if (h.lane != lane || h.kind != BK_CHAIN) continue;
+                for (int i : h.shells) {
+                    gd_step *s = b->steps[i];
+                    GD_TRY(radial_force<double>(s, st, s->nlbuf[parity].as<double>()));
+                    GD_TRY(temperature_bc(s, parity, st));
+                }
+            }
+        }
+        GD_TRY(g.fp64 ? launch_group<double>(b, parity, g, st) : launch_group<float>(b, parity, g, st));
+    }
+    return GD_OK;
+}
+
+static int fused_enqueue(gd_batch *b, int parity)
+{
+    hipStream_t main = b->ctx->stream;
+    GD_CHECK(!b->groups[parity].empty() || !b->solo[parity].empty(), GD_ERR_ARG, "gd_batch: fused tables not built");
+    for (int i : b->solo[parity]) {
+        gd_step *s = b->steps[i];
+        const int ov = s->overlap;
+        s->overlap = 0;
+        const int rc = enqueue_step(s, parity, false);
+        s->overlap = ov;
+        GD_TRY(rc);
+    }
+    const int L = b->lanes;
+    if (L == 1) {
+        GD_TRY(fused_lane(b, parity, 0, main));
+    } else {
+        GD_CHECK(L <= (int)b->streams.size(), GD_ERR_ARG, "gd_batch: %d lanes, %zu streams", L, b->streams.size());
+        GD_HIP(hipEventRecord(b->fork, main));
+        for (int l = 0; l < L; ++l) GD_HIP(hipStreamWaitEvent(b->streams[l], b->fork, 0));
+        int rc = GD_OK;
+        for (int l = 0; l < L && rc == GD_OK; ++l) rc = fused_lane(b, parity, l, b->streams[l]);
+        // every forked stream joins back, also after an error (the capture
+        // then ends with the error already set)
+        for (int l = 0; l < L; ++l) {
+            GD_HIP(hipEventRecord(b->joins[l], b->streams[l]));
+            GD_HIP(hipStreamWaitEvent(main, b->joins[l], 0));
+        }
+        GD_TRY(rc);
+    }
+    for (const BatchGroup &g : b->groups[parity])
+        if (g.kind == BK_CHAIN)
+            for (int i : g.shells) b->steps[i]->syn_ok = true; // synV describes the new state
     return GD_OK;
 }
 
@@ -1323,8 +1683,12 @@ extern "C" int gd_batch_run(gd_batch *b, int nsteps)
             }
             if (!b->gexec[par]) {
                 hipStream_t main = b->ctx->stream;
+                if (b->mode == GD_BATCH_FUSED && b->groups[par].empty() && b->solo[par].empty()) {
+                    GD_HIP(hipStreamSynchronize(main)); // the table may be in use by a replay
+                    GD_TRY(fused_build(b, par));
+                }
                 GD_HIP(hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal));
-                const int rc = batch_enqueue(b, par);
+                const int rc = b->mode == GD_BATCH_FUSED ? fused_enqueue(b, par) : batch_enqueue(b, par);
                 hipGraph_t g = nullptr;
                 const hipError_t e = hipStreamEndCapture(main, &g);
                 GD_TRY(rc);
@@ -1342,4 +1706,166 @@ extern "C" int gd_batch_run(gd_batch *b, int nsteps)
         for (gd_step *s : b->steps) ++s->step;
     }
     return GD_OK;
+}
+
+extern "C" int gd_batch_set_mode(gd_batch *b, int mode)
+{
+    GD_CHECK(b && (mode == GD_BATCH_STREAMS || mode == GD_BATCH_FUSED), GD_ERR_ARG, "gd_batch_set_mode: bad args");
+    if (mode == b->mode) return GD_OK;
+    GD_HIP(hipSetDevice(b->ctx->device));
+    GD_HIP(hipStreamSynchronize(b->ctx->stream));
+    batch_drop_graphs(b);
+    b->mode = mode;
+    return GD_OK;
+}
+
+extern "C" int gd_batch_set_tiles(gd_batch *b, const int *tiles)
+{
+    GD_CHECK(b && tiles, GD_ERR_ARG, "gd_batch_set_tiles: null argument");
+    for (size_t i = 0; i < b->steps.size(); ++i)
+        GD_CHECK(gd_synth_tile_ok(tiles[i], b->steps[i]->d.prec == GD_PREC_FP64), GD_ERR_ARG,
+                 "gd_batch_set_tiles: shell %zu: tile %d not available", i, tiles[i]);
+    GD_HIP(hipSetDevice(b->ctx->device));
+    GD_HIP(hipStreamSynchronize(b->ctx->stream));
+    batch_drop_graphs(b);
+    b->tiles.assign(tiles, tiles + b->steps.size());
+    return GD_OK;
+}
+
+extern "C" int gd_batch_get_tiles(const gd_batch *b, int *tiles)
+{
+    GD_CHECK(b && tiles, GD_ERR_ARG, "gd_batch_get_tiles: null argument");
+    std::copy(b->tiles.begin(), b->tiles.end(), tiles);
+    return GD_OK;
+}
+
+// mean time of reps launches of fn on the context stream, after one untimed launch
+template <class F> static int time_ms(gd_batch *b, int reps, F fn, double *ms)
+{
+    hipStream_t st = b->ctx->stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    GD_HIP(hipEventCreate(&e0));
+    hipError_t e = hipEventCreate(&e1);
+    int rc = e == hipSuccess ? fn() : GD_ERR_HIP;
+    if (rc == GD_OK && (e = hipEventRecord(e0, st)) != hipSuccess) rc = GD_ERR_HIP;
+    for (int k = 0; k < reps && rc == GD_OK; ++k) rc = fn();
+    if (rc == GD_OK && (e = hipEventRecord(e1, st)) != hipSuccess) rc = GD_ERR_HIP;
+    if (rc == GD_OK && (e = hipEventSynchronize(e1)) != hipSuccess) rc = GD_ERR_HIP;
+    float t = 0;
+    if (rc == GD_OK && (e = hipEventElapsedTime(&t, e0, e1)) != hipSuccess) rc = GD_ERR_HIP;
+    if (rc == GD_ERR_HIP && e != hipSuccess) gd_set_error("gd_batch_tune: %s", hipGetErrorString(e));
+    (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    *ms = t / reps;
+    return rc;
+}
+
+// The synthesis tiles of the fused mode, measured on this device: each shell's
+// synthesis alone in every tile form it has (its fastest per shell), then the
+// fused synthesis launches of the whole batch with those per-shell choices and
+// with each tile for every shell; the fastest assignment is kept.  The
+// synthesis writes only the shells' transform scratch (FB), which every step
+// rewrites before reading.
+extern "C" int gd_batch_tune(gd_batch *b, int reps, double *ms_out)
+{
+    GD_CHECK(b && reps >= 1, GD_ERR_ARG, "gd_batch_tune: bad args");
+    GD_HIP(hipSetDevice(b->ctx->device));
+    GD_HIP(hipStreamSynchronize(b->ctx->stream));
+    const int n = (int)b->steps.size();
+    std::vector<int> best(n, 0);
+    for (int i = 0; i < n; ++i) {
+        gd_step *s = b->steps[i];
+        ShellArgs x;
+        if (!fused_args(s, 0, x)) continue;
+        double tbest = 0;
+        for (int t = 0; t < 3; ++t) {
+            if (!gd_synth_tile_ok(t, x.fp64)) continue;
+            double ms = 0;
+            GD_TRY(time_ms(b, reps, [&]() -> int {
+                return x.fp64 ? gd_launch_synth_tang_tile<double>(b->ctx->stream, x.a, t)
+                              : gd_launch_synth_tang_tile<float>(b->ctx->stream, x.a, t);
+            }, &ms));
+            if (t == 0 || ms < tbest) {
+                tbest = ms;
+                best[i] = t;
+            }
+        }
+    }
+    std::vector<std::vector<int>> cands{best};
+    for (int t = 0; t < 3; ++t) {
+        std::vector<int> u(n);
+        for (int i = 0; i < n; ++i) u[i] = gd_synth_tile_ok(t, b->steps[i]->d.prec == GD_PREC_FP64) ? t : 0;
+        cands.push_back(u);
+    }
+    batch_drop_graphs(b);
+    double tmin = 0;
+    std::vector<int> pick = best;
+    for (size_t c = 0; c < cands.size(); ++c) {
+        b->tiles = cands[c];
+        GD_TRY(fused_build(b, 0));
+        double ms = 0;
+        GD_TRY(time_ms(b, reps, [&]() -> int {
+            for (const BatchGroup &g : b->groups[0])
+                if (g.kind == BK_SYNTH) GD_TRY(g.fp64 ? launch_group<double>(b, 0, g, b->ctx->stream)
+                                                      : launch_group<float>(b, 0, g, b->ctx->stream));
+            return GD_OK;
+        }, &ms));
+        if (ms_out) ms_out[c] = ms;
+        if (c == 0 || ms < tmin) {
+            tmin = ms;
+            pick = cands[c];
+        }
+    }
+    GD_HIP(hipStreamSynchronize(b->ctx->stream));
+    batch_drop_graphs(b);
+    b->tiles = pick;
+    return GD_OK;
+}
+
+extern "C" int gd_batch_set_lanes(gd_batch *b, int lanes)
+{
+    GD_CHECK(b && lanes >= 1 && lanes <= (int)b->streams.size(), GD_ERR_ARG,
+             "gd_batch_set_lanes: lanes must be 1 .. the batch's stream count");
+    if (lanes == b->lanes) return GD_OK;
+    GD_HIP(hipSetDevice(b->ctx->device));
+    GD_HIP(hipStreamSynchronize(b->ctx->stream));
+    batch_drop_graphs(b);
+    b->lanes = lanes;
+    return GD_OK;
+}
+
+extern "C" int gd_batch_get_lanes(const gd_batch *b, int *lanes)
+{
+    GD_CHECK(b && lanes, GD_ERR_ARG, "gd_batch_get_lanes: null argument");
+    *lanes = b->lanes;
+    return GD_OK;
+}
+
+// Lane count of the fused mode, measured: for each candidate (1, 2, 4, 8 up to
+// the stream count and the shell count) the batch runs 2 steps (the
+// captures), nsteps untimed and nsteps timed steps; the fastest is kept.
+// These are the batch's real steps (every lane count gives the same bits), so
+// the shells advance by 2 nsteps + 2 steps per candidate.
+extern "C" int gd_batch_tune_lanes(gd_batch *b, int nsteps, double *ms_out)
+{
+    GD_CHECK(b && nsteps >= 1 && b->mode == GD_BATCH_FUSED, GD_ERR_ARG, "gd_batch_tune_lanes: bad args (fused mode)");
+    GD_HIP(hipSetDevice(b->ctx->device));
+    const int n = (int)b->steps.size();
+    int best = 1, k = 0;
+    double tbest = 0;
+    for (int L = 1; L <= 8; L *= 2, ++k) {
+        double ms = -1;
+        if (L <= (int)b->streams.size() && L <= n) {
+            GD_TRY(gd_batch_set_lanes(b, L));
+            GD_TRY(gd_batch_run(b, 2));
+            GD_TRY(time_ms(b, 1, [&]() -> int { return gd_batch_run(b, nsteps); }, &ms));
+            ms /= nsteps;
+            if (L == 1 || ms < tbest) {
+                tbest = ms;
+                best = L;
+            }
+        }
+        if (ms_out) ms_out[k] = ms;
+    }
+    return gd_batch_set_lanes(b, best);
 }

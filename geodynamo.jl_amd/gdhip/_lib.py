@@ -32,7 +32,8 @@ EXPORTS = [
     "gd_erk2_cache_create", "gd_erk2_cache_from_matrices", "gd_erk2_cache_get", "gd_erk2_cache_destroy",
     "gd_erk2_buffers_create", "gd_erk2_buffers_destroy", "gd_erk2_prepare_field", "gd_erk2_apply_stage",
     "gd_erk2_store_stage_nonlinear", "gd_erk2_finalize_field",
-    "gd_batch_create", "gd_batch_run", "gd_batch_destroy",
+    "gd_batch_create", "gd_batch_run", "gd_batch_destroy", "gd_batch_set_mode", "gd_batch_set_tiles",
+    "gd_batch_get_tiles", "gd_batch_tune", "gd_batch_set_lanes", "gd_batch_get_lanes", "gd_batch_tune_lanes",
     "gd_step_get_history", "gd_step_restore", "gd_step_shape", "gd_step_radii", "gd_step_set_temperature_bc",
     "gd_step_enable_composition", "gd_step_set_composition", "gd_step_get_composition", "gd_step_set_composition_bc",
     "gd_step_has_composition", "gd_step_get_composition_history", "gd_step_restore_composition",
@@ -151,6 +152,13 @@ def lib():
             "gd_batch_create": (i, [vp, i, vp, i, pp]),
             "gd_batch_run": (i, [vp, i]),
             "gd_batch_destroy": (i, [vp]),
+            "gd_batch_set_mode": (i, [vp, i]),
+            "gd_batch_set_tiles": (i, [vp, vp]),
+            "gd_batch_get_tiles": (i, [vp, vp]),
+            "gd_batch_tune": (i, [vp, i, vp]),
+            "gd_batch_set_lanes": (i, [vp, i]),
+            "gd_batch_get_lanes": (i, [vp, vp]),
+            "gd_batch_tune_lanes": (i, [vp, i, vp]),
             "gd_step_get_history": (i, [vp, vp, vp, i]),
             "gd_step_restore": (i, [vp] * 5 + [i, i]),
             "gd_step_shape": (i, [vp, vp, vp, vp]),

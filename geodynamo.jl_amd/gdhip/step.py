@@ -12,6 +12,7 @@ from ._lib import (GD_FD_LITERAL, GD_FD_TRANSPOSE, StepDesc, check, lib)
 from .shtnskit_transforms import PRECISIONS
 
 FIELDS = ("u_tor", "u_pol", "B_tor", "B_pol", "T")
+BATCH_MODES = {"streams": 0, "fused": 1}
 
 
 class CNAB2Step:
@@ -27,6 +28,7 @@ class CNAB2Step:
                              int(radials), int(coriolis_index), int(bc_rows), int(full_products),
                              int(radial_force))
         self.lmax, self.mmax, self.nlat, self.nlon, self.nr = lmax, mmax, nlat, nlon, nr
+        self.precision = precision
         self.nlm = sum(min(l, mmax) + 1 for l in range(lmax + 1))
         self.device = device
         self._h = ctypes.c_void_p()
@@ -182,23 +184,86 @@ class CNAB2Step:
 
 
 class StepBatch:
-    """Independent shells advanced together: one graph per step parity holds
-    every shell's step, spread over `nstreams` streams (gd_batch_*).  The
-    shells must be at the same step count; results equal each shell's own
-    run(graph=True)."""
+    """Independent shells advanced together (gd_batch_*): one graph per step
+    parity holds every shell's step.  The shells must be at the same step
+    count; results equal each shell's own run(graph=True) bit for bit.
 
-    def __init__(self, steps, nstreams=8):
+    mode "streams": the shells' own steps spread over `nstreams` streams.
+    mode "fused": one launch per kernel form and stage over every shell of
+    that form (synthesis, ring pass, both analyses, CNAB2 solves), on one
+    stream, or split over `lanes` streams.  tiles (fused mode): "table"
+    takes each shell's Legendre-synthesis tile from the committed tuning table
+    (tile 0 where a shape is absent), "tune" runs the autotuner on this
+    device, or a list gives them.  lanes: the fused shells' stream count
+    (tune_lanes measures it; it advances the shells by real steps)."""
+
+    def __init__(self, steps, nstreams=8, mode="streams", tiles="table", lanes=1):
         if not steps:
             raise ValueError("empty batch")
+        if mode not in BATCH_MODES:
+            raise ValueError("mode: one of %s" % sorted(BATCH_MODES))
         self.steps = list(steps)
         arr = (ctypes.c_void_p * len(self.steps))(*[s._h.value for s in self.steps])
         self._arr = arr
         self._h = ctypes.c_void_p()
         check(lib().gd_batch_create(context(self.steps[0].device), len(self.steps), arr, int(nstreams),
                                     ctypes.byref(self._h)))
+        self.mode = mode
+        check(lib().gd_batch_set_mode(self._h, BATCH_MODES[mode]))
+        self.tune_ms = None
+        if mode == "fused":
+            if tiles == "tune":
+                self.tune()
+            elif tiles == "table":
+                from .tuning import synth_tiles
+
+                self.set_tiles(synth_tiles(self.steps))
+            else:
+                self.set_tiles(tiles)
+        self.lanes_ms = None
+        if mode == "fused":
+            self.set_lanes(lanes)
 
     def run(self, nsteps=1):
         check(lib().gd_batch_run(self._h, nsteps))
+
+    def set_lanes(self, lanes):
+        check(lib().gd_batch_set_lanes(self._h, int(lanes)))
+
+    def lanes(self):
+        v = ctypes.c_int()
+        check(lib().gd_batch_get_lanes(self._h, ctypes.byref(v)))
+        return v.value
+
+    def tune_lanes(self, nsteps=4):
+        """Runs the lane autotuner (gd_batch_tune_lanes): advances every shell
+        by 2 nsteps + 2 real steps per candidate lane count (1, 2, 4, 8 up to
+        the stream and shell counts); returns the chosen count.
+        self.lanes_ms: ms per step of lanes 1, 2, 4, 8 (-1: not run)."""
+        ms = np.zeros(4)
+        check(lib().gd_batch_tune_lanes(self._h, int(nsteps), ms.ctypes.data))
+        self.lanes_ms = [float(v) for v in ms]
+        return self.lanes()
+
+    def set_tiles(self, tiles):
+        t = np.ascontiguousarray(tiles, dtype=np.int32)
+        if t.shape != (len(self.steps),):
+            raise ValueError("tiles: one per shell (%d)" % len(self.steps))
+        check(lib().gd_batch_set_tiles(self._h, t.ctypes.data))
+
+    def tiles(self):
+        t = np.zeros(len(self.steps), dtype=np.int32)
+        check(lib().gd_batch_get_tiles(self._h, t.ctypes.data))
+        return [int(v) for v in t]
+
+    def tune(self, reps=5):
+        """Runs the synthesis-tile autotuner (gd_batch_tune); returns the
+        chosen tiles.  self.tune_ms: mean ms of the per-shell assignment and
+        of the uniform tiles 0, 1, 2."""
+        ms = np.zeros(4)
+        check(lib().gd_batch_tune(self._h, int(reps), ms.ctypes.data))
+        self.tune_ms = [float(v) for v in ms]
+        return self.tiles()
 
     def synchronize(self):
         check(lib().gd_ctx_synchronize(context(self.steps[0].device)))

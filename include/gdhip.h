@@ -225,6 +225,43 @@ typedef struct gd_batch gd_batch;
 int gd_batch_create(gd_ctx *ctx, int n, gd_step *const *steps, int nstreams, gd_batch **out);
 int gd_batch_run(gd_batch *b, int nsteps);
 int gd_batch_destroy(gd_batch *b);
+/* Launch form of the batch graphs (default GD_BATCH_STREAMS, as above).
+ * GD_BATCH_FUSED: one launch per kernel form and stage -- Legendre synthesis,
+ * ring FFT + products, vector analysis, scalar analysis, CNAB2 column solves --
+ * over the blocks of every shell that shares the form (synthesis tile, ring
+ * length, analysis node tile, band width), largest shells first, all on the
+ * context stream.  Shells whose step has no fused form (composition, full
+ * products, radials) run their own step in the same graph.  Results equal
+ * each shell's own gd_step_run_graph bit for bit. */
+#define GD_BATCH_STREAMS 0
+#define GD_BATCH_FUSED 1
+int gd_batch_set_mode(gd_batch *b, int mode);
+/* Legendre-synthesis tile of each shell in the fused form, tiles[n]:
+ * 0 = 3 waves x 8 levels with 16-degree chunks, 1 = 3 waves with 32-degree
+ * chunks, 2 = 6 waves x 16 levels with 16-degree chunks (FP32 and BF16x3
+ * shells only).  Every tile gives the same bits. */
+int gd_batch_set_tiles(gd_batch *b, const int *tiles);
+int gd_batch_get_tiles(const gd_batch *b, int *tiles);
+/* Autotuner of the fused form's synthesis tiles on this device: each shell's
+ * synthesis timed alone in every tile (mean of reps launches) gives a per-shell
+ * choice; that assignment and each uniform one (every shell on tile 0, 1, 2
+ * where available) are timed as the batch's fused synthesis launches, and the
+ * fastest is kept (gd_batch_get_tiles reads it).  ms_out (host, 4 entries, or
+ * NULL): the mean ms of those four assignments in that order.  Writes only
+ * the shells' transform scratch; the states are untouched. */
+int gd_batch_tune(gd_batch *b, int reps, double *ms_out);
+/* Lanes of the fused form (default 1): the fused shells split over that many of
+ * the batch's streams (1 .. nstreams), balanced by Legendre work, each lane
+ * running its own fused stages concurrently with the others.  Same bits for
+ * every lane count. */
+int gd_batch_set_lanes(gd_batch *b, int lanes);
+int gd_batch_get_lanes(const gd_batch *b, int *lanes);
+/* Lane autotuner (fused mode): lanes 1, 2, 4, 8 (up to nstreams and the shell
+ * count), each timed over nsteps of the batch's own steps after 2 + nsteps
+ * untimed ones; keeps the fastest.  The shells advance by 2 nsteps + 2 steps
+ * per candidate.  ms_out (host, 4 entries, or NULL): ms per step of each
+ * candidate, -1 where not run. */
+int gd_batch_tune_lanes(gd_batch *b, int nsteps, double *ms_out);
 
 /* ---- boundary conditions of a scalar field ----------------------------------
  * replaces apply_temperature_boundary_conditions_spectral! (thermal.jl:262-313)
