@@ -249,7 +249,9 @@ __global__ void __launch_bounds__(64) cnab2_chain_kernel(ChainArgs c)
 
 // the block's (column block, field, re/im) from its index within a shell's
 // share of a batched launch (column block fastest, as the shell's own grid)
-__device__ __forceinline__ void chain_decode(const ChainArgs &c, int loc, int &bx, int &by, int &bz)
+// a batched launch's block loc of one shell -> (column block, field, re/im)
+// of that shell's own grid (gx, nfield, 2 / NC)
+template <int NC> __device__ __forceinline__ void chain_decode(const ChainArgs &c, int loc, int &bx, int &by, int &bz)
 {
     const int gx = (c.nlm + 63) / 64, nfield = c.nf ? c.nf : (c.c_in ? 6 : 5);
     bx = loc % gx;
@@ -265,7 +267,7 @@ __global__ void __launch_bounds__(64) cnab2_chain_batch_kernel(BatchRef<ChainArg
     if (sh == BATCH_PAD) return;
     const ChainArgs &c = b.args[sh];
     int bx, by, bz;
-    chain_decode(c, blockIdx.x - b.block_off[sh], bx, by, bz);
+    chain_decode<1>(c, blockIdx.x - b.block_off[sh], bx, by, bz);
     cnab2_chain_body<BW, D, TS, DIRECT, IX>(c, bx, by, bz);
 }
 
@@ -282,7 +284,13 @@ __global__ void __launch_bounds__(64) cnab2_chain_batch_kernel(BatchRef<ChainArg
 // cnab2_chain_kernel.  Lanes past nlm repeat column nlm-1: they compute the
 // same values as its own lane and store them to the same places, which keeps
 // both waves on the barrier and the loops free of per-lane store guards.
-template <int BW, int D, typename TS>
+// NC = 2: each lane solves the re and the im column of its (idx, field)
+// (blk_z = 0): the two columns share every row offset, band-table load and
+// interface matrix, so the scalar address work and the table loads per column
+// halve and the two independent recurrences interleave.  Each column gets the
+// operations of NC = 1; the compiler may fuse them into FMAs differently, so
+// the two forms agree to rounding, not always bit for bit.
+template <int BW, int D, typename TS, int NC>
 __device__ __forceinline__ void cnab2_twist_body(const ChainArgs &c, int blk_x, int blk_y, int blk_z)
 {
     constexpr int W = 2 * BW + 1, M2 = 2 * BW;
@@ -290,7 +298,7 @@ __device__ __forceinline__ void cnab2_twist_body(const ChainArgs &c, int blk_x, 
     const int lane = threadIdx.x & 63;
     const bool bot = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) != 0;
     const int idx = min(blk_x * 64 + lane, nlm - 1);
-    const int fq = c.f0 + blk_y, ri = blk_z;
+    const int fq = c.f0 + blk_y;
     const bool isC = fq == 5;
     const int f = isC ? 4 : fq;
     const int k = isC ? 3 : (f < 2 ? 0 : (f < 4 ? 1 : 2));
@@ -298,7 +306,7 @@ __device__ __forceinline__ void cnab2_twist_body(const ChainArgs &c, int blk_x, 
     extern __shared__ double smem[];
     // Ex: row i of the explicit operator at [ER*i + q] (column i - BW + q; q = W: 1/r^2);
     // ExB: the same rows mirrored (column i + BW - q at q), so both halves read er[q]
-    double *Ex = smem, *ExB = Ex + ER * n, *rinv_ = ExB + ER * n, *xch = rinv_ + n; // xch: [half][BW][64]
+    double *Ex = smem, *ExB = Ex + ER * n, *rinv_ = ExB + ER * n, *xch = rinv_ + n; // xch: [col][half][BW][64]
     const double *Lapk = c.nuLap + (size_t)k * W * n;
     for (int t = threadIdx.x; t < ER * n; t += 128) {
         const int i = t / ER, q = t - i * ER, j = i - BW + q;
@@ -317,13 +325,25 @@ __device__ __forceinline__ void cnab2_twist_body(const ChainArgs &c, int blk_x, 
     const double *T = (bot ? c.ULt : c.LUt) + (size_t)k * W * n * c.nl + l;
     const double *rdg = (bot ? c.rdiag2 : c.rdiag) + (size_t)k * n * c.nl + l;
     const double *Rl = c.Rint + ((size_t)k * c.nl + l) * (M2 * M2);
-    const size_t fs = (size_t)nlm * n, off = (size_t)(2 * f + ri) * fs + idx;
-    const double *u = (isC ? c.c_in : c.st_in) + off;
+    const size_t fs = (size_t)nlm * n;
     const bool nlz = !isC && ((c.nl_zero_mask >> f) & 1);
-    const double *pn = nlz ? c.zero_row + idx : (isC ? c.c_nl : c.nl_arr) + off;
-    const double *pp = nlz ? c.zero_row + idx : (isC ? c.c_np : c.np_arr) + off;
     const unsigned nstride = nlz ? 0u : (unsigned)nlm;
-    double *o = (isC ? c.c_out : c.st_out) + off;
+    // column cc of this lane: re/im ri = blk_z (NC = 1) or cc (NC = 2)
+    const double *u[NC], *pn[NC], *pp[NC];
+    double *o[NC];
+    TS *pA[NC];
+    const bool wA = f == 0 || f == 1 || f == 4;
+#pragma unroll
+    for (int cc = 0; cc < NC; ++cc) {
+        const int ri = NC == 1 ? blk_z : cc;
+        const size_t off = (size_t)(2 * f + ri) * fs + idx;
+        u[cc] = (isC ? c.c_in : c.st_in) + off;
+        pn[cc] = nlz ? c.zero_row + idx : (isC ? c.c_nl : c.nl_arr) + off;
+        pp[cc] = nlz ? c.zero_row + idx : (isC ? c.c_np : c.np_arr) + off;
+        o[cc] = (isC ? c.c_out : c.st_out) + off;
+        pA[cc] = static_cast<TS *>(c.synV) +
+                 (size_t)((isC ? SYNV_COLS - 2 : (f == 0 ? 2 : (f == 1 ? 0 : 16))) + ri) * nlm + idx;
+    }
     const bool tkeep = isC || ((f == 4) && ((c.bc_mask >> 5) & 1));
     const bool tbc = (f == 4) && !tkeep;
     const bool bcr = isC ? c.c_bcr != 0 : ((c.bc_mask >> f) & 1);
@@ -340,39 +360,45 @@ __device__ __forceinline__ void cnab2_twist_body(const ChainArgs &c, int blk_x, 
         const int jc = (j >= 0 && j < n) ? j : ic;
         return T[(unsigned)(jc * W + (BW + ic - jc)) * (unsigned)c.nl];
     };
-    const bool wA = f == 0 || f == 1 || f == 4;
-    TS *pA = static_cast<TS *>(c.synV) + (size_t)((isC ? SYNV_COLS - 2 : (f == 0 ? 2 : (f == 1 ? 0 : 16))) + ri) * nlm + idx;
-    auto put = [&](int i, double v) { // x row i: state and (u, T/r, C/r) synthesis column
-        o[(unsigned)nlm * i] = v;
-        if (wA) pA[(unsigned)(i * SYNV_COLS) * (unsigned)nlm] = (TS)(f == 4 ? v * rinv_[i] : v);
+    auto put = [&](int cc, int i, double v) { // x row i: state and (u, T/r, C/r) synthesis column
+        o[cc][(unsigned)nlm * i] = v;
+        if (wA) pA[cc][(unsigned)(i * SYNV_COLS) * (unsigned)nlm] = (TS)(f == 4 ? v * rinv_[i] : v);
     };
 
     // ---- outward-in: rhs rows and the unit-triangular substitution
     const int nfw = bot ? nb : n1; // valid steps of this lane
-    double w[W];                   // u rows (top: i-BW..i+BW; bottom: i+BW..i-BW)
+    double w[NC][W];               // u rows (top: i-BW..i+BW; bottom: i+BW..i-BW)
     {
         const int i0 = bot ? n - 1 : 0;
 #pragma unroll
         for (int q = 0; q < W; ++q) {
             const int j = i0 + dir * (q - BW);
-            const double v = u[(unsigned)nlm * cl(j)];
-            w[q] = urow(j) ? v : 0.0;
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) {
+                const double v = u[cc][(unsigned)nlm * cl(j)];
+                w[cc][q] = urow(j) ? v : 0.0;
+            }
         }
     }
-    double su[D], sn[D], sp[D], sl[D][BW];
+    double su[NC][D], sn[NC][D], sp[NC][D], sl[D][BW];
     auto fload = [&](int sl_, int s) {
         const int i = bot ? n - 1 - s : s, ic = cl(i), jn = i + dir * (BW + 1);
-        su[sl_] = u[(unsigned)nlm * cl(jn)]; // masked with urow at its use
-        sn[sl_] = pn[nstride * ic];
-        sp[sl_] = pp[nstride * ic];
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc) {
+            su[cc][sl_] = u[cc][(unsigned)nlm * cl(jn)]; // masked with urow at its use
+            sn[cc][sl_] = pn[cc][nstride * ic];
+            sp[cc][sl_] = pp[cc][nstride * ic];
+        }
 #pragma unroll
         for (int q = 0; q < BW; ++q) {
             sl[sl_][q] = band(ic, i + dir * (q - BW)); // the q-th of the BW rows already eliminated
         }
     };
-    double yw[BW];
+    double yw[NC][BW];
 #pragma unroll
-    for (int q = 0; q < BW; ++q) yw[q] = 0.0;
+    for (int cc = 0; cc < NC; ++cc)
+#pragma unroll
+        for (int q = 0; q < BW; ++q) yw[cc][q] = 0.0;
 #pragma unroll
     for (int d = 0; d < D; ++d) fload(d, d);
     // step s of this half from prefetch slot d; `valid` is false only in the
@@ -380,22 +406,26 @@ __device__ __forceinline__ void cnab2_twist_body(const ChainArgs &c, int blk_x, 
     auto fstep = [&](int d, int s, bool valid) {
         const int i = bot ? n - 1 - s : s, ic = cl(i);
         const double *er = (bot ? ExB : Ex) + ER * ic;
-        double lr = 0.0;
+        const bool unext = urow(i + dir * (BW + 1));
 #pragma unroll
-        for (int q = 0; q < W; ++q) lr += ((q == BW) ? er[BW] - nulf * er[W] : er[q]) * w[q];
-        double v = inv_dt * w[BW] + 1.5 * sn[d] - 0.5 * sp[d];
-        v += lw * lr;
-        v = (bcr && (i == 0 || i == n - 1)) ? (tkeep ? w[BW] : 0.0) : v; // Dirichlet value
+        for (int cc = 0; cc < NC; ++cc) {
+            double lr = 0.0;
 #pragma unroll
-        for (int q = 0; q < BW; ++q) v -= sl[d][q] * yw[q];
-        if (valid) {
+            for (int q = 0; q < W; ++q) lr += ((q == BW) ? er[BW] - nulf * er[W] : er[q]) * w[cc][q];
+            double v = inv_dt * w[cc][BW] + 1.5 * sn[cc][d] - 0.5 * sp[cc][d];
+            v += lw * lr;
+            v = (bcr && (i == 0 || i == n - 1)) ? (tkeep ? w[cc][BW] : 0.0) : v; // Dirichlet value
 #pragma unroll
-            for (int q = 0; q < BW - 1; ++q) yw[q] = yw[q + 1];
-            yw[BW - 1] = v;
+            for (int q = 0; q < BW; ++q) v -= sl[d][q] * yw[cc][q];
+            if (valid) {
 #pragma unroll
-            for (int q = 0; q < W - 1; ++q) w[q] = w[q + 1];
-            w[W - 1] = urow(i + dir * (BW + 1)) ? su[d] : 0.0;
-            o[(unsigned)nlm * i] = v;
+                for (int q = 0; q < BW - 1; ++q) yw[cc][q] = yw[cc][q + 1];
+                yw[cc][BW - 1] = v;
+#pragma unroll
+                for (int q = 0; q < W - 1; ++q) w[cc][q] = w[cc][q + 1];
+                w[cc][W - 1] = unext ? su[cc][d] : 0.0;
+                o[cc][(unsigned)nlm * i] = v;
+            }
         }
         fload(d, s + D);
     };
@@ -408,44 +438,50 @@ __device__ __forceinline__ void cnab2_twist_body(const ChainArgs &c, int blk_x, 
     for (int d = 0; d < D; ++d) fstep(d, s0 + d, s0 + d < nfw); // the last nfw % D steps
 
     // ---- interface: top yw[q] = y(n1-BW+q), bottom yw[q] = z(n1+BW-1-q)
-    double g[M2];
 #pragma unroll
-    for (int q = 0; q < BW; ++q) xch[((bot ? BW : 0) + q) * 64 + lane] = yw[q];
+    for (int cc = 0; cc < NC; ++cc)
+#pragma unroll
+        for (int q = 0; q < BW; ++q) xch[((cc * 2 + (bot ? 1 : 0)) * BW + q) * 64 + lane] = yw[cc][q];
     __syncthreads();
+    double xw[NC][BW]; // the BW rows solved last, nearest first
 #pragma unroll
-    for (int q = 0; q < BW; ++q) {
-        const double other = xch[((bot ? 0 : BW) + q) * 64 + lane];
-        const double mine = yw[q];
-        // g[t] = y(n1-BW+t); g[BW+t] = z(n1+t) = bottom's yw[BW-1-t]
-        if (bot) {
-            g[q] = other;
-            g[M2 - 1 - q] = mine;
-        } else {
-            g[q] = mine;
-            g[M2 - 1 - q] = other;
+    for (int cc = 0; cc < NC; ++cc) {
+        double g[M2];
+#pragma unroll
+        for (int q = 0; q < BW; ++q) {
+            const double other = xch[((cc * 2 + (bot ? 0 : 1)) * BW + q) * 64 + lane];
+            const double mine = yw[cc][q];
+            // g[t] = y(n1-BW+t); g[BW+t] = z(n1+t) = bottom's yw[BW-1-t]
+            if (bot) {
+                g[q] = other;
+                g[M2 - 1 - q] = mine;
+            } else {
+                g[q] = mine;
+                g[M2 - 1 - q] = other;
+            }
         }
+        double xb[BW]; // top: x(n1-BW+t); bottom: x(n1+t)
+#pragma unroll
+        for (int t = 0; t < BW; ++t) {
+            const int a = (bot ? BW : 0) + t;
+            double v = 0.0;
+#pragma unroll
+            for (int b = 0; b < M2; ++b) v += Rl[a * M2 + b] * g[b];
+            xb[t] = v;
+        }
+#pragma unroll
+        for (int t = 0; t < BW; ++t) put(cc, bot ? n1 + t : n1 - BW + t, xb[t]);
+#pragma unroll
+        for (int q = 0; q < BW; ++q) xw[cc][q] = bot ? xb[BW - 1 - q] : xb[q];
     }
-    double xb[BW]; // top: x(n1-BW+t); bottom: x(n1+t)
-#pragma unroll
-    for (int t = 0; t < BW; ++t) {
-        const int a = (bot ? BW : 0) + t;
-        double v = 0.0;
-#pragma unroll
-        for (int b = 0; b < M2; ++b) v += Rl[a * M2 + b] * g[b];
-        xb[t] = v;
-    }
-#pragma unroll
-    for (int t = 0; t < BW; ++t) put(bot ? n1 + t : n1 - BW + t, xb[t]);
 
     // ---- inside-out: back substitution (top: U1 upward; bottom: L2 downward)
     const int nbk = bot ? nb - BW : n1 - BW; // valid steps of this lane
-    double xw[BW];                           // the BW rows solved last, nearest first
-#pragma unroll
-    for (int q = 0; q < BW; ++q) xw[q] = bot ? xb[BW - 1 - q] : xb[q];
-    double by[D], bd[D], bu[D][BW];
+    double by[NC][D], bd[D], bu[D][BW];
     auto bload = [&](int sl_, int t) {
         const int i = bot ? n1 + BW + t : n1 - BW - 1 - t, ic = cl(i);
-        by[sl_] = o[(unsigned)nlm * ic];
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc) by[cc][sl_] = o[cc][(unsigned)nlm * ic];
         bd[sl_] = rdg[(unsigned)ic * (unsigned)c.nl];
 #pragma unroll
         for (int q = 0; q < BW; ++q) {
@@ -456,15 +492,18 @@ __device__ __forceinline__ void cnab2_twist_body(const ChainArgs &c, int blk_x, 
     for (int d = 0; d < D; ++d) bload(d, d);
     auto bstep = [&](int d, int t, bool valid) { // as fstep: only the tail block is guarded
         const int i = bot ? n1 + BW + t : n1 - BW - 1 - t;
-        double v = by[d];
 #pragma unroll
-        for (int q = 0; q < BW; ++q) v -= bu[d][q] * xw[q];
-        v *= bd[d];
-        if (valid) {
+        for (int cc = 0; cc < NC; ++cc) {
+            double v = by[cc][d];
 #pragma unroll
-            for (int q = BW - 1; q > 0; --q) xw[q] = xw[q - 1];
-            xw[0] = v;
-            put(i, v);
+            for (int q = 0; q < BW; ++q) v -= bu[d][q] * xw[cc][q];
+            v *= bd[d];
+            if (valid) {
+#pragma unroll
+                for (int q = BW - 1; q > 0; --q) xw[cc][q] = xw[cc][q - 1];
+                xw[cc][0] = v;
+                put(cc, i, v);
+            }
         }
         bload(d, t + D);
     };
@@ -477,20 +516,23 @@ __device__ __forceinline__ void cnab2_twist_body(const ChainArgs &c, int blk_x, 
     for (int d = 0; d < D; ++d) bstep(d, t0 + d, t0 + d < nbk); // the last nbk % D steps
 }
 
-template <int BW, int D, typename TS> __global__ void __launch_bounds__(128, 3) cnab2_twist_kernel(ChainArgs c)
+// three waves per SIMD hold the one-column form's registers; the paired form
+// launches half the waves and takes up to 256 VGPRs at two
+template <int BW, int D, typename TS, int NC>
+__global__ void __launch_bounds__(128, NC == 1 ? 3 : 2) cnab2_twist_kernel(ChainArgs c)
 {
-    cnab2_twist_body<BW, D, TS>(c, blockIdx.x, blockIdx.y, blockIdx.z);
+    cnab2_twist_body<BW, D, TS, NC>(c, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
-template <int BW, int D, typename TS>
-__global__ void __launch_bounds__(128, 3) cnab2_twist_batch_kernel(BatchRef<ChainArgs> b)
+template <int BW, int D, typename TS, int NC>
+__global__ void __launch_bounds__(128, NC == 1 ? 3 : 2) cnab2_twist_batch_kernel(BatchRef<ChainArgs> b)
 {
     const int sh = b.shell_of_block[blockIdx.x];
     if (sh == BATCH_PAD) return;
     const ChainArgs &c = b.args[sh];
     int bx, by, bz;
-    chain_decode(c, blockIdx.x - b.block_off[sh], bx, by, bz);
-    cnab2_twist_body<BW, D, TS>(c, bx, by, bz);
+    chain_decode<NC>(c, blockIdx.x - b.block_off[sh], bx, by, bz);
+    cnab2_twist_body<BW, D, TS, NC>(c, bx, by, bz);
 }
 } // namespace
 
@@ -519,19 +561,23 @@ void gd_cnab2_chain_form(const ChainArgs &c, ChainForm *f)
 {
     f->lds_lane = sizeof(double) * (size_t)((((2 * c.bw + 1) + 2) & ~1) + 2 * (2 * c.bw + 1) + 2) * c.n;
     f->direct = c.synV && c.syn_tang;
-    f->lds_twist = sizeof(double) * ((size_t)(2 * (((2 * c.bw + 1) + 2) & ~1) + 1) * c.n + 2 * (size_t)c.bw * 64);
+    f->lds_twist = sizeof(double) * ((size_t)(2 * (((2 * c.bw + 1) + 2) & ~1) + 1) * c.n +
+                                     2 * (size_t)c.bw * 64 * (c.pair ? 2 : 1));
     // row, synthesis-column and table offsets from the per-column base pointers
     // fit 32 bits (fewer scalar instructions per level); the twisted kernel needs it
     f->off32 = !c.off64 && (size_t)SYNV_COLS * c.nlm * c.n < ((size_t)1 << 32) &&
                (size_t)(2 * c.bw + 1) * c.n * c.nl < ((size_t)1 << 32);
     f->twist = f->direct && c.ULt && c.rdiag2 && c.Rint && c.bw <= 3 && c.n >= 2 * c.bw + 2 &&
                f->lds_twist <= 64 * 1024 && f->off32;
+    f->pair = f->twist ? c.pair : 0;
 }
 
 int gd_cnab2_chain_blocks(const ChainArgs &c)
 {
     const int nfield = c.nf ? c.nf : (c.c_in ? 6 : 5);
-    return ((c.nlm + 63) / 64) * nfield * 2;
+    ChainForm fm;
+    gd_cnab2_chain_form(c, &fm);
+    return ((c.nlm + 63) / 64) * nfield * (fm.pair ? 1 : 2);
 }
 
 // several shells' column solves in one grid, every shell in form `fm` (as
@@ -542,14 +588,19 @@ int gd_launch_cnab2_chain_batch(hipStream_t st, const BatchRef<ChainArgs> &b, in
     if (nblocks <= 0) return GD_OK;
     GD_CHECK(lds <= 64 * 1024, GD_ERR_ARG, "cnab2_chain batch: %zu B of LDS", lds);
     if (fm.twist) {
+#define GD_TWISTB_NC(B, NC, DD)                                                                                         \
+    {                                                                                                                  \
+        if (syn_fp64) hipLaunchKernelGGL((cnab2_twist_batch_kernel<B, DD, double, NC>), dim3(nblocks), dim3(128), lds, st, b); \
+        else hipLaunchKernelGGL((cnab2_twist_batch_kernel<B, DD, float, NC>), dim3(nblocks), dim3(128), lds, st, b);    \
+    }
 #define GD_TWISTB(B)                                                                                                   \
     if (bw == B) {                                                                                                     \
-        if (syn_fp64) hipLaunchKernelGGL((cnab2_twist_batch_kernel<B, 4, double>), dim3(nblocks), dim3(128), lds, st, b); \
-        else hipLaunchKernelGGL((cnab2_twist_batch_kernel<B, 4, float>), dim3(nblocks), dim3(128), lds, st, b);        \
+        if (fm.pair == 3) GD_TWISTB_NC(B, 2, 3) else if (fm.pair) GD_TWISTB_NC(B, 2, 2) else GD_TWISTB_NC(B, 1, 4)                                                   \
         return gd_check_launch("cnab2_twist_batch_kernel");                                                            \
     }
         GD_TWISTB(1) GD_TWISTB(2) GD_TWISTB(3)
 #undef GD_TWISTB
+#undef GD_TWISTB_NC
         GD_CHECK(false, GD_ERR_ARG, "cnab2_chain batch: twisted form with bw=%d", bw);
     }
     GD_CHECK(fm.direct, GD_ERR_ARG, "cnab2_chain batch: the tangential (direct) form only");
@@ -590,15 +641,20 @@ int gd_launch_cnab2_chain(hipStream_t st, const ChainArgs &c, bool syn_fp64)
     const bool direct = fm.direct, off32 = fm.off32;
     if (fm.twist) {
         // twisted form: 64 columns per block, one wave per half
-        const dim3 tgrid((c.nlm + 63) / 64, grid.y, 2);
+        const dim3 tgrid((c.nlm + 63) / 64, grid.y, fm.pair ? 1 : 2);
+#define GD_TWIST_NC(B, NC, DD)                                                                                    \
+    {                                                                                                            \
+        if (syn_fp64) hipLaunchKernelGGL((cnab2_twist_kernel<B, DD, double, NC>), tgrid, dim3(128), tlds, st, c); \
+        else hipLaunchKernelGGL((cnab2_twist_kernel<B, DD, float, NC>), tgrid, dim3(128), tlds, st, c);           \
+    }
 #define GD_TWIST(B)                                                                                              \
     if (c.bw == B) {                                                                                             \
-        if (syn_fp64) hipLaunchKernelGGL((cnab2_twist_kernel<B, 4, double>), tgrid, dim3(128), tlds, st, c);     \
-        else hipLaunchKernelGGL((cnab2_twist_kernel<B, 4, float>), tgrid, dim3(128), tlds, st, c);               \
+        if (fm.pair == 3) GD_TWIST_NC(B, 2, 3) else if (fm.pair) GD_TWIST_NC(B, 2, 2) else GD_TWIST_NC(B, 1, 4)                                               \
         return gd_check_launch("cnab2_twist_kernel");                                                            \
     }
         GD_TWIST(1) GD_TWIST(2) GD_TWIST(3) // wider bands spill at 3 waves per SIMD
 #undef GD_TWIST
+#undef GD_TWIST_NC
     }
 #define GD_CHAIN_IX(B, IX)                                                                                       \
     {                                                                                                            \

@@ -53,6 +53,9 @@ struct ChainArgs {
     // reciprocal diagonals [k][row][l], interface inverses [k][l][2BW][2BW]
     const double *ULt, *rdiag2, *Rint;
     int off64; // nonzero: 64-bit offsets even where 32 bits suffice (one lane per column)
+    // twisted form: 2 or 3 = each lane solves the re and the im column of its
+    // (idx, field), prefetching that many levels ahead; 0 = one column per lane
+    int pair;
 };
 int gd_cnab2_chain_supported(int n, int bw);
 // host: the twisted-solve tables of one diffusivity from S_l and LU(S_l) in
@@ -65,6 +68,7 @@ int gd_launch_cnab2_chain(hipStream_t st, const ChainArgs &c, bool syn_fp64);
 // the kernel form gd_launch_cnab2_chain picks for c, and its LDS bytes
 struct ChainForm {
     bool twist, off32, direct;
+    int pair; // the twisted form's columns per lane (0: one, 2 or 3: re and im, prefetch depth)
     size_t lds_lane, lds_twist;
 };
 void gd_cnab2_chain_form(const ChainArgs &c, ChainForm *f);

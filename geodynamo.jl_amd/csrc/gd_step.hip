@@ -494,6 +494,7 @@ static void chain_args(gd_step *s, int parity, bool boot, ChainArgs &c)
     c.synS = s->synS.p;
     c.syn_tang = tangential(s) ? 1 : 0;
     c.off64 = s->chain_off64;
+    c.pair = s->chain_pair;
     if (s->twist) {
         c.ULt = s->ULt.as<double>();
         c.rdiag2 = s->rdiag2.as<double>();
@@ -750,6 +751,10 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
     // GD_CHAIN_OFF64 = 1 runs the CNAB2 column kernel with 64-bit offsets
     const char *o64 = getenv("GD_CHAIN_OFF64");
     s->chain_off64 = (o64 && o64[0] == '1') ? 1 : 0;
+    // GD_CHAIN_PAIR = 0 solves one column per lane in the twisted kernel too;
+    // 2 or 3: re and im per lane with that prefetch depth (measurement aid)
+    const char *cp = getenv("GD_CHAIN_PAIR");
+    s->chain_pair = (cp && (cp[0] == '0' || cp[0] == '3')) ? cp[0] - '0' : 2;
     return GD_OK;
 }
 
@@ -1435,7 +1440,7 @@ static int fused_build(gd_batch *b, int parity)
     auto find = [&](int kind, int fp64, int mm, int key, const ChainForm *fm) -> BatchGroup & {
         for (auto &g : groups)
             if (g.lane == cur_lane && g.kind == kind && g.fp64 == fp64 && g.mm == mm && g.key == key &&
-                (!fm || (g.fm.twist == fm->twist && g.fm.off32 == fm->off32)))
+                (!fm || (g.fm.twist == fm->twist && g.fm.off32 == fm->off32 && g.fm.pair == fm->pair)))
                 return g;
         groups.emplace_back();
         BatchGroup &g = groups.back();

@@ -48,6 +48,7 @@ struct gd_step {
     // CNAB2 column kernels with 64-bit offsets where 32 bits suffice (1; the
     // form large shells use, the same bits) or as sized (0)
     int chain_off64 = 0;
+    int chain_pair = 2; // twisted solve: re and im columns in one lane (prefetch depth 2)
     // twisted column solve (gd_chain_twist_tables): tables for the four
     // diffusivity slots; twist = 0 keeps one lane per column
     DevBuf ULt, rdiag2, Rint;

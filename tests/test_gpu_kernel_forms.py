@@ -11,6 +11,10 @@ composition, a small shell and lmax=127, nr=65):
     per column (GD_CHAIN_TWIST=0): the same linear system solved in another
     order, so equal to rounding (max relative difference <= 1e-6 after 3
     steps of the FP32 path, whose nonlinear terms are FP32);
+  * equal to rounding: the twisted solve with the re and im columns of an
+    (idx, field) in one lane (the default) against one column per lane
+    (GD_CHAIN_PAIR=0); the FP64 path has no twisted solve, so there the
+    two are the same kernel (same bits);
   * same bits on the FP64 path: the CNAB2 column kernel with 64-bit offsets
     (GD_CHAIN_OFF64=1, the form shells with SYNV_COLS*nlm*nr >= 2^32 run)
     against the 32-bit offsets used where they fit; on the FP32 path the
@@ -24,7 +28,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-ALT = {"GD_SYNTH_GRAD": "0", "GD_CHAIN_TWIST": "0", "GD_CHAIN_OFF64": "1"}  # the non-default form
+ALT = {"GD_SYNTH_GRAD": "0", "GD_CHAIN_TWIST": "0", "GD_CHAIN_OFF64": "1", "GD_CHAIN_PAIR": "0"}  # non-default forms
 
 
 def _pair(var, prec, L, nlat, nlon, nr, comp):
@@ -60,7 +64,7 @@ def test_kernel_forms_agree(var, prec, comp, L, nlat, nlon, nr):
             s.set_composition(gdhip.seeded_state(52, L, nr, nfields=1, amp=1e-3)[0])
         s.run(3, graph=True)
     a, b = old.get_state_complex(), new.get_state_complex()
-    bitwise = var == "GD_SYNTH_GRAD" or (var == "GD_CHAIN_OFF64" and prec == "fp64")
+    bitwise = var == "GD_SYNTH_GRAD" or (var in ("GD_CHAIN_OFF64", "GD_CHAIN_PAIR") and prec == "fp64")
     if bitwise:
         assert np.array_equal(a, b)
     else:
@@ -76,15 +80,17 @@ def test_kernel_forms_agree(var, prec, comp, L, nlat, nlon, nr):
             assert np.abs(ca - cb).max() <= (1e-13 if prec == "fp64" else 1e-6) * np.abs(ca).max()
 
 
+@pytest.mark.parametrize("pair", ["2", "0"])
 @pytest.mark.parametrize("nr,bw", [(8, 3), (9, 3), (12, 3), (13, 3), (6, 2), (11, 2), (4, 1), (17, 1), (33, 2)])
-def test_twisted_solve_edge_shapes(nr, bw):
-    """The twisted solve at the smallest shell it takes (nr = 2 bw + 2), odd and
-    even nr where only the guarded tail blocks run, and every band width it is
-    built for (bw 1..3), with composition: equal to the one-lane solve to the
-    FP32 path's rounding bound after 3 steps."""
+def test_twisted_solve_edge_shapes(nr, bw, pair):
+    """The twisted solve (both column layouts) at the smallest shell it takes
+    (nr = 2 bw + 2), odd and even nr where only the guarded tail blocks run,
+    and every band width it is built for (bw 1..3), with composition: equal
+    to the one-lane solve to the FP32 path's rounding bound after 3 steps."""
     import gdhip
 
-    print("case: twisted solve edge shape nr=%d bw=%d" % (nr, bw), flush=True)
+    print("case: twisted solve edge shape nr=%d bw=%d pair=%s" % (nr, bw, pair), flush=True)
+    os.environ["GD_CHAIN_PAIR"] = pair
     L, nlat, nlon = 15, 24, 48
     st = gdhip.seeded_state(53, L, nr, amp=1e-3)
     C0 = gdhip.seeded_state(54, L, nr, nfields=1, amp=1e-3)[0]
@@ -98,6 +104,7 @@ def test_twisted_solve_edge_shapes(nr, bw):
         s.enable_composition(2.0)
         out.append(s)
     os.environ.pop("GD_CHAIN_TWIST", None)
+    os.environ.pop("GD_CHAIN_PAIR", None)
     for s in out:
         s.set_state_complex(st)
         s.set_composition(C0)

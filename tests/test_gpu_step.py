@@ -124,6 +124,23 @@ def test_steps_bf16x3_close_to_oracle(oracle):
     assert max(err) < 1e-4
 
 
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-6), ("bf16x3", 1e-4)])
+def test_steps_c2_reduced_precision_against_oracle(oracle, prec, tol):
+    """The C2 workload (lmax=127, 192x384, nr=65, Ra=500) on the mixed and the
+    BF16x3 paths -- the twisted CNAB2 solve with paired columns, the FP32 and
+    split-BF16 transforms -- against the FP64 oracle after 3 steps, per field."""
+    print("case: 3 steps %s C2 vs oracle" % prec, flush=True)
+    oracle.set_threads(8)
+    st, g, o = _make(oracle, prec, 127, 192, 384, 65, Ra=500.0)
+    s, nlp = st, None
+    for k in range(3):
+        s, nl, nlp = o.step(s, nlp, k)
+    g.run(3, graph=True)
+    err = _relf(g.get_state_complex(), s)
+    print("%s C2 state rel err" % prec, err)
+    assert max(err) < tol
+
+
 def test_step_fp64_matches_oracle_c2_shape(oracle):
     print("case: 1 step fp64 lmax=127 nr=65", flush=True)
     st, g, o = _make(oracle, "fp64", lmax=127, nlat=192, nlon=384, nr=65, Ra=500.0)
