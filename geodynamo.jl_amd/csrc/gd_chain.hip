@@ -307,6 +307,9 @@ __device__ __forceinline__ void cnab2_twist_body(const ChainArgs &c, int blk_x, 
     // Ex: row i of the explicit operator at [ER*i + q] (column i - BW + q; q = W: 1/r^2);
     // ExB: the same rows mirrored (column i + BW - q at q), so both halves read er[q]
     double *Ex = smem, *ExB = Ex + ER * n, *rinv_ = ExB + ER * n, *xch = rinv_ + n; // xch: [col][half][BW][64]
+    // ys: the first c.ylds forward results y of each half, [half][s][col][64]
+    // (the rest go through st_out as in the one-column form)
+    double *ys = xch + NC * 2 * BW * 64;
     const double *Lapk = c.nuLap + (size_t)k * W * n;
     for (int t = threadIdx.x; t < ER * n; t += 128) {
         const int i = t / ER, q = t - i * ER, j = i - BW + q;
@@ -403,7 +406,12 @@ __device__ __forceinline__ void cnab2_twist_body(const ChainArgs &c, int blk_x, 
     for (int d = 0; d < D; ++d) fload(d, d);
     // step s of this half from prefetch slot d; `valid` is false only in the
     // tail block, so full blocks shift the windows as plain register renames
-    auto fstep = [&](int d, int s, bool valid) {
+    // y rows kept in LDS: the first Rw of this half (a multiple of D, so the
+    // loop below switches stores between whole blocks), never the last BW
+    // (the interface rows, which the back substitution does not read)
+    const int Rw = NC == 2 ? (min(c.ylds, nfw - BW) / D) * D : 0;
+    double *yh = ys + (size_t)(bot ? c.ylds : 0) * NC * 64 + lane;
+    auto fstep = [&](int d, int s, bool valid, bool yl) {
         const int i = bot ? n - 1 - s : s, ic = cl(i);
         const double *er = (bot ? ExB : Ex) + ER * ic;
         const bool unext = urow(i + dir * (BW + 1));
@@ -424,18 +432,23 @@ __device__ __forceinline__ void cnab2_twist_body(const ChainArgs &c, int blk_x, 
 #pragma unroll
                 for (int q = 0; q < W - 1; ++q) w[cc][q] = w[cc][q + 1];
                 w[cc][W - 1] = unext ? su[cc][d] : 0.0;
-                o[cc][(unsigned)nlm * i] = v;
+                if (yl) yh[(s * NC + cc) * 64] = v;
+                else o[cc][(unsigned)nlm * i] = v;
             }
         }
         fload(d, s + D);
     };
     int s0 = 0;
+    for (; s0 < Rw; s0 += D) { // y to LDS
+#pragma unroll
+        for (int d = 0; d < D; ++d) fstep(d, s0 + d, true, true);
+    }
     for (; s0 + D <= nfw; s0 += D) {
 #pragma unroll
-        for (int d = 0; d < D; ++d) fstep(d, s0 + d, true);
+        for (int d = 0; d < D; ++d) fstep(d, s0 + d, true, false);
     }
 #pragma unroll
-    for (int d = 0; d < D; ++d) fstep(d, s0 + d, s0 + d < nfw); // the last nfw % D steps
+    for (int d = 0; d < D; ++d) fstep(d, s0 + d, s0 + d < nfw, false); // the last nfw % D steps
 
     // ---- interface: top yw[q] = y(n1-BW+q), bottom yw[q] = z(n1+BW-1-q)
 #pragma unroll
@@ -476,12 +489,16 @@ __device__ __forceinline__ void cnab2_twist_body(const ChainArgs &c, int blk_x, 
     }
 
     // ---- inside-out: back substitution (top: U1 upward; bottom: L2 downward)
+    // step t is forward step s = nbk - 1 - t of this half: its y is in LDS
+    // from t = T1 on
     const int nbk = bot ? nb - BW : n1 - BW; // valid steps of this lane
+    const int T1 = nbk - Rw;
     double by[NC][D], bd[D], bu[D][BW];
-    auto bload = [&](int sl_, int t) {
+    auto bload = [&](int sl_, int t, bool yl) {
         const int i = bot ? n1 + BW + t : n1 - BW - 1 - t, ic = cl(i);
+        if (!yl)
 #pragma unroll
-        for (int cc = 0; cc < NC; ++cc) by[cc][sl_] = o[cc][(unsigned)nlm * ic];
+            for (int cc = 0; cc < NC; ++cc) by[cc][sl_] = o[cc][(unsigned)nlm * ic];
         bd[sl_] = rdg[(unsigned)ic * (unsigned)c.nl];
 #pragma unroll
         for (int q = 0; q < BW; ++q) {
@@ -489,12 +506,12 @@ __device__ __forceinline__ void cnab2_twist_body(const ChainArgs &c, int blk_x, 
         }
     };
 #pragma unroll
-    for (int d = 0; d < D; ++d) bload(d, d);
-    auto bstep = [&](int d, int t, bool valid) { // as fstep: only the tail block is guarded
+    for (int d = 0; d < D; ++d) bload(d, d, false);
+    auto bstep = [&](int d, int t, bool valid, bool yl) { // as fstep: only the tail blocks are guarded
         const int i = bot ? n1 + BW + t : n1 - BW - 1 - t;
 #pragma unroll
         for (int cc = 0; cc < NC; ++cc) {
-            double v = by[cc][d];
+            double v = yl ? yh[((nbk - 1 - t) * NC + cc) * 64] : by[cc][d];
 #pragma unroll
             for (int q = 0; q < BW; ++q) v -= bu[d][q] * xw[cc][q];
             v *= bd[d];
@@ -505,15 +522,27 @@ __device__ __forceinline__ void cnab2_twist_body(const ChainArgs &c, int blk_x, 
                 put(cc, i, v);
             }
         }
-        bload(d, t + D);
+        bload(d, t + D, yl);
     };
     int t0 = 0;
-    for (; t0 + D <= nbk; t0 += D) {
+    for (; t0 + D <= T1; t0 += D) {
 #pragma unroll
-        for (int d = 0; d < D; ++d) bstep(d, t0 + d, true);
+        for (int d = 0; d < D; ++d) bstep(d, t0 + d, true, false);
     }
 #pragma unroll
-    for (int d = 0; d < D; ++d) bstep(d, t0 + d, t0 + d < nbk); // the last nbk % D steps
+    for (int d = 0; d < D; ++d) bstep(d, t0 + d, t0 + d < T1, false); // the last T1 % D steps
+    if (Rw > 0) {
+        // the LDS rows: restart the table prefetch at T1, y read at its use
+        t0 = T1;
+#pragma unroll
+        for (int d = 0; d < D; ++d) bload(d, t0 + d, true);
+        for (; t0 + D <= nbk; t0 += D) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) bstep(d, t0 + d, true, true);
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) bstep(d, t0 + d, t0 + d < nbk, true);
+    }
 }
 
 // three waves per SIMD hold the one-column form's registers; the paired form
@@ -562,7 +591,8 @@ void gd_cnab2_chain_form(const ChainArgs &c, ChainForm *f)
     f->lds_lane = sizeof(double) * (size_t)((((2 * c.bw + 1) + 2) & ~1) + 2 * (2 * c.bw + 1) + 2) * c.n;
     f->direct = c.synV && c.syn_tang;
     f->lds_twist = sizeof(double) * ((size_t)(2 * (((2 * c.bw + 1) + 2) & ~1) + 1) * c.n +
-                                     2 * (size_t)c.bw * 64 * (c.pair ? 2 : 1));
+                                     2 * (size_t)c.bw * 64 * (c.pair ? 2 : 1) +
+                                     (c.pair ? 2 * (size_t)c.ylds * 2 * 64 : 0));
     // row, synthesis-column and table offsets from the per-column base pointers
     // fit 32 bits (fewer scalar instructions per level); the twisted kernel needs it
     f->off32 = !c.off64 && (size_t)SYNV_COLS * c.nlm * c.n < ((size_t)1 << 32) &&
@@ -570,6 +600,19 @@ void gd_cnab2_chain_form(const ChainArgs &c, ChainForm *f)
     f->twist = f->direct && c.ULt && c.rdiag2 && c.Rint && c.bw <= 3 && c.n >= 2 * c.bw + 2 &&
                f->lds_twist <= 64 * 1024 && f->off32;
     f->pair = f->twist ? c.pair : 0;
+}
+
+int gd_chain_ylds(const ChainArgs &c, size_t lds_budget)
+{
+    ChainArgs b = c;
+    b.ylds = 0;
+    ChainForm fm;
+    gd_cnab2_chain_form(b, &fm);
+    if (!fm.pair || lds_budget <= fm.lds_twist) return 0;
+    const size_t per_row = sizeof(double) * 2 * 2 * 64; // both halves, re and im, 64 lanes
+    const int rows = (int)((lds_budget - fm.lds_twist) / per_row);
+    const int n1 = (c.n + 1) / 2;
+    return std::max(0, std::min(rows, n1 - c.bw));
 }
 
 int gd_cnab2_chain_blocks(const ChainArgs &c)

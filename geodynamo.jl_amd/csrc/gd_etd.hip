@@ -344,6 +344,7 @@ struct DenseProg {
     const int *slot_of_l; // l -> matrix slot (the host has checked every l of the field)
     const int4 *tiles;    // (l, first lm column, column count, 0)
     int nr, nlm, ldm;
+    int nch;              // row chunks per (tile, plane)
 };
 
 // Launch shape: wave w of a block computes the 16-row tile w of the block's
@@ -356,8 +357,10 @@ __host__ __device__ inline int kpad_of(int nr) { return (nr + 3) & ~3; }
 // LDS: x tile (kpad x 32) then the block's matrix rows (kpad columns of 16 W)
 inline size_t dense_lds(int nr) { return sizeof(double) * (size_t)kpad_of(nr) * (TC + 16 * waves_of(nr)); }
 
-// Block (tile of <= 32 columns of one l, plane blockIdx.y, row chunk
-// blockIdx.z).  Per pass the x tile is loaded (coalesced, two rows per wave
+// Block (tile of <= 32 columns of one l, plane, row chunk), from a 1-D grid
+// in XCD-aware order: the blocks of one tile (both planes, every chunk) and
+// the tiles of one l are consecutive logical blocks, so the blocks that read
+// one matrix share an XCD's L2.  Per pass the x tile is loaded (coalesced, two rows per wave
 // load; x columns past the tile are clamped copies whose results are not
 // stored) and, when it changes, the block's matrix rows are copied global ->
 // LDS by the lanes directly (global_load_lds, 16 bytes per lane, no
@@ -372,8 +375,10 @@ template <int W> __global__ void __launch_bounds__(64 * W) dense_mfma_kernel(Den
     const int n = a.nr, ldm = a.ldm, kp = kpad_of(n);
     double *Sx = sm;           // [k][TC]
     double *Sm = sm + kp * TC; // [k][RB]: Sm[k RB + rr] = M(r0 + rr, k)
-    const int4 t = a.tiles[blockIdx.x];
-    const int pl = blockIdx.y, r0 = blockIdx.z * RB, tid = threadIdx.x;
+    const int L = xcd_logical(blockIdx.x, gridDim.x);
+    const int ch = L % a.nch, pl = (L / a.nch) & 1;
+    const int4 t = a.tiles[L / (2 * a.nch)];
+    const int r0 = ch * RB, tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6, ncol = t.z;
     const size_t nlm = (size_t)a.nlm, col0 = (size_t)t.y;
     const size_t moff = (size_t)a.slot_of_l[t.x] * n * ldm;
@@ -426,7 +431,7 @@ template <int W> __global__ void __launch_bounds__(64 * W) dense_mfma_kernel(Den
         }
         __builtin_amdgcn_s_waitcnt(0); // this wave's direct LDS copies have landed
         __syncthreads();
-        if (P.xcopy[pl] && blockIdx.z == 0) {
+        if (P.xcopy[pl] && ch == 0) {
             for (int e = tid; e < n * TC; e += 64 * W)
                 if ((e & (TC - 1)) < ncol) P.xcopy[pl][col0 + (e & (TC - 1)) + nlm * (e >> 5)] = Sx[e];
         }
@@ -608,7 +613,8 @@ int launch_prog(gd_exp_cache *c, DenseProg a, int lmax, int mmax)
     a.nlm = gd_nlm_of(lmax, mmax);
     a.ldm = c->ldm;
     const int W = waves_of(c->nr);
-    const dim3 grid(c->ntiles, 2, chunks_of(c->nr)), block(64 * W);
+    a.nch = chunks_of(c->nr);
+    const dim3 grid(c->ntiles * 2 * a.nch), block(64 * W);
     const size_t lds = dense_lds(c->nr);
     hipStream_t st = c->ctx->stream;
     switch (W) {

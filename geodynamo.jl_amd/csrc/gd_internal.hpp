@@ -134,6 +134,17 @@ template <> __device__ __forceinline__ int acc_row<double>(int lane, int v) { re
 
 template <typename T> struct Cplx { T re, im; };
 
+// XCD-aware block order: workgroup b of an n-block launch runs on XCD b % 8,
+// and each XCD has its own L2, so a kernel whose consecutive logical blocks
+// share operands takes logical block xcd_logical(b, n): XCD x gets the x-th
+// contiguous range of the logical order (a bijection of 0 .. n-1).
+__device__ __forceinline__ int xcd_logical(int b, int n)
+{
+    constexpr int NX = 8;
+    const int q = n / NX, rem = n % NX, x = b % NX, s = b / NX;
+    return (x < rem ? x * (q + 1) : rem * (q + 1) + (x - rem) * q) + s;
+}
+
 // ---------------------------------------------------------------- FFT plan
 struct FftPlan {
     int n = 0;

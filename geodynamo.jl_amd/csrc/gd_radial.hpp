@@ -56,6 +56,9 @@ struct ChainArgs {
     // twisted form: 2 or 3 = each lane solves the re and the im column of its
     // (idx, field), prefetching that many levels ahead; 0 = one column per lane
     int pair;
+    // paired twisted form: forward results y of the first ylds rows of each
+    // half kept in LDS instead of st_out (gd_chain_ylds picks it)
+    int ylds;
 };
 int gd_cnab2_chain_supported(int n, int bw);
 // host: the twisted-solve tables of one diffusivity from S_l and LU(S_l) in
@@ -68,11 +71,17 @@ int gd_launch_cnab2_chain(hipStream_t st, const ChainArgs &c, bool syn_fp64);
 // the kernel form gd_launch_cnab2_chain picks for c, and its LDS bytes
 struct ChainForm {
     bool twist, off32, direct;
-    int pair; // the twisted form's columns per lane (0: one, 2 or 3: re and im, prefetch depth)
+    int pair;
+    // paired twisted form: forward results y of the first ylds rows of each
+    // half kept in LDS instead of st_out (gd_chain_ylds picks it)
+    int ylds; // the twisted form's columns per lane (0: one, 2 or 3: re and im, prefetch depth)
     size_t lds_lane, lds_twist;
 };
 void gd_cnab2_chain_form(const ChainArgs &c, ChainForm *f);
 int gd_cnab2_chain_blocks(const ChainArgs &c);
+// LDS rows of y per half for the paired twisted form of c (c.ylds unset):
+// as many as keep a block within lds_budget bytes
+int gd_chain_ylds(const ChainArgs &c, size_t lds_budget);
 int gd_launch_cnab2_chain_batch(hipStream_t st, const BatchRef<ChainArgs> &b, int nblocks, int bw, const ChainForm &fm,
                                 size_t lds, bool syn_fp64);
 int gd_launch_energies(hipStream_t st, const double *state, const int *l_of_idx, int nlm, int n, double *part,

@@ -96,13 +96,7 @@ template <typename T> __device__ __forceinline__ T shfl_xor3(T v) { return __shf
 // so that consecutive logical blocks run on one XCD: XCD x takes the x-th
 // contiguous range of the logical order.  The m dimension is ordered
 // 0, mmax, 1, mmax-1, ... so that equal ranges carry equal work (the degree
-// count of m falls linearly with m).
-__device__ __forceinline__ int xcd_logical(int b, int n)
-{
-    constexpr int NX = 8;
-    const int q = n / NX, rem = n % NX, x = b % NX, s = b / NX;
-    return (x < rem ? x * (q + 1) : rem * (q + 1) + (x - rem) * q) + s;
-}
+// count of m falls linearly with m).  (xcd_logical: gd_internal.hpp)
 __device__ __forceinline__ int paired_m(int k, int nm) { return (k & 1) ? nm - 1 - (k >> 1) : (k >> 1); }
 
 // 16-byte vector of T (stores of the staged output tiles)

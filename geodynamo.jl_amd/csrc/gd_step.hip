@@ -21,6 +21,7 @@
 //      simulation.jl:1153), rhs + FP64 banded solve per column
 //      (timestep.jl:610-730); the nl history rolls by buffer swap
 //      (simulation.jl:1312-1329).
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 
@@ -495,6 +496,7 @@ static void chain_args(gd_step *s, int parity, bool boot, ChainArgs &c)
     c.syn_tang = tangential(s) ? 1 : 0;
     c.off64 = s->chain_off64;
     c.pair = s->chain_pair;
+    c.ylds = 0;
     if (s->twist) {
         c.ULt = s->ULt.as<double>();
         c.rdiag2 = s->rdiag2.as<double>();
@@ -506,6 +508,14 @@ static void chain_args(gd_step *s, int parity, bool boot, ChainArgs &c)
         c.c_nl = s->cnl[parity].as<double>();
         c.c_np = boot ? c.c_nl : s->cnl[parity ^ 1].as<double>();
         c.c_bcr = s->d.bc_rows == 1;
+    }
+    // the paired twisted solve keeps as many forward results in LDS as leave
+    // the launch in one round of resident blocks (at most 64 KB a block)
+    if (s->chain_ylds != 0) {
+        const long blocks = (long)gd_cnab2_chain_blocks(c), per_cu = std::max(1L, (blocks + 255) / 256);
+        size_t budget = std::min<size_t>(64 * 1024, 160 * 1024 / per_cu);
+        c.ylds = gd_chain_ylds(c, budget);
+        if (s->chain_ylds > 0) c.ylds = std::min(c.ylds, s->chain_ylds);
     }
 }
 
@@ -755,6 +765,10 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
     // 2 or 3: re and im per lane with that prefetch depth (measurement aid)
     const char *cp = getenv("GD_CHAIN_PAIR");
     s->chain_pair = (cp && (cp[0] == '0' || cp[0] == '3')) ? cp[0] - '0' : 2;
+    // GD_CHAIN_YLDS = 0 keeps every forward result y in st_out; k > 0 caps the
+    // LDS rows per half at k (measurement aids)
+    const char *yl = getenv("GD_CHAIN_YLDS");
+    s->chain_ylds = yl ? atoi(yl) : -1;
     return GD_OK;
 }
 
@@ -1388,6 +1402,12 @@ static bool fused_args(gd_step *s, int parity, ShellArgs &x)
     int mode;
     tang_args(s, s->nlbuf[parity].as<double>(), x.a, x.r, x.gv, x.gs, &mode);
     chain_args(s, parity, false, x.c);
+    if (x.c.ylds > 0) {
+        // a batched grid holds many rounds of blocks: LDS for the four
+        // resident blocks per CU that the paired kernel's registers allow
+        x.c.ylds = gd_chain_ylds(x.c, 40 * 1024);
+        if (s->chain_ylds > 0) x.c.ylds = std::min(x.c.ylds, s->chain_ylds);
+    }
     gd_cnab2_chain_form(x.c, &x.fm);
     x.fp64 = s->d.prec == GD_PREC_FP64;
     x.kn = gd_ana_kn(x.a.nlat2, x.fp64);
