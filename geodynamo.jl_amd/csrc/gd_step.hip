@@ -765,10 +765,11 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
     // 2 or 3: re and im per lane with that prefetch depth (measurement aid)
     const char *cp = getenv("GD_CHAIN_PAIR");
     s->chain_pair = (cp && (cp[0] == '0' || cp[0] == '3')) ? cp[0] - '0' : 2;
-    // GD_CHAIN_YLDS = 0 keeps every forward result y in st_out; k > 0 caps the
-    // LDS rows per half at k (measurement aids)
+    // GD_CHAIN_YLDS: forward results y of the paired twisted solve in LDS --
+    // -1 as many rows as fit, k > 0 at most k rows per half; unset or 0: all
+    // through st_out (measured faster; a measurement aid)
     const char *yl = getenv("GD_CHAIN_YLDS");
-    s->chain_ylds = yl ? atoi(yl) : -1;
+    s->chain_ylds = yl ? atoi(yl) : 0;
     return GD_OK;
 }
 
