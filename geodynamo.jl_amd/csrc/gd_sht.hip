@@ -17,6 +17,7 @@
 #include <cmath>
 
 #include <cstdint>
+#include <cstdlib>
 #include <type_traits>
 
 #include "gd_sht.hpp"
@@ -1577,6 +1578,129 @@ template <typename T, int MODE, int NC> __global__ void __launch_bounds__(256) r
     ring_body<T, MODE, NC>(a, i, r);
 }
 
+// The boundary passes (RING_SYNTH: Fourier -> physical rings, RING_ANALYSIS:
+// physical rings -> Fourier) with RG consecutive ring pairs per block.  The
+// physical arrays are colatitude-fastest, so one ring's points lie a whole
+// row of nlat doubles apart: with one ring pair per block every lane's load or
+// store is its own cache line.  Here lanes take (longitude, ring) with the
+// ring fastest, so a wave's access covers runs of RG consecutive colatitudes
+// (and their southern mirrors); the RG * NF transforms of the block share the
+// twiddles and one batched FFT.  Each transform gets the operations of the
+// one-ring form.
+template <typename T, int MODE, int NC, int RG>
+__device__ __forceinline__ void ring_io_body(const RingArgs &a, int g, int r)
+{
+    static_assert(MODE == RING_SYNTH || MODE == RING_ANALYSIS, "boundary passes only");
+    extern __shared__ char smem_raw[];
+    const int n = NC > 0 ? NC : a.nlon, nm = a.mmax + 1;
+    const int ld = ring_ld(n);
+    C2<T> *tw = reinterpret_cast<C2<T> *>(smem_raw);
+    C2<T> *buf = tw + n;
+    __shared__ int srad[12];
+    __shared__ int sfn[16], sfs[16];
+    const int tid = threadIdx.x;
+    constexpr int nthr = RING_THREADS;
+    const int NF = a.nfield;
+    const int i0 = g * RG, nq = min(RG, a.nlat2 - i0); // this block's ring pairs i0 .. i0 + nq - 1
+    const int NT = nq * NF;                             // transform t = q * NF + F
+    const C2<T> *twg = static_cast<const C2<T> *>(a.tw);
+    for (int k = tid; k < n; k += nthr) tw[k] = twg[k];
+    if (tid < 12) srad[tid] = a.radix[tid];
+    if (tid < 16) {
+        sfn[tid] = a.fn_off[tid];
+        sfs[tid] = a.fs_off[tid];
+    }
+    __syncthreads();
+    if (MODE == RING_SYNTH) {
+        // Hermitian spectra of the north (X) and south (Y) rings packed as Z = X + iY
+        for (int e0 = 0; e0 < NT * nm; e0 += 8 * nthr) {
+            T xr[8], xi[8], yr[8], yi[8];
+#pragma unroll
+            for (int it = 0; it < 8; ++it) { // loads of 8 elements in flight before their stores
+                const int e = e0 + tid + it * nthr;
+                if (e < NT * nm) {
+                    const int F = e % NF, m = (e / NF) % nm, q = e / (NF * nm);
+                    const T *s = static_cast<const T *>(a.fb) + (((size_t)(i0 + q) * a.nr + r) * nm + m) * a.nslot_f;
+                    xr[it] = s[sfn[F]];
+                    xi[it] = s[sfn[F] + 1];
+                    yr[it] = s[sfs[F]];
+                    yi[it] = s[sfs[F] + 1];
+                }
+            }
+#pragma unroll
+            for (int it = 0; it < 8; ++it) {
+                const int e = e0 + tid + it * nthr;
+                if (e < NT * nm) {
+                    const int F = e % NF, m = (e / NF) % nm, q = e / (NF * nm);
+                    C2<T> *b = buf + (q * NF + F) * ld;
+                    if (m == 0) {
+                        b[0] = C2<T>{xr[it], yr[it]};
+                    } else {
+                        b[sk(m)] = C2<T>{xr[it] - yi[it], xi[it] + yr[it]};
+                        b[sk(n - m)] = C2<T>{xr[it] + yi[it], yr[it] - xi[it]};
+                    }
+                }
+            }
+        }
+        for (int k = nm + tid; k <= n - nm; k += nthr)
+            for (int t = 0; t < NT; ++t) buf[t * ld + sk(k)] = C2<T>{T(0), T(0)};
+        __syncthreads();
+        if constexpr (NC > 0) fft_ct<T, NC, +1, 0>(buf, tw, NT);
+        else fft_batch<T>(buf, tw, NT, n, ld, srad, a.nstages, +1);
+        for (int e = tid; e < n * nq; e += nthr) {
+            const int q = e % nq, j = e / nq, i = i0 + q, is = a.nlat - 1 - i;
+            for (int F = 0; F < NF; ++F) {
+                double *o = a.phys_out[F == 0 ? 0 : 1];
+                const C2<T> z = buf[(q * NF + F) * ld + sk(j)];
+                o[((size_t)r * n + j) * a.nlat + i] = (double)z.x;
+                o[((size_t)r * n + j) * a.nlat + is] = (double)z.y;
+            }
+        }
+        return;
+    }
+    for (int e = tid; e < n * nq; e += nthr) {
+        const int q = e % nq, j = e / nq, i = i0 + q, is = a.nlat - 1 - i;
+        for (int F = 0; F < NF; ++F) {
+            const double *p = a.phys_in[F == 0 ? 0 : 1];
+            buf[(q * NF + F) * ld + sk(j)] =
+                C2<T>{(T)p[((size_t)r * n + j) * a.nlat + i], (T)p[((size_t)r * n + j) * a.nlat + is]};
+        }
+    }
+    __syncthreads();
+    if constexpr (NC > 0) fft_ct<T, NC, -1, 0>(buf, tw, NT);
+    else fft_batch<T>(buf, tw, NT, n, ld, srad, a.nstages, -1);
+    // split Z into north X and south Y, form G+- = (X +- Y) * 2 pi w_i / n
+    T *gb = static_cast<T *>(a.gb);
+    for (int e = tid; e < NT * nm; e += nthr) {
+        const int F = e % NF, m = (e / NF) % nm, q = e / (NF * nm), i = i0 + q;
+        const T c = static_cast<const T *>(a.wq)[i];
+        const C2<T> *b = buf + (q * NF + F) * ld;
+        const C2<T> Z = b[sk(m)];
+        C2<T> X, Y;
+        if (m == 0) {
+            X = C2<T>{Z.x, T(0)};
+            Y = C2<T>{Z.y, T(0)};
+        } else {
+            const C2<T> Zc = b[sk(n - m)];
+            X = C2<T>{T(0.5) * (Z.x + Zc.x), T(0.5) * (Z.y - Zc.y)};
+            Y = C2<T>{T(0.5) * (Z.y + Zc.y), T(0.5) * (Zc.x - Z.x)};
+        }
+        const size_t rec = ((size_t)m * a.nlat2 + i) * a.nr + r;
+        T *d = gb + rec * a.nslot_g + 4 * F;
+        d[0] = (X.x + Y.x) * c;
+        d[1] = (X.y + Y.y) * c;
+        d[2] = (X.x - Y.x) * c;
+        d[3] = (X.y - Y.y) * c;
+    }
+}
+
+// blocks (ring group, level) in XCD-aware order, as the one-ring boundary form
+template <typename T, int MODE, int NC, int RG> __global__ void __launch_bounds__(256) ring_io_kernel(RingArgs a)
+{
+    const int L = xcd_logical(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    ring_io_body<T, MODE, NC, RG>(a, L % gridDim.x, L / gridDim.x);
+}
+
 // one grid over the ring passes of several shells (step modes): a shell's
 // blocks are its (node, level) pairs, node fastest
 template <typename T, int MODE, int NC> __global__ void __launch_bounds__(256) ring_batch_kernel(BatchRef<RingArgs> b)
@@ -1635,10 +1759,55 @@ int gd_ring_tang_batchable(const RingArgs &a, bool fp64)
     return GD_OK;
 }
 
+template <typename T, int MODE, int RG> static void ring_io_nc(hipStream_t st, dim3 grid, size_t lds, const RingArgs &a)
+{
+    switch (a.nlon) {
+    case 384: hipLaunchKernelGGL((ring_io_kernel<T, MODE, 384, RG>), grid, dim3(RING_THREADS), lds, st, a); break;
+    case 768: hipLaunchKernelGGL((ring_io_kernel<T, MODE, 768, RG>), grid, dim3(RING_THREADS), lds, st, a); break;
+    default: hipLaunchKernelGGL((ring_io_kernel<T, MODE, 0, RG>), grid, dim3(RING_THREADS), lds, st, a); break;
+    }
+}
+
+// ring pairs per block of the boundary passes: 2 for FP64 rings, 4 for FP32
+// (measured at lmax=255, 384 x 768, 97 levels: FP64 round trip 5.78 / 5.40 /
+// 5.58 / 6.10 ms and FP32 4.08 / 3.61 / 3.54 / 3.73 ms for 1 / 2 / 4 / 8;
+// GD_RING_RG = 1, 2, 4 or 8 overrides, a measurement aid), halved until the
+// block's transforms fit the FFT's per-thread registers (RG nfield nlon <=
+// MAXC threads) and 160 KB of LDS
+template <typename T> static int ring_io_rg(const RingArgs &a, int nf)
+{
+    static const int env = [] {
+        const char *e = getenv("GD_RING_RG");
+        const int v = e ? atoi(e) : 0;
+        return (v == 1 || v == 2 || v == 4 || v == 8) ? v : 0;
+    }();
+    int rg = env ? env : (sizeof(T) == 8 ? 2 : 4);
+    while (rg > 1 && ((long)rg * nf * a.nlon > (long)MAXC * RING_THREADS ||
+                      sizeof(C2<T>) * ((size_t)ring_ld(a.nlon) * nf * rg + a.nlon) > 160 * 1024))
+        rg /= 2;
+    return rg;
+}
+
 template <typename T> int gd_launch_ring(hipStream_t st, int mode, const RingArgs &a)
 {
     int nf;
     GD_TRY(ring_check(mode, a, &nf));
+    if (mode == RING_SYNTH || mode == RING_ANALYSIS) {
+        const int rg = ring_io_rg<T>(a, nf);
+        if (rg > 1) {
+            const size_t lds = sizeof(C2<T>) * ((size_t)ring_ld(a.nlon) * nf * rg + a.nlon);
+            const dim3 grid((a.nlat2 + rg - 1) / rg, a.nr);
+#define GD_RIO(MODE)                                                                 \
+    {                                                                                \
+        if (rg == 8) ring_io_nc<T, MODE, 8>(st, grid, lds, a);                       \
+        else if (rg == 4) ring_io_nc<T, MODE, 4>(st, grid, lds, a);                  \
+        else ring_io_nc<T, MODE, 2>(st, grid, lds, a);                               \
+    }
+            if (mode == RING_SYNTH) GD_RIO(RING_SYNTH) else GD_RIO(RING_ANALYSIS)
+#undef GD_RIO
+            return gd_check_launch("ring_io_kernel");
+        }
+    }
     size_t lds = sizeof(C2<T>) * ((size_t)ring_ld(a.nlon) * nf + a.nlon);
     GD_CHECK(lds <= 160 * 1024, GD_ERR_ARG, "ring: LDS %zu too large", lds);
     dim3 grid(a.nlat2, a.nr);

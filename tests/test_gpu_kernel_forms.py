@@ -15,6 +15,9 @@ composition, a small shell and lmax=127, nr=65):
     (idx, field) in one lane (the default) against one column per lane
     (GD_CHAIN_PAIR=0); the FP64 path has no twisted solve, so there the
     two are the same kernel (same bits);
+  * same bits: the paired twisted solve with the first rows of each half's
+    forward results in LDS (GD_CHAIN_YLDS=-1) against all of them through
+    the state buffer (the default): the same operations, another store;
   * same bits on the FP64 path: the CNAB2 column kernel with 64-bit offsets
     (GD_CHAIN_OFF64=1, the form shells with SYNV_COLS*nlm*nr >= 2^32 run)
     against the 32-bit offsets used where they fit; on the FP32 path the
@@ -28,7 +31,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-ALT = {"GD_SYNTH_GRAD": "0", "GD_CHAIN_TWIST": "0", "GD_CHAIN_OFF64": "1", "GD_CHAIN_PAIR": "0"}  # non-default forms
+ALT = {"GD_SYNTH_GRAD": "0", "GD_CHAIN_TWIST": "0", "GD_CHAIN_OFF64": "1", "GD_CHAIN_PAIR": "0",
+       "GD_CHAIN_YLDS": "-1"}  # non-default forms
 
 
 def _pair(var, prec, L, nlat, nlon, nr, comp):
@@ -64,7 +68,7 @@ def test_kernel_forms_agree(var, prec, comp, L, nlat, nlon, nr):
             s.set_composition(gdhip.seeded_state(52, L, nr, nfields=1, amp=1e-3)[0])
         s.run(3, graph=True)
     a, b = old.get_state_complex(), new.get_state_complex()
-    bitwise = var == "GD_SYNTH_GRAD" or (var in ("GD_CHAIN_OFF64", "GD_CHAIN_PAIR") and prec == "fp64")
+    bitwise = var in ("GD_SYNTH_GRAD", "GD_CHAIN_YLDS") or (var in ("GD_CHAIN_OFF64", "GD_CHAIN_PAIR") and prec == "fp64")
     if bitwise:
         assert np.array_equal(a, b)
     else:
