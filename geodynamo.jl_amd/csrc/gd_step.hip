@@ -1687,6 +1687,13 @@ extern "C" int gd_batch_run(gd_batch *b, int nsteps)
     GD_HIP(hipSetDevice(b->ctx->device));
     const int s0 = b->steps[0]->step;
     for (gd_step *s : b->steps) GD_CHECK(s->step == s0, GD_ERR_ARG, "gd_batch_run: shells at different steps");
+    // the per-shell stream branches run FP64 shells only: with FP32 shells
+    // their graph capture once ended in a host fault whose cause was not
+    // found (the fused form runs every precision)
+    if (b->mode == GD_BATCH_STREAMS)
+        for (gd_step *s : b->steps)
+            GD_CHECK(s->d.prec == GD_PREC_FP64, GD_ERR_ARG,
+                     "gd_batch_run: the stream-branch form takes FP64 shells; use GD_BATCH_FUSED for FP32 and BF16x3");
     for (int k = 0; k < nsteps; ++k) {
         const int step = b->steps[0]->step;
         if (step == 0) {

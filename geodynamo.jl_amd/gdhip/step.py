@@ -188,8 +188,9 @@ class StepBatch:
     parity holds every shell's step.  The shells must be at the same step
     count; results equal each shell's own run(graph=True) bit for bit.
 
-    mode "streams": the shells' own steps spread over `nstreams` streams.
-    mode "fused": one launch per kernel form and stage over every shell of
+    mode "streams": the shells' own steps spread over `nstreams` streams
+    (FP64 shells only).
+    mode "fused" (default): one launch per kernel form and stage over every shell of
     that form (synthesis, ring pass, both analyses, CNAB2 solves), on one
     stream, or split over `lanes` streams.  tiles (fused mode): "table"
     takes each shell's Legendre-synthesis tile from the committed tuning table
@@ -197,7 +198,7 @@ class StepBatch:
     device, or a list gives them.  lanes: the fused shells' stream count
     (tune_lanes measures it; it advances the shells by real steps)."""
 
-    def __init__(self, steps, nstreams=8, mode="streams", tiles="table", lanes=1):
+    def __init__(self, steps, nstreams=8, mode="fused", tiles="table", lanes=1):
         if not steps:
             raise ValueError("empty batch")
         if mode not in BATCH_MODES:

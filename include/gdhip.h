@@ -233,7 +233,7 @@ int gd_batch_destroy(gd_batch *b);
  * context stream.  Shells whose step has no fused form (composition, full
  * products, radials) run their own step in the same graph.  Results equal
  * each shell's own gd_step_run_graph bit for bit. */
-#define GD_BATCH_STREAMS 0
+#define GD_BATCH_STREAMS 0 /* FP64 shells only (GD_ERR_ARG otherwise) */
 #define GD_BATCH_FUSED 1
 int gd_batch_set_mode(gd_batch *b, int mode);
 /* Legendre-synthesis tile of each shell in the fused form, tiles[n]:

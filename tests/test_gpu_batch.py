@@ -36,7 +36,7 @@ def test_batch_equals_separate_runs(prec):
             group.append(s)
     for s in alone:
         s.run(6, graph=True)
-    batch = gdhip.StepBatch(batched, nstreams=3)
+    batch = gdhip.StepBatch(batched, nstreams=3, mode="streams")
     batch.run(6)
     batch.synchronize()
     for a, b in zip(alone, batched):
@@ -44,6 +44,20 @@ def test_batch_equals_separate_runs(prec):
         sa, sb = a.get_state_complex(), b.get_state_complex()
         assert np.all(np.isfinite(sb))
         assert np.array_equal(sa, sb)
+    batch.close()
+
+
+def test_stream_form_refuses_reduced_precision_shells():
+    """The stream-branch form takes FP64 shells only; FP32 shells run in the
+    fused form (the refusal comes before any capture or launch)."""
+    import gdhip
+
+    s = gdhip.CNAB2Step(15, 24, 48, 9, precision="fp32")
+    s.set_state_complex(gdhip.seeded_state(3, 15, 9, amp=1e-3))
+    batch = gdhip.StepBatch([s], mode="streams")
+    with pytest.raises(gdhip.GDError):
+        batch.run(1)
+    assert s.step_count == 0
     batch.close()
 
 
