@@ -94,13 +94,14 @@ def kernel_costs(prec_bytes, tangential=True):
 # kernel of each class on the headline (tangential) path, as rocprofv3 names it
 # (synthesis: u and grad T as gradient fields, or, with GD_SYNTH_GRAD=0, as
 # full fields; CNAB2: one lane per column on the FP64 path, the twisted form
-# on the FP32-synthesis path)
+# on the FP32-synthesis path, re and im columns paired in a lane by default)
 TANG_KERNELS = {"spectral_pre": "spectral_pre_kernel<{T}>",
                 "synth_vector": ("synth_kernel<{T}, true, 1, 3, 16, true, 1, 0>",
                                  "synth_kernel<{T}, true, 2, 4, 16, true, 0, 0>"),
                 "ring_fft_products": "ring_kernel<{T}, 3, 384>", "analysis_vector": "ana_res_kernel<{T}, true, 1, 96, 0>",
                 "analysis_scalar": "ana_res_kernel<{T}, false, 1, 96, 0>",
-                "cnab2_solve": ("cnab2_chain_kernel<3, 4, {T}, true,", "cnab2_twist_kernel<3, 4, {T}>")}
+                "cnab2_solve": ("cnab2_chain_kernel<3, 4, {T}, true,", "cnab2_twist_kernel<3, 2, {T}, 2>",
+                                "cnab2_twist_kernel<3, 4, {T}, 1>")}
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "pmc_step_lmax127_nr65.json")
 COPY_PROBE_BYTES = 268435456
 
