@@ -71,6 +71,31 @@ function shtnskit_spectral_to_physical!(p::HIPSHTPlan, re::Array{Float64,3}, im:
     return nothing
 end
 
+"""
+    batch_shtnskit_transforms!(p, specs, physs)
+
+batch_shtnskit_transforms! (shtnskit_field_functions.jl:651-663): the
+syntheses of many fields of one plan.  specs[k] = (data_real, data_imag),
+each (nlm, 1, nr_k); physs[k] :: (nlat, nlon, nr_k).  The fields are stacked
+along r and synthesised in one batched pass (one Legendre launch and one
+ring launch), then split back; the results equal the one-by-one calls.
+"""
+function batch_shtnskit_transforms!(p::HIPSHTPlan, specs::AbstractVector, physs::AbstractVector)
+    length(specs) == length(physs) || throw(ArgumentError("batch_shtnskit_transforms!: $(length(specs)) spectral vs $(length(physs)) physical fields"))
+    isempty(specs) && return nothing
+    re = cat([s[1] for s in specs]...; dims=3)
+    im = cat([s[2] for s in specs]...; dims=3)
+    out = Array{Float64,3}(undef, size(physs[1], 1), size(physs[1], 2), size(re, 3))
+    shtnskit_spectral_to_physical!(p, re, im, out)
+    k = 0
+    for ph in physs
+        n = size(ph, 3)
+        ph .= view(out, :, :, k+1:k+n)
+        k += n
+    end
+    return nothing
+end
+
 "shtnskit_physical_to_spectral! (shtnskit_field_functions.jl:131)"
 function shtnskit_physical_to_spectral!(p::HIPSHTPlan, phys::Array{Float64,3}, re::Array{Float64,3},
                                         im::Array{Float64,3})
