@@ -112,6 +112,31 @@ def test_fused_batch_equals_separate_runs(prec):
     batch.close()
 
 
+def test_fused_batch_mixed_precisions_and_band_widths():
+    """One fused batch of FP64, FP32 and BF16x3 shells with band half-widths
+    1, 2 and 3: every kernel form gets its own launch group (precision,
+    split-precision flag, band width, solve form) and each shell its own bits."""
+    import gdhip
+
+    print("case: fused batch of mixed precisions and band widths", flush=True)
+    kinds = [("fp64", 3), ("fp32", 2), ("bf16x3", 3), ("fp32", 1), ("fp64", 2)]
+    shells = SHELLS + [(15, 17, 24, 48, 0.3)]
+    alone, batched = [], []
+    for k, ((L, nr, nlat, nlon, rr), (prec, bw)) in enumerate(zip(shells, kinds)):
+        st = gdhip.seeded_state(200 + k, L, nr, amp=1e-3)
+        for group in (alone, batched):
+            s = gdhip.CNAB2Step(L, nlat, nlon, nr, rratio=rr, Ra=100.0, precision=prec, bw=bw)
+            s.set_state_complex(st)
+            group.append(s)
+    for s in alone:
+        s.run(5, graph=True)
+    batch = gdhip.StepBatch(batched, nstreams=2, mode="fused", lanes=2)
+    batch.run(5)
+    batch.synchronize()
+    _assert_same(alone, batched, 5)
+    batch.close()
+
+
 @pytest.mark.parametrize("prec,lanes", [("fp64", 2), ("fp64", 3), ("fp32", 2)])
 def test_fused_lanes_equal_separate_runs(prec, lanes):
     """The fused stages split over several streams (lanes) give the same bits."""
