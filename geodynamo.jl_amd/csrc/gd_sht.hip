@@ -1258,6 +1258,29 @@ constexpr int ct_radix(int n, int stage)
 
 constexpr int RING_THREADS = 256; // ring_kernel block size
 
+// The Stockham write of a butterfly goes to idxD + r NS with idxD = q NS R + k,
+// k < NS.  sk(idxD + r NS) - sk(idxD) = r NS + [(idxD + r NS) / 16 - idxD / 16]
+// is a constant of r when the bracket is the same for every residue idxD mod
+// 16 the butterflies can have: then the R addresses are one base and
+// constant offsets.
+constexpr bool sk_write_const(int NS, int R)
+{
+    for (int r = 0; r < R; ++r) {
+        int val = -1;
+        for (int q = 0; q < 16; ++q)
+            for (int k = 0; k < NS && k < 16; ++k) {
+                const int res = (q * NS * R + k) % 16;
+                const int d = (res + r * NS) / 16;
+                if (val < 0) val = d;
+                else if (d != val) return false;
+            }
+    }
+    return true;
+}
+// the offset of write r: the bracket at residue 0 (q = k = 0), which every
+// residue shares when sk_write_const holds
+constexpr int sk_write_off(int NS, int r) { return r * NS + (r * NS) / 16; }
+
 // NFFT > 0: the transform count is a compile-time constant (nfft is ignored)
 template <int R, int NS, int N, int DIR, typename T, int NFFT>
 __device__ __forceinline__ void fft_stage_ct(C2<T> *buf, const C2<T> *tw, int nfft)
@@ -1267,6 +1290,9 @@ __device__ __forceinline__ void fft_stage_ct(C2<T> *buf, const C2<T> *tw, int nf
     constexpr int IT = NFFT > 0 ? (NFFT * nb + nthr - 1) / nthr : MAXC / R;
     const int total = (NFFT > 0 ? NFFT : nfft) * nb;
     const int tid = threadIdx.x;
+    // sk(e + r * s) = sk(e) + r * (s + s / 16) when s is a multiple of 16: the
+    // R addresses of a butterfly are then one base and constant offsets
+    constexpr bool RD16 = nb % 16 == 0, WR16 = sk_write_const(NS, R);
     C2<T> v[IT][R];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
@@ -1274,8 +1300,9 @@ __device__ __forceinline__ void fft_stage_ct(C2<T> *buf, const C2<T> *tw, int nf
         if (g < total) {
             const int f = g / nb, j = g - f * nb;
             const int k = j % NS;
+            const int rb = f * LD + sk(j);
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[it][r] = buf[f * LD + sk(j + r * nb)];
+            for (int r = 0; r < R; ++r) v[it][r] = buf[RD16 ? rb + r * (nb + nb / 16) : f * LD + sk(j + r * nb)];
             if (NS > 1) {
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
@@ -1295,8 +1322,9 @@ __device__ __forceinline__ void fft_stage_ct(C2<T> *buf, const C2<T> *tw, int nf
             const int f = g / nb, j = g - f * nb;
             const int k = j % NS;
             const int idxD = (j / NS) * NS * R + k;
+            const int wb = f * LD + sk(idxD);
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[f * LD + sk(idxD + r * NS)] = v[it][r];
+            for (int r = 0; r < R; ++r) buf[WR16 ? wb + sk_write_off(NS, r) : f * LD + sk(idxD + r * NS)] = v[it][r];
         }
     }
     __syncthreads();
