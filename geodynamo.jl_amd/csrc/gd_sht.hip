@@ -623,7 +623,39 @@ __device__ __forceinline__ void ana_res_body(const AnaArgs &a, int blk, int nblk
         sinv[tid] = l > 0 ? 1.0 / ((double)l * (double)(l + 1)) : 0.0;
     }
     // table tile: all loads in flight together, then into LDS
-    {
+    // 16-byte loads of EV consecutive nodes where every table row is 16-byte
+    // aligned (nlat/2 a multiple of EV, the chunk start a multiple of 16)
+    constexpr int EV = 16 / (int)sizeof(T);
+    constexpr int NAV = (NT * TM * KN / EV + 64 * ANA_WAVES - 1) / (64 * ANA_WAVES);
+    using VT = typename Vec16<T>::type;
+    const bool vl = a.nlat2 % EV == 0 && nk % EV == 0 && ((uintptr_t)tab0 % 16) == 0 &&
+                    (NT == 1 || ((uintptr_t)tab1 % 16) == 0);
+    if (vl) {
+        VT ra[NAV];
+#pragma unroll
+        for (int q = 0; q < NAV; ++q) {
+            const int e = tid + q * 64 * ANA_WAVES;
+            const int t = e / (TM * KN / EV), rem = e - t * (TM * KN / EV);
+            const int i = rem / (KN / EV), k = (rem - i * (KN / EV)) * EV;
+            VT v;
+            T *pv = reinterpret_cast<T *>(&v);
+#pragma unroll
+            for (int u = 0; u < EV; ++u) pv[u] = T(0);
+            if (e < NT * TM * KN / EV && row0 + i < Kp && k < nk)
+                v = *reinterpret_cast<const VT *>((t == 0 ? tab0 : tab1) + (size_t)(kbase + row0 + i) * a.nlat2 + k0 + k);
+            ra[q] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < NAV; ++q) {
+            const int e = tid + q * 64 * ANA_WAVES;
+            const int t = e / (TM * KN / EV), rem = e - t * (TM * KN / EV);
+            const int i = rem / (KN / EV), k = (rem - i * (KN / EV)) * EV;
+            const T *pv = reinterpret_cast<const T *>(&ra[q]);
+            if (e < NT * TM * KN / EV)
+#pragma unroll
+                for (int u = 0; u < EV; ++u) sA[(t * KN + k + u) * TMA + i] = pv[u];
+        }
+    } else {
         T ra[NA];
 #pragma unroll
         for (int q = 0; q < NA; ++q) {
