@@ -259,21 +259,22 @@ def c3_shells(nshell=64, seed=3):
     return out
 
 
-def test_fused_batch_c3_64_shells(oracle):
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 1e-6)])
+def test_fused_batch_c3_64_shells(oracle, prec, tol):
     """The full 64-shell workload in the fused form (tiles from the tuning
-    table): every shell equal to its own step graph after 4 steps, and the
-    first two steps (the eager AB1 bootstrap, then the first fused graph) of
-    three sampled shells (the smallest, a middle and the largest) within
-    1e-10 of the FP64 oracle per field."""
+    table), FP64 and mixed: every shell equal to its own step graph after 4
+    steps, and the first two steps (the eager AB1 bootstrap, then the first
+    fused graph) of three sampled shells (the smallest, a middle and the
+    largest) within tol of the FP64 oracle per field."""
     import gdhip
 
     shells = c3_shells()
-    print("case: fused batch C3 64 shells fp64", flush=True)
+    print("case: fused batch C3 64 shells %s" % prec, flush=True)
     alone, batched = [], []
     for k, (L, nr, nlat, nlon, rr) in enumerate(shells):
         st = oracle.seeded_state(500 + k, L, L, nr, amp=1e-3)
         for group in (alone, batched):
-            s = gdhip.CNAB2Step(L, nlat, nlon, nr, rratio=rr, Ra=500.0, precision="fp64")
+            s = gdhip.CNAB2Step(L, nlat, nlon, nr, rratio=rr, Ra=500.0, precision=prec)
             s.set_state_complex(st)
             group.append(s)
     order = sorted(range(len(shells)), key=lambda k: shells[k][0] * shells[k][0] * shells[k][1])
@@ -291,7 +292,7 @@ def test_fused_batch_c3_64_shells(oracle):
         got = batched[k].get_state_complex()
         err = [np.abs(got[f] - ref[f]).max() / max(np.abs(ref[f]).max(), 1e-300) for f in range(5)]
         print("shell", k, shells[k][:2], "rel err vs oracle", err, flush=True)
-        assert max(err) < 1e-10
+        assert max(err) < tol
     batch.run(2)
     for s in alone:
         s.run(4, graph=True)
