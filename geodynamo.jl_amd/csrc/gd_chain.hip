@@ -28,8 +28,8 @@ namespace {
 // then one scalar multiply instead of a 64-bit product), size_t otherwise.
 // (blk_x, blk_y, blk_z): the block's column block, field offset and re/im (the
 // launch's own grid, or its share of a batched launch)
-template <int BW, int D, typename TS, bool DIRECT, typename IX>
-__device__ __forceinline__ void cnab2_chain_body(const ChainArgs &c, int blk_x, int blk_y, int blk_z)
+template <int BW, int D, typename TS, bool DIRECT, typename IX, class AT>
+__device__ __forceinline__ void cnab2_chain_body(const AT &c, int blk_x, int blk_y, int blk_z)
 {
     constexpr int W = 2 * BW + 1;
     const int n = c.n, nlm = c.nlm;
@@ -251,7 +251,7 @@ __global__ void __launch_bounds__(64) cnab2_chain_kernel(ChainArgs c)
 // share of a batched launch (column block fastest, as the shell's own grid)
 // a batched launch's block loc of one shell -> (column block, field, re/im)
 // of that shell's own grid (gx, nfield, 2 / NC)
-template <int NC> __device__ __forceinline__ void chain_decode(const ChainArgs &c, int loc, int &bx, int &by, int &bz)
+template <int NC, class AT> __device__ __forceinline__ void chain_decode(const AT &c, int loc, int &bx, int &by, int &bz)
 {
     const int gx = (c.nlm + 63) / 64, nfield = c.nf ? c.nf : (c.c_in ? 6 : 5);
     bx = loc % gx;
@@ -265,7 +265,7 @@ __global__ void __launch_bounds__(64) cnab2_chain_batch_kernel(BatchRef<ChainArg
 {
     const int sh = b.shell_of_block[blockIdx.x];
     if (sh == BATCH_PAD) return;
-    const ChainArgs &c = b.args[sh];
+    const auto &c = batch_args(b, sh);
     int bx, by, bz;
     chain_decode<1>(c, blockIdx.x - b.block_off[sh], bx, by, bz);
     cnab2_chain_body<BW, D, TS, DIRECT, IX>(c, bx, by, bz);
@@ -290,8 +290,8 @@ __global__ void __launch_bounds__(64) cnab2_chain_batch_kernel(BatchRef<ChainArg
 // halve and the two independent recurrences interleave.  Each column gets the
 // operations of NC = 1; the compiler may fuse them into FMAs differently, so
 // the two forms agree to rounding, not always bit for bit.
-template <int BW, int D, typename TS, int NC>
-__device__ __forceinline__ void cnab2_twist_body(const ChainArgs &c, int blk_x, int blk_y, int blk_z)
+template <int BW, int D, typename TS, int NC, class AT>
+__device__ __forceinline__ void cnab2_twist_body(const AT &c, int blk_x, int blk_y, int blk_z)
 {
     constexpr int W = 2 * BW + 1, M2 = 2 * BW;
     const int n = c.n, nlm = c.nlm, n1 = (n + 1) / 2, nb = n - n1;
@@ -558,7 +558,7 @@ __global__ void __launch_bounds__(128, NC == 1 ? 3 : 2) cnab2_twist_batch_kernel
 {
     const int sh = b.shell_of_block[blockIdx.x];
     if (sh == BATCH_PAD) return;
-    const ChainArgs &c = b.args[sh];
+    const auto &c = batch_args(b, sh);
     int bx, by, bz;
     chain_decode<NC>(c, blockIdx.x - b.block_off[sh], bx, by, bz);
     cnab2_twist_body<BW, D, TS, NC>(c, bx, by, bz);

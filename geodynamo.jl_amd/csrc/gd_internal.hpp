@@ -167,6 +167,19 @@ template <class A> struct BatchRef {
     const int *block_off, *block_cnt;
 };
 
+// A batched kernel reads its shell's arguments through the constant address
+// space: the table is written before the launch and never during it, so, as
+// with a kernel's own arguments, the compiler may load each field once into
+// scalar registers.  Through a generic pointer every store of the body may
+// alias the table, and the fields are loaded again inside the loops (each
+// reload waits on the scalar and LDS counter together).  The kernel bodies
+// take the arguments as a deduced type for this.
+template <class A> using ConstArgs = const __attribute__((address_space(4))) A;
+template <class A> __device__ __forceinline__ ConstArgs<A> &batch_args(const BatchRef<A> &b, int sh)
+{
+    return *(ConstArgs<A> *)(b.args + sh);
+}
+
 // launch helper: returns GD_ERR_HIP on launch failure
 int gd_check_launch(const char *what);
 

@@ -154,8 +154,8 @@ template <bool VEC, int NF, int NW, int NG = 0> struct SynthTile {
 //     (lane / 16) takes degrees 4g .. 4g+3 over the chunk's 4 MFMA steps (the
 //     same permutation indexes A), so each lane reads 4 consecutive degrees.
 constexpr int TMA = TM + 4;
-template <typename T, bool VEC, int NF, int NW, int KC, bool VL, int NG, int MM>
-__device__ __forceinline__ void synth_pipeline(const SynthArgs &a, const T *__restrict__ in, const T *__restrict__ tab0,
+template <typename T, bool VEC, int NF, int NW, int KC, bool VL, int NG, int MM, class AT>
+__device__ __forceinline__ void synth_pipeline(const AT &a, const T *__restrict__ in, const T *__restrict__ tab0,
                                                const T *__restrict__ tab1, T *sA, const int *scol, int m,
                                                int i0, int r0, acc_t<T> (&accE)[SynthTile<VEC, NF, NW, NG>::NT][2],
                                                acc_t<T> (&accO)[SynthTile<VEC, NF, NW, NG>::NT][2])
@@ -295,8 +295,8 @@ template <typename T, bool VEC, int NF, int NW, int KC, int NG = 0> constexpr si
 // one record run after another.
 // blk, nblk: the block's index and the block count of this synthesis (the
 // launch's own, or its share of a batched launch over several shells)
-template <typename T, bool VEC, int NF, int NW, int KC, bool VL, int NG, int MM>
-__device__ __forceinline__ void synth_body(const SynthArgs &a, int blk, int nblk)
+template <typename T, bool VEC, int NF, int NW, int KC, bool VL, int NG, int MM, class AT>
+__device__ __forceinline__ void synth_body(const AT &a, int blk, int nblk)
 {
     static_assert(!MM || sizeof(T) == 4, "the BF16x3 contraction takes FP32 operands");
     using TL = SynthTile<VEC, NF, NW, NG>;
@@ -406,7 +406,7 @@ __global__ void __launch_bounds__(64 * NW) synth_batch_kernel(BatchRef<SynthArgs
 {
     const int sh = b.shell_of_block[blockIdx.x];
     if (sh == BATCH_PAD) return;
-    synth_body<T, VEC, NF, NW, KC, VL, NG, MM>(b.args[sh], blockIdx.x - b.block_off[sh], b.block_cnt[sh]);
+    synth_body<T, VEC, NF, NW, KC, VL, NG, MM>(batch_args(b, sh), blockIdx.x - b.block_off[sh], b.block_cnt[sh]);
 }
 
 // ---------------------------------------------------------------- analysis
@@ -585,8 +585,8 @@ template <typename T, bool VEC, int NF> __global__ void __launch_bounds__(64 * N
 // 48 us: a wave's next-tile operand prefetch then spans fewer tiles; for FP64
 // it would exceed the register budget of a 512-thread block)
 template <typename T> constexpr int ana_waves() { return 4; }
-template <typename T, bool VEC, int NF, int KN, int MM>
-__device__ __forceinline__ void ana_res_body(const AnaArgs &a, int blk, int nblk)
+template <typename T, bool VEC, int NF, int KN, int MM, class AT>
+__device__ __forceinline__ void ana_res_body(const AT &a, int blk, int nblk)
 {
     static_assert(!MM || sizeof(T) == 4, "the BF16x3 contraction takes FP32 operands");
     constexpr int ANA_WAVES = ana_waves<T>();
@@ -845,7 +845,7 @@ __global__ void __launch_bounds__(64 * ana_waves<T>()) ana_res_batch_kernel(Batc
 {
     const int sh = b.shell_of_block[blockIdx.x];
     if (sh == BATCH_PAD) return;
-    ana_res_body<T, VEC, NF, KN, MM>(b.args[sh], blockIdx.x - b.block_off[sh], b.block_cnt[sh]);
+    ana_res_body<T, VEC, NF, KN, MM>(batch_args(b, sh), blockIdx.x - b.block_off[sh], b.block_cnt[sh]);
 }
 
 template <typename T, bool VEC, int NF, int KN> constexpr size_t ana_res_lds()
@@ -1377,7 +1377,7 @@ __device__ __forceinline__ void fft_ct(C2<T> *buf, const C2<T> *tw, int nfft)
 // the runtime plan.
 // (i, r): the block's northern node and level (the launch's own grid, or its
 // share of a batched launch)
-template <typename T, int MODE, int NC> __device__ __forceinline__ void ring_body(const RingArgs &a, int i, int r)
+template <typename T, int MODE, int NC, class AT> __device__ __forceinline__ void ring_body(const AT &a, int i, int r)
 {
     extern __shared__ char smem_raw[];
     const int n = NC > 0 ? NC : a.nlon, nm = a.mmax + 1;
@@ -1768,7 +1768,7 @@ template <typename T, int MODE, int NC> __global__ void __launch_bounds__(256) r
     const int sh = b.shell_of_block[blockIdx.x];
     if (sh == BATCH_PAD) return;
     const int loc = blockIdx.x - b.block_off[sh];
-    const RingArgs &a = b.args[sh];
+    const auto &a = batch_args(b, sh);
     ring_body<T, MODE, NC>(a, loc % a.nlat2, loc / a.nlat2);
 }
 } // namespace
