@@ -692,11 +692,11 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
     GD_CHECK(d->bc_rows >= 0 && d->bc_rows <= 2, GD_ERR_ARG, "gd_step_create: bc_rows=%d", d->bc_rows);
     GD_CHECK(d->radial_force == 0 || d->radial_force == 1, GD_ERR_ARG, "gd_step_create: radial_force=%d",
              d->radial_force);
-    // the twisted column solve on the FP32-synthesis path with bw <= 3 (at
-    // lmax=127, nr=65, bw=3: 68 vs 73 us; on the FP64 path 87 vs 83 us, so
-    // FP64 keeps one lane per column); GD_CHAIN_TWIST = 0 keeps one lane here too
+    // the twisted column solve with bw <= 3, every precision (at lmax=127,
+    // nr=65, bw=3, paired columns: FP64-synthesis path 81.3 -> 68.8 us against
+    // one lane per column); GD_CHAIN_TWIST = 0 keeps one lane per column
     const char *tw = getenv("GD_CHAIN_TWIST");
-    s->twist = (!(tw && tw[0] == '0') && d->prec != GD_PREC_FP64 && bw <= 3 && nr >= 2 * bw + 2 &&
+    s->twist = (!(tw && tw[0] == '0') && bw <= 3 && nr >= 2 * bw + 2 &&
                 gd_cnab2_chain_supported(nr, bw)) ? 1 : 0;
     if (s->twist && rc == GD_OK) {
         rc = s->ULt.ensure(sizeof(double) * nb * 4);

@@ -6,23 +6,21 @@ composition, a small shell and lmax=127, nr=65):
     (GD_SYNTH_GRAD=0): each MFMA output column depends only on its own input
     column, and the dropped terms are exact zeros;
   * equal to rounding: the twisted CNAB2 column solve (a top and a bottom
-    wave per 64 columns meeting at an interface system; the FP32-synthesis
-    path uses it, the FP64 path keeps one lane per column) against one lane
-    per column (GD_CHAIN_TWIST=0): the same linear system solved in another
-    order, so equal to rounding (max relative difference <= 1e-6 after 3
-    steps of the FP32 path, whose nonlinear terms are FP32);
+    wave per 64 columns meeting at an interface system; both paths use it)
+    against one lane per column (GD_CHAIN_TWIST=0): the same linear system
+    solved in another order, so equal to rounding (max relative difference
+    <= 1e-6 after 3 steps of the FP32 path, whose nonlinear terms are FP32,
+    and <= 1e-13 on the FP64 path);
   * equal to rounding: the twisted solve with the re and im columns of an
     (idx, field) in one lane (the default) against one column per lane
-    (GD_CHAIN_PAIR=0); the FP64 path has no twisted solve, so there the
-    two are the same kernel (same bits);
+    (GD_CHAIN_PAIR=0);
   * same bits: the paired twisted solve with the first rows of each half's
     forward results in LDS (GD_CHAIN_YLDS=-1) against all of them through
     the state buffer (the default): the same operations, another store;
-  * same bits on the FP64 path: the CNAB2 column kernel with 64-bit offsets
+  * same bits: the one-lane CNAB2 column kernel with 64-bit offsets
     (GD_CHAIN_OFF64=1, the form shells with SYNV_COLS*nlm*nr >= 2^32 run)
-    against the 32-bit offsets used where they fit; on the FP32 path the
-    64-bit form is one lane per column, so there it is the twisted solve's
-    rounding bound."""
+    against the 32-bit offsets used where they fit, both with
+    GD_CHAIN_TWIST=0 (the twisted solve takes 32-bit offsets only)."""
 import os
 
 import numpy as np
@@ -33,13 +31,17 @@ pytestmark = pytest.mark.gpu
 
 ALT = {"GD_SYNTH_GRAD": "0", "GD_CHAIN_TWIST": "0", "GD_CHAIN_OFF64": "1", "GD_CHAIN_PAIR": "0",
        "GD_CHAIN_YLDS": "-1"}  # non-default forms
+BASE = {"GD_CHAIN_OFF64": {"GD_CHAIN_TWIST": "0"}}  # set for both sides of a pair
+BITWISE = ("GD_SYNTH_GRAD", "GD_CHAIN_YLDS", "GD_CHAIN_OFF64")
 
 
 def _pair(var, prec, L, nlat, nlon, nr, comp):
     import gdhip
 
+    base = BASE.get(var, {})
     out = []
     for flag in (ALT[var], None):
+        os.environ.update(base)
         if flag is None:
             os.environ.pop(var, None)
         else:
@@ -48,7 +50,8 @@ def _pair(var, prec, L, nlat, nlon, nr, comp):
         if comp:
             s.enable_composition(2.0)
         out.append(s)
-    os.environ.pop(var, None)
+    for k in [var] + list(base):
+        os.environ.pop(k, None)
     return out
 
 
@@ -68,7 +71,7 @@ def test_kernel_forms_agree(var, prec, comp, L, nlat, nlon, nr):
             s.set_composition(gdhip.seeded_state(52, L, nr, nfields=1, amp=1e-3)[0])
         s.run(3, graph=True)
     a, b = old.get_state_complex(), new.get_state_complex()
-    bitwise = var in ("GD_SYNTH_GRAD", "GD_CHAIN_YLDS") or (var in ("GD_CHAIN_OFF64", "GD_CHAIN_PAIR") and prec == "fp64")
+    bitwise = var in BITWISE
     if bitwise:
         assert np.array_equal(a, b)
     else:
@@ -84,16 +87,19 @@ def test_kernel_forms_agree(var, prec, comp, L, nlat, nlon, nr):
             assert np.abs(ca - cb).max() <= (1e-13 if prec == "fp64" else 1e-6) * np.abs(ca).max()
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
 @pytest.mark.parametrize("pair", ["2", "0"])
 @pytest.mark.parametrize("nr,bw", [(8, 3), (9, 3), (12, 3), (13, 3), (6, 2), (11, 2), (4, 1), (17, 1), (33, 2)])
-def test_twisted_solve_edge_shapes(nr, bw, pair):
+def test_twisted_solve_edge_shapes(nr, bw, pair, prec):
     """The twisted solve (both column layouts) at the smallest shell it takes
     (nr = 2 bw + 2), odd and even nr where only the guarded tail blocks run,
     and every band width it is built for (bw 1..3), with composition: equal
-    to the one-lane solve to the FP32 path's rounding bound after 3 steps."""
+    to the one-lane solve after 3 steps, to the FP32 path's rounding bound
+    (1e-6) or the FP64 path's (1e-13)."""
     import gdhip
 
-    print("case: twisted solve edge shape nr=%d bw=%d pair=%s" % (nr, bw, pair), flush=True)
+    print("case: twisted solve edge shape nr=%d bw=%d pair=%s %s" % (nr, bw, pair, prec), flush=True)
+    tol = 1e-6 if prec == "fp32" else 1e-13
     os.environ["GD_CHAIN_PAIR"] = pair
     L, nlat, nlon = 15, 24, 48
     st = gdhip.seeded_state(53, L, nr, amp=1e-3)
@@ -104,7 +110,7 @@ def test_twisted_solve_edge_shapes(nr, bw, pair):
             os.environ.pop("GD_CHAIN_TWIST", None)
         else:
             os.environ["GD_CHAIN_TWIST"] = flag
-        s = gdhip.CNAB2Step(L, nlat, nlon, nr, Ra=500.0, dt=1e-5, precision="fp32", bw=bw)
+        s = gdhip.CNAB2Step(L, nlat, nlon, nr, Ra=500.0, dt=1e-5, precision=prec, bw=bw)
         s.enable_composition(2.0)
         out.append(s)
     os.environ.pop("GD_CHAIN_TWIST", None)
@@ -116,9 +122,9 @@ def test_twisted_solve_edge_shapes(nr, bw, pair):
     a, b = out[0].get_state_complex(), out[1].get_state_complex()
     for f in range(5):
         scale = max(np.abs(a[f]).max(), 1e-300)
-        assert np.abs(a[f] - b[f]).max() <= 1e-6 * scale, (f, np.abs(a[f] - b[f]).max() / scale)
+        assert np.abs(a[f] - b[f]).max() <= tol * scale, (f, np.abs(a[f] - b[f]).max() / scale)
     ca, cb = out[0].get_composition(), out[1].get_composition()
-    assert np.abs(ca - cb).max() <= 1e-6 * np.abs(ca).max()
+    assert np.abs(ca - cb).max() <= tol * np.abs(ca).max()
     assert np.all(np.isfinite(b))
     # the two solves round differently somewhere: the twisted form did run
     assert not np.array_equal(a, b)
