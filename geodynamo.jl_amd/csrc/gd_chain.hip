@@ -310,24 +310,12 @@ __device__ __forceinline__ void cnab2_twist_body(const AT &c, int blk_x, int blk
     // ys: the first c.ylds forward results y of each half, [half][s][col][64]
     // (the rest go through st_out as in the one-column form)
     double *ys = xch + NC * 2 * BW * 64;
-    const double *Lapk = c.nuLap + (size_t)k * W * n;
-    for (int t = threadIdx.x; t < ER * n; t += 128) {
-        const int i = t / ER, q = t - i * ER, j = i - BW + q;
-        double v = 0.0;
-        if (q == BW) v = Lapk[BW + W * i];
-        else if (q < W && j >= 0 && j < n) v = Lapk[(2 * BW - q) + W * j];
-        else if (q == W) v = c.rtab[i + n];
-        Ex[t] = v;
-        ExB[ER * i + (q < W ? 2 * BW - q : q)] = v;
-    }
-    for (int t = threadIdx.x; t < n; t += 128) rinv_[t] = c.rtab[t + 2 * n];
-    __syncthreads();
+    // The prologue issues its loads together: every value that a condition
+    // would zero is read from the zero row instead (the load address is
+    // selected, never the loaded value), because a select of a loaded value
+    // on a wave-uniform condition becomes a branch around the load and a wait
+    // for it, one memory latency per load.
     const int l = c.l_of_idx[idx];
-    const double lf = (double)l * (double)(l + 1);
-    const double nulf = c.nus[k] * lf;
-    const double *T = (bot ? c.ULt : c.LUt) + (size_t)k * W * n * c.nl + l;
-    const double *rdg = (bot ? c.rdiag2 : c.rdiag) + (size_t)k * n * c.nl + l;
-    const double *Rl = c.Rint + ((size_t)k * c.nl + l) * (M2 * M2);
     const size_t fs = (size_t)nlm * n;
     const bool nlz = !isC && ((c.nl_zero_mask >> f) & 1);
     const unsigned nstride = nlz ? 0u : (unsigned)nlm;
@@ -354,6 +342,48 @@ __device__ __forceinline__ void cnab2_twist_body(const AT &c, int blk_x, int blk
     const int dir = bot ? -1 : 1;
     auto cl = [&](int j) { return j < 0 ? 0 : (j > n - 1 ? n - 1 : j); };
     auto urow = [&](int j) { return j >= 0 && j < n && !(tbc && (j == 0 || j == n - 1)); };
+
+    // ---- outward-in: rhs rows and the unit-triangular substitution
+    const int nfw = bot ? nb : n1; // valid steps of this lane
+    double w[NC][W];               // u rows (top: i-BW..i+BW; bottom: i+BW..i-BW)
+    {
+        const int i0 = bot ? n - 1 : 0;
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
+            const int j = i0 + dir * (q - BW);
+            const bool keep = urow(j);
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) w[cc][q] = *(keep ? u[cc] + (unsigned)nlm * j : c.zero_row + idx);
+        }
+    }
+    // the operator rows (UF entries per thread and round, loads before stores)
+    const double *Lapk = c.nuLap + (size_t)k * W * n;
+    constexpr int UF = 4;
+    for (int t0 = threadIdx.x; t0 < ER * n; t0 += UF * 128) {
+        double v[UF];
+#pragma unroll
+        for (int e = 0; e < UF; ++e) {
+            const int t = min(t0 + e * 128, ER * n - 1), i = t / ER, q = t - i * ER, j = i - BW + q;
+            const bool bnd = q < W && j >= 0 && j < n; // q = BW: the diagonal, j = i
+            v[e] = *(bnd ? Lapk + (2 * BW - q) + W * j : (q == W ? c.rtab + i + n : c.zero_row));
+        }
+#pragma unroll
+        for (int e = 0; e < UF; ++e) {
+            const int t = t0 + e * 128;
+            if (t < ER * n) {
+                const int i = t / ER, q = t - i * ER;
+                Ex[t] = v[e];
+                ExB[ER * i + (q < W ? 2 * BW - q : q)] = v[e];
+            }
+        }
+    }
+    for (int t = threadIdx.x; t < n; t += 128) rinv_[t] = c.rtab[t + 2 * n];
+    __syncthreads();
+    const double lf = (double)l * (double)(l + 1);
+    const double nulf = c.nus[k] * lf;
+    const double *T = (bot ? c.ULt : c.LUt) + (size_t)k * W * n * c.nl + l;
+    const double *rdg = (bot ? c.rdiag2 : c.rdiag) + (size_t)k * n * c.nl + l;
+    const double *Rl = c.Rint + ((size_t)k * c.nl + l) * (M2 * M2);
     // band entry (i, j) of this half's table; j outside 0..n-1 reads the
     // (finite) diagonal instead.  No mask is needed: the forward rows outside
     // 0..n-1 meet yw entries that start at 0, and the back substitution's rows
@@ -367,22 +397,6 @@ __device__ __forceinline__ void cnab2_twist_body(const AT &c, int blk_x, int blk
         o[cc][(unsigned)nlm * i] = v;
         if (wA) pA[cc][(unsigned)(i * SYNV_COLS) * (unsigned)nlm] = (TS)(f == 4 ? v * rinv_[i] : v);
     };
-
-    // ---- outward-in: rhs rows and the unit-triangular substitution
-    const int nfw = bot ? nb : n1; // valid steps of this lane
-    double w[NC][W];               // u rows (top: i-BW..i+BW; bottom: i+BW..i-BW)
-    {
-        const int i0 = bot ? n - 1 : 0;
-#pragma unroll
-        for (int q = 0; q < W; ++q) {
-            const int j = i0 + dir * (q - BW);
-#pragma unroll
-            for (int cc = 0; cc < NC; ++cc) {
-                const double v = u[cc][(unsigned)nlm * cl(j)];
-                w[cc][q] = urow(j) ? v : 0.0;
-            }
-        }
-    }
     double su[NC][D], sn[NC][D], sp[NC][D], sl[D][BW];
     auto fload = [&](int sl_, int s) {
         const int i = bot ? n - 1 - s : s, ic = cl(i), jn = i + dir * (BW + 1);
