@@ -1415,13 +1415,21 @@ template <typename T, int MODE, int NC, class AT> __device__ __forceinline__ voi
         // Hermitian spectra of the north (X) and south (Y) rings packed as Z = X + iY
         const T *src = static_cast<const T *>(a.fb) + ((size_t)i * a.nr + r) * nm * a.nslot_f;
         // fixed trip count, fully unrolled: every thread's global loads are in
-        // flight together before the LDS stores consume them
+        // flight together before the LDS stores consume them.  With a
+        // compile-time ring length the first MAXU iterations (enough for the
+        // nm <= n/3 orders of the dealiased grids) load unconditionally,
+        // clamped to the last element: a guarded load becomes a branch whose
+        // join waits for it.  Larger nm take the guarded iterations after.
         constexpr int MAXI = 8;
+        constexpr int MAXU0 = (NC > 0 && NFC > 0) ? (NFC * (NC / 3) + nthr - 1) / nthr : 0;
+        constexpr int MAXU = MAXU0 < MAXI ? MAXU0 : MAXI;
+        const int ne = NF * nm;
         T xr[MAXI], xi[MAXI], yr[MAXI], yi[MAXI];
 #pragma unroll
         for (int it = 0; it < MAXI; ++it) {
-            const int e = tid + it * nthr;
-            if (e < NF * nm) {
+            const int e0 = tid + it * nthr;
+            if (it < MAXU || e0 < ne) {
+                const int e = e0 < ne ? e0 : ne - 1;
                 const int F = e % NF, m = e / NF;
                 const T *s = src + (size_t)m * a.nslot_f;
                 xr[it] = s[sfn[F]];
