@@ -1683,17 +1683,18 @@ __device__ __forceinline__ void ring_io_body(const RingArgs &a, int g, int r)
         // Hermitian spectra of the north (X) and south (Y) rings packed as Z = X + iY
         for (int e0 = 0; e0 < NT * nm; e0 += 8 * nthr) {
             T xr[8], xi[8], yr[8], yi[8];
+            // loads of 8 elements in flight before their stores; unconditional
+            // (clamped to the last element): a guarded load becomes a branch
+            // whose join waits for it
 #pragma unroll
-            for (int it = 0; it < 8; ++it) { // loads of 8 elements in flight before their stores
-                const int e = e0 + tid + it * nthr;
-                if (e < NT * nm) {
-                    const int F = e % NF, m = (e / NF) % nm, q = e / (NF * nm);
-                    const T *s = static_cast<const T *>(a.fb) + (((size_t)(i0 + q) * a.nr + r) * nm + m) * a.nslot_f;
-                    xr[it] = s[sfn[F]];
-                    xi[it] = s[sfn[F] + 1];
-                    yr[it] = s[sfs[F]];
-                    yi[it] = s[sfs[F] + 1];
-                }
+            for (int it = 0; it < 8; ++it) {
+                const int e = min(e0 + tid + it * nthr, NT * nm - 1);
+                const int F = e % NF, m = (e / NF) % nm, q = e / (NF * nm);
+                const T *s = static_cast<const T *>(a.fb) + (((size_t)(i0 + q) * a.nr + r) * nm + m) * a.nslot_f;
+                xr[it] = s[sfn[F]];
+                xi[it] = s[sfn[F] + 1];
+                yr[it] = s[sfs[F]];
+                yi[it] = s[sfs[F] + 1];
             }
 #pragma unroll
             for (int it = 0; it < 8; ++it) {
