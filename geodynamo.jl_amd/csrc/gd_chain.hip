@@ -47,19 +47,57 @@ __device__ __forceinline__ void cnab2_chain_body(const AT &c, int blk_x, int blk
     constexpr int ER = (W + 2) & ~1;
     extern __shared__ double smem[];
     double *Ex = smem, *sD1 = Ex + ER * n, *sD2 = sD1 + W * n, *rinv_ = sD2 + W * n, *rinv2_ = rinv_ + n;
+    // The prologue issues its loads together (the lane's column clamped to a
+    // valid one until the range check below): values that a condition would
+    // zero are read from the zero row instead (the load address is selected,
+    // never the loaded value), because a select of a loaded value on a
+    // wave-uniform condition becomes a branch around the load and a wait for
+    // it, one memory latency per load.
+    const int idc = idx < nlm ? idx : nlm - 1;
+    const int l = c.l_of_idx[idc];
+    const size_t fs = (size_t)nlm * n, off = (size_t)(2 * f + ri) * fs + idc;
+    const double *u = (isC ? c.c_in : c.st_in) + off;
+    // bit 5 of bc_mask: a boundary-condition stage has already set the
+    // temperature rows in st_in; read them as stored, and Dirichlet rhs rows
+    // of the temperature take the stored boundary value
+    const bool tkeep = isC || ((f == 4) && ((c.bc_mask >> 5) & 1));
+    const bool tbc = (f == 4) && !tkeep;
+    auto cl = [&](int j) { return j < 0 ? 0 : (j > n - 1 ? n - 1 : j); }; // a valid row
+    auto urow = [&](int j) { return j >= 0 && j < n && !(tbc && (j == 0 || j == n - 1)); };
+    double w[W]; // u rows i-BW..i+BW of the first forward step (temperature boundary rows read as 0)
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+        const int j = q - BW;
+        w[q] = *(urow(j) ? u + IX(nlm) * IX(j) : c.zero_row + idc);
+    }
     const double *Lapk = c.nuLap + (size_t)k * W * n;
+    constexpr int UF = 4; // table entries per thread and round, loads before stores
     if constexpr (!DIRECT) // the radial derivatives serve only the general synthesis columns
-        for (int t = threadIdx.x; t < W * n; t += 64) {
-            sD1[t] = c.D1[t];
-            sD2[t] = c.D2[t];
+        for (int t0 = threadIdx.x; t0 < W * n; t0 += UF * 64) {
+            double v1[UF], v2[UF];
+#pragma unroll
+            for (int e = 0; e < UF; ++e) {
+                const int t = min(t0 + e * 64, W * n - 1);
+                v1[e] = c.D1[t];
+                v2[e] = c.D2[t];
+            }
+#pragma unroll
+            for (int e = 0; e < UF; ++e) {
+                const int t = min(t0 + e * 64, W * n - 1); // past the end: entry W n - 1 again
+                sD1[t] = v1[e];
+                sD2[t] = v2[e];
+            }
         }
-    for (int t = threadIdx.x; t < ER * n; t += 64) {
-        const int i = t / ER, q = t - i * ER, j = i - BW + q;
-        double v = 0.0;
-        if (q == BW) v = Lapk[BW + W * i];
-        else if (q < W && j >= 0 && j < n) v = Lapk[(2 * BW - q) + W * j];
-        else if (q == W) v = c.rtab[i + n];
-        Ex[t] = v;
+    for (int t0 = threadIdx.x; t0 < ER * n; t0 += UF * 64) {
+        double v[UF];
+#pragma unroll
+        for (int e = 0; e < UF; ++e) {
+            const int t = min(t0 + e * 64, ER * n - 1), i = t / ER, q = t - i * ER, j = i - BW + q;
+            const bool bnd = q < W && j >= 0 && j < n; // q = BW: the diagonal, j = i
+            v[e] = *(bnd ? Lapk + (2 * BW - q) + W * j : (q == W ? c.rtab + i + n : c.zero_row));
+        }
+#pragma unroll
+        for (int e = 0; e < UF; ++e) Ex[min(t0 + e * 64, ER * n - 1)] = v[e]; // past the end: the last entry again
     }
     for (int t = threadIdx.x; t < n; t += 64) {
         rinv_[t] = c.rtab[t + 2 * n];
@@ -67,39 +105,22 @@ __device__ __forceinline__ void cnab2_chain_body(const AT &c, int blk_x, int blk
     }
     __syncthreads();
     if (idx >= nlm) return;
-    const int l = c.l_of_idx[idx];
     const double lf = (double)l * (double)(l + 1);
     const double nulf = c.nus[k] * lf;
     const double *LU = c.LUt + (size_t)k * W * n * c.nl + l;
     const double *rdg = c.rdiag + (size_t)k * n * c.nl + l;
-    const size_t fs = (size_t)nlm * n, off = (size_t)(2 * f + ri) * fs + idx;
-    const double *u = (isC ? c.c_in : c.st_in) + off;
     // a field whose nonlinear terms are known to be zero reads one zero row (L2-resident)
     const bool nlz = !isC && ((c.nl_zero_mask >> f) & 1);
     const double *pn = nlz ? c.zero_row + idx : (isC ? c.c_nl : c.nl_arr) + off;
     const double *pp = nlz ? c.zero_row + idx : (isC ? c.c_np : c.np_arr) + off;
     const IX nstride = nlz ? IX(0) : IX(nlm);
     double *o = (isC ? c.c_out : c.st_out) + off;
-    // bit 5 of bc_mask: a boundary-condition stage has already set the
-    // temperature rows in st_in; read them as stored, and Dirichlet rhs rows
-    // of the temperature take the stored boundary value
-    const bool tkeep = isC || ((f == 4) && ((c.bc_mask >> 5) & 1));
-    const bool tbc = (f == 4) && !tkeep;
     const bool bcr = isC ? c.c_bcr != 0 : ((c.bc_mask >> f) & 1);
     const double inv_dt = 1.0 / c.dt, lw = 1.0 - c.theta;
-    auto cl = [&](int j) { return j < 0 ? 0 : (j > n - 1 ? n - 1 : j); }; // a valid row
-    auto urow = [&](int j) { return j >= 0 && j < n && !(tbc && (j == 0 || j == n - 1)); };
     // band element (row i, column j), |i - j| <= BW, of the transposed LU table at valid rows
     auto lu = [&](int i, int j) { return LU[IX(cl(j) * W + (BW + i - j)) * IX(c.nl)]; };
 
     // ---- forward: rhs row and L-substitution
-    double w[W]; // u rows i-BW..i+BW (temperature boundary rows read as 0)
-#pragma unroll
-    for (int q = 0; q < W; ++q) {
-        const int j = q - BW;
-        const double v = u[IX(nlm) * IX(cl(j))];
-        w[q] = urow(j) ? v : 0.0;
-    }
     double su[D], sn[D], sp[D], sl[D][BW];
     auto fload = [&](int s, int i) {
         const int jn = i + BW + 1, ic = cl(i);
