@@ -510,10 +510,13 @@ static void chain_args(gd_step *s, int parity, bool boot, ChainArgs &c)
         c.c_bcr = s->d.bc_rows == 1;
     }
     // the paired twisted solve keeps as many forward results in LDS as leave
-    // the launch in one round of resident blocks (at most 64 KB a block)
+    // the launch in one round of resident blocks (at most 64 KB a block).
+    // The budget is a whole number of 2 KB granules: at lmax=127, nr=65 (645
+    // blocks, three per CU) 18 rows (50.6 KB a block) ran in one round and 19
+    // rows (52.6 KB, below a third of 160 KB) in two, 49 against 67 us
     if (s->chain_ylds != 0) {
         const long blocks = (long)gd_cnab2_chain_blocks(c), per_cu = std::max(1L, (blocks + 255) / 256);
-        size_t budget = std::min<size_t>(64 * 1024, 160 * 1024 / per_cu);
+        size_t budget = std::min<size_t>(64 * 1024, (160 * 1024 / per_cu) & ~(size_t)2047);
         c.ylds = gd_chain_ylds(c, budget);
         if (s->chain_ylds > 0) c.ylds = std::min(c.ylds, s->chain_ylds);
     }
@@ -766,10 +769,10 @@ static int step_build(gd_ctx *ctx, const gd_step_desc *d, gd_step *s)
     const char *cp = getenv("GD_CHAIN_PAIR");
     s->chain_pair = (cp && (cp[0] == '0' || cp[0] == '3')) ? cp[0] - '0' : 2;
     // GD_CHAIN_YLDS: forward results y of the paired twisted solve in LDS --
-    // -1 as many rows as fit, k > 0 at most k rows per half; unset or 0: all
-    // through st_out (measured faster; a measurement aid)
+    // unset or -1 as many rows as keep the launch in one round of resident
+    // blocks, k > 0 at most k rows per half, 0 all through st_out
     const char *yl = getenv("GD_CHAIN_YLDS");
-    s->chain_ylds = yl ? atoi(yl) : 0;
+    s->chain_ylds = yl ? atoi(yl) : -1;
     return GD_OK;
 }
 

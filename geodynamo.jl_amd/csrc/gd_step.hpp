@@ -49,7 +49,7 @@ struct gd_step {
     // form large shells use, the same bits) or as sized (0)
     int chain_off64 = 0;
     int chain_pair = 2; // twisted solve: re and im columns in one lane (prefetch depth 2)
-    int chain_ylds = 0; // paired solve: LDS rows of y (0: none, -1: as many as fit, k: at most k)
+    int chain_ylds = -1; // paired solve: LDS rows of y (0: none, -1: as many as fit, k: at most k)
     // twisted column solve (gd_chain_twist_tables): tables for the four
     // diffusivity slots; twist = 0 keeps one lane per column
     DevBuf ULt, rdiag2, Rint;

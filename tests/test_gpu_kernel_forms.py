@@ -15,8 +15,8 @@ composition, a small shell and lmax=127, nr=65):
     (idx, field) in one lane (the default) against one column per lane
     (GD_CHAIN_PAIR=0);
   * same bits: the paired twisted solve with the first rows of each half's
-    forward results in LDS (GD_CHAIN_YLDS=-1) against all of them through
-    the state buffer (the default): the same operations, another store;
+    forward results in LDS (the default) against all of them through the
+    state buffer (GD_CHAIN_YLDS=0): the same operations, another store;
   * same bits: the one-lane CNAB2 column kernel with 64-bit offsets
     (GD_CHAIN_OFF64=1, the form shells with SYNV_COLS*nlm*nr >= 2^32 run)
     against the 32-bit offsets used where they fit, both with
@@ -30,7 +30,7 @@ pytestmark = pytest.mark.gpu
 
 
 ALT = {"GD_SYNTH_GRAD": "0", "GD_CHAIN_TWIST": "0", "GD_CHAIN_OFF64": "1", "GD_CHAIN_PAIR": "0",
-       "GD_CHAIN_YLDS": "-1"}  # non-default forms
+       "GD_CHAIN_YLDS": "0"}  # non-default forms
 BASE = {"GD_CHAIN_OFF64": {"GD_CHAIN_TWIST": "0"}}  # set for both sides of a pair
 BITWISE = ("GD_SYNTH_GRAD", "GD_CHAIN_YLDS", "GD_CHAIN_OFF64")
 
