@@ -703,7 +703,8 @@ int gd_launch_cnab2_chain(hipStream_t st, const ChainArgs &c, bool syn_fp64)
 {
     GD_CHECK(c.st_in != c.st_out, GD_ERR_ARG, "cnab2_chain: in and out state must differ");
     GD_CHECK(c.rdiag, GD_ERR_ARG, "cnab2_chain: reciprocal diagonal table missing");
-    GD_CHECK(!c.nl_zero_mask || c.zero_row, GD_ERR_ARG, "cnab2_chain: nl_zero_mask needs zero_row");
+    // the kernels' prologues read every value a condition zeroes from zero_row
+    GD_CHECK(c.zero_row, GD_ERR_ARG, "cnab2_chain: zero_row (nlm zeros) missing");
     GD_CHECK(!c.synV || (c.synS && c.D1 && c.D2 && c.rtab), GD_ERR_ARG, "cnab2_chain: incomplete synthesis outputs");
     GD_CHECK(gd_cnab2_chain_supported(c.n, c.bw), GD_ERR_ARG, "cnab2_chain: n=%d bw=%d unsupported", c.n, c.bw);
     const int nfield = c.nf ? c.nf : (c.c_in ? 6 : 5);
