@@ -702,11 +702,14 @@ __device__ __forceinline__ void ana_res_body(const AT &a, int blk, int nblk)
     auto node = [&](int ks) { return (ks >> 2) * 16 + 4 * kq + (ks & 3); };
     const size_t nstride = (size_t)a.nr * a.nslot_g; // one node further in GB
     const T *gm = gb + (size_t)m * a.nlat2 * nstride;
+    // Unconditional at clamped indices (a guarded load becomes a branch, and
+    // after branches that may skip loads the compiler waits for all of them
+    // at the next use): a node past the chunk meets table rows that are zero
+    // in LDS, and a level past nr (or a tile past the last) belongs to an
+    // output column that is never stored.
     auto gload = [&](int tile, int t, int ks) {
-        const int r = tile * LT + lv, k = node(ks);
-        T v = T(0);
-        if (tile < ntiles && r < a.nr && k < nk) v = gm[(size_t)(k0 + k) * nstride + (size_t)r * a.nslot_g + slot[t]];
-        return v;
+        const int r = min(tile * LT + lv, a.nr - 1), k = min(node(ks), nk - 1);
+        return gm[(size_t)(k0 + k) * nstride + (size_t)r * a.nslot_g + slot[t]];
     };
     T b[NT][KS];
 #pragma unroll
