@@ -72,11 +72,21 @@ template <int BW, int D> __global__ void __launch_bounds__(64) cnab2_adj_kernel(
     extern __shared__ double smem[];
     double *Lc = smem, *r2 = smem + W * n;
     const double *Lap = c.nuLap + (size_t)k * W * n;
-    for (int t = threadIdx.x; t < W * n; t += 64) Lc[t] = Lap[t];
+    // the column's degree and the table fill are loaded together (UF entries
+    // per thread and round, loads before the LDS stores; past the end a
+    // thread stores the last entry again), not one wait per entry
+    const int l = c.l_of_idx[idx < nlm ? idx : nlm - 1];
+    constexpr int UF = 4;
+    for (int t0 = threadIdx.x; t0 < W * n; t0 += UF * 64) {
+        double v[UF];
+#pragma unroll
+        for (int e = 0; e < UF; ++e) v[e] = Lap[min(t0 + e * 64, W * n - 1)];
+#pragma unroll
+        for (int e = 0; e < UF; ++e) Lc[min(t0 + e * 64, W * n - 1)] = v[e];
+    }
     for (int t = threadIdx.x; t < n; t += 64) r2[t] = c.rinv2[t];
     __syncthreads();
     if (idx >= nlm) return;
-    const int l = c.l_of_idx[idx];
     const double nulf = c.nus[k] * (double)l * (double)(l + 1);
     const double *LU = c.LUt + (size_t)k * W * n * c.nl + l;
     const double *rdg = c.rdiag + (size_t)k * n * c.nl + l;
