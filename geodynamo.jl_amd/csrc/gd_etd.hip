@@ -449,18 +449,44 @@ template <int W> __global__ void __launch_bounds__(64 * W) dense_mfma_kernel(Den
         if (P.out[pl]) {
             __syncthreads(); // every wave's reads of this pass are done (in-place outputs)
             // accumulator element v of lane: row (lane / 16) + 4 v, column lane % 16
+            const double *y0 = P.y0[pl], *y1 = P.y1[pl];
+            if (y0 || y1) {
+                // the y terms of all 8 outputs loaded first (at clamped rows and
+                // columns; the pass's choice of terms is uniform), so the
+                // epilogue waits once, not once per guarded load
+                double t0[8], t1[8];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int r = r0 + wave * 16 + (lane >> 4) + 4 * v;
+                for (int v = 0; v < 4; ++v) {
+                    const int r = min(r0 + wave * 16 + (lane >> 4) + 4 * v, n - 1);
 #pragma unroll
-                for (int ct = 0; ct < 2; ++ct) {
-                    const int cc = ct * 16 + (lane & 15);
-                    if (r < n && cc < ncol) {
-                        const size_t o = col0 + cc + nlm * r;
-                        double val = ct ? acc1[v] : acc0[v];
-                        if (P.y0[pl]) val += P.b0 * P.y0[pl][o];
-                        if (P.y1[pl]) val += P.b1 * P.y1[pl][o];
-                        P.out[pl][o] = val;
+                    for (int ct = 0; ct < 2; ++ct) {
+                        const size_t o = col0 + min(ct * 16 + (lane & 15), ncol - 1) + nlm * r;
+                        t0[2 * v + ct] = y0 ? y0[o] : 0.0;
+                        t1[2 * v + ct] = y1 ? y1[o] : 0.0;
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = r0 + wave * 16 + (lane >> 4) + 4 * v;
+#pragma unroll
+                    for (int ct = 0; ct < 2; ++ct) {
+                        const int cc = ct * 16 + (lane & 15);
+                        if (r < n && cc < ncol) {
+                            double val = ct ? acc1[v] : acc0[v];
+                            if (y0) val += P.b0 * t0[2 * v + ct];
+                            if (y1) val += P.b1 * t1[2 * v + ct];
+                            P.out[pl][col0 + cc + nlm * r] = val;
+                        }
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = r0 + wave * 16 + (lane >> 4) + 4 * v;
+#pragma unroll
+                    for (int ct = 0; ct < 2; ++ct) {
+                        const int cc = ct * 16 + (lane & 15);
+                        if (r < n && cc < ncol) P.out[pl][col0 + cc + nlm * r] = ct ? acc1[v] : acc0[v];
                     }
                 }
             }
