@@ -1730,12 +1730,32 @@ __device__ __forceinline__ void ring_io_body(const RingArgs &a, int g, int r)
         }
         return;
     }
-    for (int e = tid; e < n * nq; e += nthr) {
-        const int q = e % nq, j = e / nq, i = i0 + q, is = a.nlat - 1 - i;
-        for (int F = 0; F < NF; ++F) {
-            const double *p = a.phys_in[F == 0 ? 0 : 1];
-            buf[(q * NF + F) * ld + sk(j)] =
-                C2<T>{(T)p[((size_t)r * n + j) * a.nlat + i], (T)p[((size_t)r * n + j) * a.nlat + is]};
+    // the rings' points: UR elements per thread and round, all loads issued
+    // (clamped to the last element) before any LDS store and conversion, so
+    // a round waits once, not once per element
+    constexpr int UR = 4;
+    for (int e0 = tid; e0 < n * nq; e0 += UR * nthr) {
+        double px[UR][2], py[UR][2];
+#pragma unroll
+        for (int u = 0; u < UR; ++u) {
+            const int e = min(e0 + u * nthr, n * nq - 1);
+            const int q = e % nq, j = e / nq, i = i0 + q, is = a.nlat - 1 - i;
+            const size_t o = ((size_t)r * n + j) * a.nlat;
+            px[u][0] = a.phys_in[0][o + i];
+            py[u][0] = a.phys_in[0][o + is];
+            if (NF > 1) {
+                px[u][1] = a.phys_in[1][o + i];
+                py[u][1] = a.phys_in[1][o + is];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UR; ++u) {
+            const int e = e0 + u * nthr;
+            if (e < n * nq) {
+                const int q = e % nq, j = e / nq;
+                buf[(q * NF) * ld + sk(j)] = C2<T>{(T)px[u][0], (T)py[u][0]};
+                if (NF > 1) buf[(q * NF + 1) * ld + sk(j)] = C2<T>{(T)px[u][1], (T)py[u][1]};
+            }
         }
     }
     __syncthreads();
@@ -1865,6 +1885,8 @@ template <typename T> int gd_launch_ring(hipStream_t st, int mode, const RingArg
     int nf;
     GD_TRY(ring_check(mode, a, &nf));
     if (mode == RING_SYNTH || mode == RING_ANALYSIS) {
+        // the boundary passes take one scalar or one (theta, phi) pair: phys_in / phys_out [0], [1]
+        GD_CHECK(nf <= 2, GD_ERR_ARG, "ring: %d fields in a boundary pass", nf);
         const int rg = ring_io_rg<T>(a, nf);
         if (rg > 1) {
             const size_t lds = sizeof(C2<T>) * ((size_t)ring_ld(a.nlon) * nf * rg + a.nlon);
